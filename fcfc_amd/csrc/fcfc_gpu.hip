@@ -1,0 +1,1447 @@
+/*
+ * fcfc_gpu.hip -- gfx950 pair-count engine for FCFC: grid build, tiling,
+ * the persistent counting kernels (FP64, FP32 + FP64 edge recheck, weight
+ * backward) and the C ABI declared in include/fcfc_gpu.h.
+ *
+ * Build (see fcfc_amd/build.py):
+ *   hipcc --offload-arch=gfx950 -O3 -ffp-contract=off -fPIC -shared ...
+ * -ffp-contract=off matters: the FP64 pair arithmetic must round exactly as
+ * the reference's (every fma below is explicit).
+ */
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "fcfc_gpu.h"
+#include "fcfc_kernels.h"
+
+#define MODE_C64  0   /* FP64 counts                                    */
+#define MODE_C32  1   /* FP32 distances + FP64 recheck near bin edges   */
+#define MODE_GRAD 2   /* weight gradient (FP64 bins)                    */
+
+/* ======================================================================== */
+/*                          grid build kernels                              */
+/* ======================================================================== */
+
+__device__ __forceinline__ int fk_cell1(double v, double o, double w, int nc) {
+  int q = (int) floor((v - o) / w);
+  return q < 0 ? 0 : (q >= nc ? nc - 1 : q);
+}
+
+__global__ void k_fill_int(int *p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+__global__ void k_cell_assign(FkGrid g, int n, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z,
+    int *__restrict__ cid, int *__restrict__ rank, int *__restrict__ count) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int ix = fk_cell1(x[i], g.ox, g.wx, g.ncx);
+    int iy = fk_cell1(y[i], g.oy, g.wy, g.ncy);
+    int iz = fk_cell1(z[i], g.oz, g.wz, g.ncz);
+    int c = (ix * g.ncy + iy) * g.ncz + iz;
+    cid[i] = c;
+    rank[i] = atomicAdd(count + c, 1);
+  }
+}
+
+/* Exclusive scan of n ints into out[0..n] (out[n] = total), one workgroup
+ * of 1024 threads, each scanning a contiguous slice. */
+__global__ void __launch_bounds__(1024) k_scan1(const int *__restrict__ in,
+    int *__restrict__ out, int n) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; i++) s += in[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int i = b; i < e; i++) { int v = in[i]; out[i] = run; run += v; }
+  if (t == 1023) out[n] = part[1023];
+}
+
+__global__ void k_scatter(int n, const int *__restrict__ cid,
+    const int *__restrict__ rank, const int *__restrict__ cstart,
+    const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ s,
+    const double *__restrict__ w, double *__restrict__ ox,
+    double *__restrict__ oy, double *__restrict__ oz,
+    double *__restrict__ os, double *__restrict__ ow, int *__restrict__ oidx) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int d = cstart[cid[i]] + rank[i];
+    ox[d] = x[i]; oy[d] = y[i]; oz[d] = z[i];
+    if (s) os[d] = s[i];
+    if (w) ow[d] = w[i];
+    oidx[d] = i;
+  }
+}
+
+/* |r|^2 with FCFC's data_preprocess lane rule (2pt/build_tree.c:59-92):
+ * FMA for the first floor(n/8)*8 objects, unfused for the tail. */
+__global__ void k_survey_norm(int n, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z,
+    double *__restrict__ s) {
+  const int nv = (n / 8) * 8;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double a = x[i], b = y[i], c = z[i];
+    if (i < nv) s[i] = __builtin_fma(c, c, __builtin_fma(b, b, a * a));
+    else { double t = a * a; t = t + b * b; s[i] = t + c * c; }
+  }
+}
+
+/* Tiles of one row: consecutive cells, at most T objects, at most zmaxc
+ * z-cells; a cell with more than T objects is split into T-sized tiles. */
+__device__ int row_tiles(const int *__restrict__ cs, int ncz, int T, int zmaxc,
+    int row, FkTile *out) {
+  int nt = 0, open = 0, tp0 = 0, tcnt = 0, tz0 = 0, tz1 = 0;
+  for (int iz = 0; iz < ncz; iz++) {
+    const int c = cs[iz + 1] - cs[iz];
+    if (c == 0) continue;
+    if (open && (tcnt + c > T || iz - tz0 + 1 > zmaxc)) {
+      if (out) out[nt] = FkTile{row, tp0, tp0 + tcnt, tz0, tz1};
+      nt++; open = 0;
+    }
+    if (c > T) {
+      for (int k = 0; k < c; k += T) {
+        if (out) out[nt] = FkTile{row, cs[iz] + k, cs[iz] + min(k + T, c), iz, iz};
+        nt++;
+      }
+      continue;
+    }
+    if (!open) { open = 1; tp0 = cs[iz]; tcnt = 0; tz0 = iz; }
+    tcnt += c; tz1 = iz;
+  }
+  if (open) { if (out) out[nt] = FkTile{row, tp0, tp0 + tcnt, tz0, tz1}; nt++; }
+  return nt;
+}
+
+__global__ void k_tile_count(int nrow, int ncz, const int *__restrict__ cstart,
+    int T, int zmaxc, int *__restrict__ ntile) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
+    ntile[r] = row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, nullptr);
+}
+__global__ void k_tile_emit(int nrow, int ncz, const int *__restrict__ cstart,
+    int T, int zmaxc, const int *__restrict__ toff, FkTile *__restrict__ tiles) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
+    row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, tiles + toff[r]);
+}
+
+/* ======================================================================== */
+/*                             count kernel                                 */
+/* ======================================================================== */
+
+struct FkArgs {
+  FkCat A, B;
+  FkGrid g;
+  FkBins bins;
+  const FkTile *tiles;
+  int ntiles;
+  const int *strips;          /* (dx, dy, diag) triples */
+  int nstrips;
+  int autoc;                  /* COUNT: A == B, half shell + self tile;
+                                 GRAD: A == B, exclude j == i          */
+  unsigned int *queue;
+  unsigned long long *gcnt_u;
+  double *gcnt_d;
+  unsigned long long *stats;  /* evals, rechecked, census */
+  int nh;                     /* LDS histogram copies */
+  /* LDS byte offsets */
+  int off_hist, off_stab, off_ptab, off_mutab, off_edges, off_strip, off_chunk;
+  /* FP32 error model (MODE_C32) */
+  float e_s2;                 /* absolute window on s^2 (or s_perp^2)       */
+  float e_pi;                 /* absolute window on pi                      */
+  float e_rrel, e_rabs;       /* mu window: dr = r*e_rrel + inv_s*(r*e_rabs + c_rz) */
+  float e_rz;
+  float s2_tiny;              /* force recheck below this s^2                */
+  int census;
+  /* GRAD */
+  const double *G;
+  double *gA;
+};
+
+/* LDS strip table */
+struct FkStripLds {
+  int off[FK_NSB + 1];
+  int b0[FK_NSB * FK_MAXSEG];
+  int len[FK_NSB * FK_MAXSEG];
+  int code[FK_NSB * FK_MAXSEG];
+};
+
+/* Compute the (up to three) B segments of one strip for tile t. */
+__device__ void fk_strip_segments(const FkArgs &a, const FkTile &t, int dx,
+    int dy, int diag, int *b0, int *len, int *code) {
+  const FkGrid &g = a.g;
+  b0[0] = b0[1] = b0[2] = 0;
+  len[0] = len[1] = len[2] = 0;
+  code[0] = code[1] = code[2] = 13;
+  const int ix = t.row / g.ncy, iy = t.row - (t.row / g.ncy) * g.ncy;
+  int bx = ix + dx, by = iy + dy, cx = 0, cy = 0;
+  if (g.periodic) {
+    if (bx < 0) { bx += g.ncx; cx = -1; } else if (bx >= g.ncx) { bx -= g.ncx; cx = 1; }
+    if (by < 0) { by += g.ncy; cy = -1; } else if (by >= g.ncy) { by -= g.ncy; cy = 1; }
+  }
+  else if (bx < 0 || bx >= g.ncx || by < 0 || by >= g.ncy) return;
+  const double gx = (double) max(abs(dx) - 1, 0) * g.wx;
+  const double gy = (double) max(abs(dy) - 1, 0) * g.wy;
+  const double d2 = gx * gx + gy * gy;
+  if (d2 >= g.Rxy2) return;
+  const double hz = (g.hz_const >= 0) ? g.hz_const : sqrt(fmax(g.R2 - d2, 0.0));
+  const double zc0 = (double) t.iz0 - hz / g.wz - 1e-6;
+  const double zc1 = (double) (t.iz1 + 1) + hz / g.wz + 1e-6;
+  int izlo = (int) floor(zc0), izhi = (int) floor(zc1);
+  const int *cs = a.B.cell_start + (size_t) (bx * g.ncy + by) * g.ncz;
+  const int cxy = (cx + 1) + 3 * (cy + 1);
+  int ns = 0;
+  if (g.periodic) {
+    if (izhi - izlo + 1 > g.ncz) {   /* cannot happen with the tile limits */
+      izlo = 0; izhi = g.ncz - 1;
+    }
+    if (izlo < 0) {                  /* image below z = 0: B shifted by -Lz */
+      int c0 = izlo + g.ncz, c1 = g.ncz - 1;
+      b0[ns] = cs[c0]; len[ns] = cs[c1 + 1] - cs[c0]; code[ns] = cxy + 9 * 0; ns++;
+    }
+    {
+      int c0 = max(izlo, 0), c1 = min(izhi, g.ncz - 1);
+      if (diag) {
+        /* auto counts, same row: only objects after the tile (j > i) */
+        int e = cs[c1 + 1];
+        b0[ns] = t.p1; len[ns] = max(e - t.p1, 0); code[ns] = cxy + 9 * 1; ns++;
+      }
+      else if (c1 >= c0) { b0[ns] = cs[c0]; len[ns] = cs[c1 + 1] - cs[c0]; code[ns] = cxy + 9 * 1; ns++; }
+    }
+    if (izhi >= g.ncz && !diag) {    /* image above: B shifted by +Lz */
+      int c0 = 0, c1 = izhi - g.ncz;
+      b0[ns] = cs[c0]; len[ns] = cs[c1 + 1] - cs[c0]; code[ns] = cxy + 9 * 2; ns++;
+    }
+  }
+  else {
+    int c0 = max(izlo, 0), c1 = min(izhi, g.ncz - 1);
+    if (diag) {
+      int e = (c1 >= 0) ? cs[c1 + 1] : t.p1;
+      b0[0] = t.p1; len[0] = max(e - t.p1, 0); code[0] = 13;
+    }
+    else if (c1 >= c0) { b0[0] = cs[c0]; len[0] = cs[c1 + 1] - cs[c0]; code[0] = 13; }
+  }
+}
+
+/* ---------------- FP64 pair evaluation (the arithmetic contract) -------- */
+
+template <int BIN>
+__device__ __forceinline__ int fk_sbin(double s2, const FkBins &bn,
+    const uint16_t *stab, const double *s2e) {
+  int k = (int) s2 - bn.soff + 1;            /* (int) truncates like cvtt */
+  k = k < 0 ? 0 : (k > bn.stab_len + 1 ? bn.stab_len + 1 : k);
+  int sb = stab[k];
+  if (bn.tabtype && sb >= bn.ns && sb != (int) FK_OUT) {   /* hybrid refine */
+    sb -= bn.ns;
+    while (sb != 0 && s2 < s2e[sb]) sb--;
+  }
+  return sb;
+}
+
+__device__ __forceinline__ int fk_pbin(double p, const FkBins &bn,
+    const uint16_t *ptab, const double *pe) {
+  int k = (int) p - bn.poff + 1;
+  k = k < 0 ? 0 : (k > bn.ptab_len + 1 ? bn.ptab_len + 1 : k);
+  int pb = ptab[k];
+  if (bn.tabtype && pb >= bn.np && pb != (int) FK_OUT) {
+    pb -= bn.np;
+    while (pb != 0 && p < pe[pb]) pb--;
+  }
+  return pb;
+}
+
+/* Box pair, B already shifted: returns the histogram index or -1. */
+template <int BIN>
+__device__ __forceinline__ int fk_pair_box64(double ax, double ay, double az,
+    double bx, double by, double bz, const FkBins &bn, const uint16_t *stab,
+    const uint16_t *ptab, const uint8_t *mutab, const double *s2e,
+    const double *pe) {
+  const double dx = bx - ax, dy = by - ay, dz = bz - az;
+  if (BIN == FCFC_BIN_SPI) {
+    const double pi = fabs(dz);
+    const double s2 = __builtin_fma(dy, dy, dx * dx);
+    if (!(s2 < bn.s2max) || s2 < bn.s2min || !(pi < bn.pmax) || pi < bn.pmin) return -1;
+    return fk_sbin<BIN>(s2, bn, stab, s2e) + bn.ns * fk_pbin(pi, bn, ptab, pe);
+  }
+  const double dz2 = dz * dz;
+  const double s2 = __builtin_fma(dy, dy, __builtin_fma(dx, dx, dz2));
+  if (!(s2 < bn.s2max) || s2 < bn.s2min) return -1;
+  const int sb = fk_sbin<BIN>(s2, bn, stab, s2e);
+  if (BIN == FCFC_BIN_ISO) return sb;
+  int m = 0;
+  if (s2 >= 2.220446049250313e-16) {
+    double mq = fk_floor_div(dz2 * bn.nmu2, s2);
+    if (mq >= bn.nmu2) return -1;            /* mu = 1 excluded */
+    m = (int) mq;
+  }
+  const int mb = bn.mutab_lds ? (int) mutab[m] : fk_isqrt(m);
+  return sb + bn.ns * mb;
+}
+
+/* Survey pair (src/fcfc/2pt/metric_common.c:283-460, FMA vector formulas). */
+template <int BIN>
+__device__ __forceinline__ int fk_pair_sv64(double ax, double ay, double az,
+    double as, double bx, double by, double bz, double bs, const FkBins &bn,
+    const uint16_t *stab, const uint16_t *ptab, const uint8_t *mutab,
+    const double *s2e, const double *pe) {
+  if (BIN == FCFC_BIN_ISO) {
+    const double dx = ax - bx, dy = ay - by, dz = az - bz;
+    const double s2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    if (!(s2 < bn.s2max) || s2 < bn.s2min) return -1;
+    return fk_sbin<BIN>(s2, bn, stab, s2e);
+  }
+  const double t = __builtin_fma(az, bz, __builtin_fma(ay, by, ax * bx)) * 2.0;
+  const double s = as + bs;
+  const double d = as - bs;
+  const double p2 = (d * d) / (t + s);
+  if (BIN == FCFC_BIN_SMU) {
+    const double s2 = s - t;
+    if (!(s2 < bn.s2max) || s2 < bn.s2min) return -1;
+    const int sb = fk_sbin<BIN>(s2, bn, stab, s2e);
+    int m = 0;
+    if (s2 >= 2.220446049250313e-16) {
+      double mq = fk_floor_div(p2 * bn.nmu2, s2);
+      if (mq >= bn.nmu2) return -1;
+      m = (int) mq;
+    }
+    const int mb = bn.mutab_lds ? (int) mutab[m] : fk_isqrt(m);
+    return sb + bn.ns * mb;
+  }
+  /* (s_perp, pi): pbin holds pi^2 edges */
+  if (!(p2 < bn.pmax) || p2 < bn.pmin) return -1;
+  const double sp2 = (s - t) - p2;
+  if (!(sp2 < bn.s2max)) return -1;
+  if (bn.s2min != 0 && sp2 < bn.s2min) return -1;
+  return fk_sbin<BIN>(sp2, bn, stab, s2e) + bn.ns * fk_pbin(p2, bn, ptab, pe);
+}
+
+/* The other arithmetic paths of the reference, for the rounding census
+ * (box only): vectorised a-side ((a+S)-b), per-pair periodic wrap, and the
+ * unfused scalar remainder path.  Returns histogram index or -1. */
+template <int BIN>
+__device__ int fk_pair_box_variant(int var, const double a[3], const double bo[3],
+    const int dcode[3], const FkGrid &g, const FkBins &bn, const uint16_t *stab,
+    const uint16_t *ptab, const double *s2e, const double *pe) {
+  const double L[3] = {g.Lx, g.Ly, g.Lz};
+  double d[3];
+  for (int k = 0; k < 3; k++) {
+    const double D = dcode[k] * L[k];        /* b image = b + D, S = -D */
+    if (var == 0 || var == 3) d[k] = (a[k] - D) - bo[k];          /* (a + S) - b */
+    else {                                                        /* wrap */
+      double v = a[k] - bo[k];
+      const double h = L[k] * 0.5;
+      if (v > h) v = v - L[k]; else if (v < -h) v = v + L[k];
+      d[k] = v;
+    }
+  }
+  const int scalar = (var == 3);
+  double s2, dz2 = 0, pi = 0;
+  if (BIN == FCFC_BIN_SPI) {
+    pi = fabs(d[2]);
+    s2 = scalar ? (d[0] * d[0] + d[1] * d[1]) : __builtin_fma(d[1], d[1], d[0] * d[0]);
+    if (!(s2 < bn.s2max) || s2 < bn.s2min || !(pi < bn.pmax) || pi < bn.pmin) return -1;
+    return fk_sbin<BIN>(s2, bn, stab, s2e) + bn.ns * fk_pbin(pi, bn, ptab, pe);
+  }
+  dz2 = d[2] * d[2];
+  if (scalar) { double t = d[0] * d[0]; t = t + d[1] * d[1]; s2 = t + dz2; }
+  else s2 = __builtin_fma(d[1], d[1], __builtin_fma(d[0], d[0], dz2));
+  if (!(s2 < bn.s2max) || s2 < bn.s2min) return -1;
+  const int sb = fk_sbin<BIN>(s2, bn, stab, s2e);
+  if (BIN == FCFC_BIN_ISO) return sb;
+  int m = 0;
+  if (s2 >= 2.220446049250313e-16) {
+    if (scalar) {
+      double q = (dz2 / s2) * bn.nmu2;
+      if (q >= bn.nmu2) return -1;
+      m = (int) q;
+    }
+    else {
+      double mq = fk_floor_div(dz2 * bn.nmu2, s2);
+      if (mq >= bn.nmu2) return -1;
+      m = (int) mq;
+    }
+  }
+  return sb + bn.ns * fk_isqrt(m);
+}
+
+/* ---------------------------- the kernel --------------------------------- */
+
+template <int BIN, int GEOM, bool WT, int MODE, int R>
+__global__ void __launch_bounds__(FK_NT)
+k_count(const FkArgs a) {
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  constexpr bool F32 = (MODE == MODE_C32);
+  constexpr bool NEED_S = (GEOM == FCFC_GEOM_SURVEY && BIN != FCFC_BIN_ISO);
+  constexpr bool NEED_W = WT || (MODE == MODE_GRAD);
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const FkBins &bn = a.bins;
+
+  /* ---- LDS carve-up ---- */
+  unsigned int *hist_u = (unsigned int *) (lds + a.off_hist);
+  double *hist_d = (double *) (lds + a.off_hist);
+  uint16_t *stab = (uint16_t *) (lds + a.off_stab);
+  uint16_t *ptab = (uint16_t *) (lds + a.off_ptab);
+  uint8_t *mutab = (uint8_t *) (lds + a.off_mutab);
+  double *s2e = (double *) (lds + a.off_edges);
+  double *pe = s2e + (bn.ns + 1);
+  FkStripLds *st = (FkStripLds *) (lds + a.off_strip);
+  char *chunk = lds + a.off_chunk;
+  /* chunk arrays */
+  double *cx = (double *) chunk, *cy = cx + FK_CH, *cz = cy + FK_CH;
+  float *fx = (float *) chunk, *fy = fx + FK_CH, *fz = fy + FK_CH;
+  char *after = F32 ? (char *) (fz + FK_CH) : (char *) (cz + FK_CH);
+  double *cs_ = (double *) after;                     /* survey |r|^2 */
+  double *cw = NEED_S ? cs_ + FK_CH : cs_;            /* weights */
+  int *cj = (int *) (NEED_W ? (char *) (cw + FK_CH) : (char *) cw);  /* sorted index */
+  int *cc = cj + FK_CH;                               /* shift code */
+  __shared__ int s_tile;
+
+  /* ---- stage tables ---- */
+  const int hist_elems = a.nh * bn.ntot;
+  if (MODE == MODE_GRAD) {
+    for (int i = tid; i < bn.ntot; i += FK_NT) hist_d[i] = a.G[i];
+  }
+  else if (WT) { for (int i = tid; i < hist_elems; i += FK_NT) hist_d[i] = 0.0; }
+  else { for (int i = tid; i < hist_elems; i += FK_NT) hist_u[i] = 0u; }
+  for (int i = tid; i < bn.stab_len + 2; i += FK_NT)
+    stab[i] = (i == 0 || i == bn.stab_len + 1) ? (uint16_t) FK_OUT : bn.stab[i - 1];
+  if (BIN == FCFC_BIN_SPI)
+    for (int i = tid; i < bn.ptab_len + 2; i += FK_NT)
+      ptab[i] = (i == 0 || i == bn.ptab_len + 1) ? (uint16_t) FK_OUT : bn.ptab[i - 1];
+  if (BIN == FCFC_BIN_SMU && bn.mutab_lds) {
+    const int nm2 = bn.nmu * bn.nmu;
+    for (int i = tid; i < nm2; i += FK_NT) mutab[i] = (uint8_t) fk_isqrt(i);
+  }
+  if (bn.tabtype) {
+    for (int i = tid; i <= bn.ns; i += FK_NT) s2e[i] = bn.s2bin[i];
+    if (BIN == FCFC_BIN_SPI) for (int i = tid; i <= bn.np; i += FK_NT) pe[i] = bn.pbin[i];
+  }
+
+  double evals = 0, nrecheck = 0, ncensus = 0;
+  double since_flush = 0;
+  unsigned int *myhist_u = hist_u + (wave % a.nh) * bn.ntot;
+  double *myhist_d = hist_d + (wave % a.nh) * bn.ntot;
+
+  auto flush = [&]() {
+    __syncthreads();
+    for (int i = tid; i < bn.ntot; i += FK_NT) {
+      if (WT) {
+        double v = 0;
+        for (int c = 0; c < a.nh; c++) { v += hist_d[c * bn.ntot + i]; hist_d[c * bn.ntot + i] = 0.0; }
+        if (v != 0) unsafeAtomicAdd(a.gcnt_d + i, v);
+      }
+      else {
+        unsigned long long v = 0;
+        for (int c = 0; c < a.nh; c++) { v += hist_u[c * bn.ntot + i]; hist_u[c * bn.ntot + i] = 0u; }
+        if (v) atomicAdd(a.gcnt_u + i, v);
+      }
+    }
+    __syncthreads();
+  };
+
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) s_tile = (int) atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const int tix = s_tile;
+    if (tix >= a.ntiles) break;
+    const FkTile t = a.tiles[tix];
+    const int ix = t.row / a.g.ncy, iy = t.row - ix * a.g.ncy;
+
+    /* A objects of this lane */
+    double ax[R], ay[R], az[R], as[R], aw[R], gacc[R];
+    float fax[R], fay[R], faz[R];
+    int ai[R];
+    bool av[R];
+    /* tile centre for the FP32 relative coordinates */
+    const double tcx = a.g.ox + (ix + 0.5) * a.g.wx;
+    const double tcy = a.g.oy + (iy + 0.5) * a.g.wy;
+    const double tcz = a.g.oz + 0.5 * (t.iz0 + t.iz1 + 1) * a.g.wz;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int p = t.p0 + tid + r * FK_NT;
+      av[r] = p < t.p1;
+      const int q = av[r] ? p : t.p0;
+      ai[r] = p;
+      ax[r] = a.A.x[q]; ay[r] = a.A.y[q]; az[r] = a.A.z[q];
+      as[r] = NEED_S ? a.A.s[q] : 0.0;
+      aw[r] = WT ? a.A.w[q] : 1.0;
+      gacc[r] = 0.0;
+      fax[r] = (float) (ax[r] - tcx); fay[r] = (float) (ay[r] - tcy); faz[r] = (float) (az[r] - tcz);
+    }
+    const int nva = min(t.p1 - t.p0, R * FK_NT);
+
+    /* process one staged chunk of `cnt` B objects */
+    auto run_chunk = [&](int cnt, bool self) {
+      for (int j = 0; j < cnt; j++) {
+        double bx = 0, by = 0, bz = 0, bs = 0, bw = 1;
+        float gx = 0, gy = 0, gz = 0;
+        if (F32) { gx = fx[j]; gy = fy[j]; gz = fz[j]; }
+        else { bx = cx[j]; by = cy[j]; bz = cz[j]; }
+        if (NEED_S) bs = cs_[j];
+        if (NEED_W) bw = cw[j];
+        int jg = 0;
+        if (self || MODE == MODE_GRAD || F32) jg = cj[j];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (!av[r]) continue;
+          if (self && !(jg > ai[r])) continue;
+          if (MODE == MODE_GRAD && a.autoc && jg == ai[r]) continue;
+          int idx;
+          if (!F32) {
+            if (GEOM == FCFC_GEOM_BOX)
+              idx = fk_pair_box64<BIN>(ax[r], ay[r], az[r], bx, by, bz, bn, stab, ptab, mutab, s2e, pe);
+            else
+              idx = fk_pair_sv64<BIN>(ax[r], ay[r], az[r], as[r], bx, by, bz, bs, bn, stab, ptab, mutab, s2e, pe);
+          }
+          else {
+            /* ---------- FP32 fast path with exactness windows ---------- */
+            const float dx = gx - fax[r], dy = gy - fay[r], dz = gz - faz[r];
+            bool amb = false;
+            int sb, mb = 0;
+            float s2;
+            if (BIN == FCFC_BIN_SPI) s2 = __builtin_fmaf(dy, dy, dx * dx);
+            else { const float dz2 = dz * dz; s2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, dz2)); }
+            if (!(s2 < (float) bn.s2max + a.e_s2)) continue;          /* certainly out */
+            {
+              const int k = (int) s2;
+              const float f = s2 - (float) k;
+              int e = k - bn.soff + 1;
+              e = e < 0 ? 0 : (e > bn.stab_len + 1 ? bn.stab_len + 1 : e);
+              sb = stab[e];
+              if (f < a.e_s2) { int e2 = e > 0 ? e - 1 : 0; if (stab[e2] != sb) amb = true; }
+              if (f > 1.0f - a.e_s2) { int e2 = e < bn.stab_len + 1 ? e + 1 : e; if (stab[e2] != sb) amb = true; }
+              if (sb >= bn.ns && sb != (int) FK_OUT) amb = true;       /* hybrid entry */
+              if (!amb && sb == (int) FK_OUT) continue;               /* certainly out */
+            }
+            if (BIN == FCFC_BIN_SPI) {
+              const float pi = fabsf(dz);
+              const int k = (int) pi;
+              const float f = pi - (float) k;
+              int e = k - bn.poff + 1;
+              e = e < 0 ? 0 : (e > bn.ptab_len + 1 ? bn.ptab_len + 1 : e);
+              mb = ptab[e];
+              if (f < a.e_pi) { int e2 = e > 0 ? e - 1 : 0; if (ptab[e2] != mb) amb = true; }
+              if (f > 1.0f - a.e_pi) { int e2 = e < bn.ptab_len + 1 ? e + 1 : e; if (ptab[e2] != mb) amb = true; }
+              if (mb >= bn.np && mb != (int) FK_OUT) amb = true;
+              if (!amb && mb == (int) FK_OUT) continue;
+            }
+            if (BIN == FCFC_BIN_SMU) {
+              if (s2 < a.s2_tiny) amb = true;
+              const float inv_s = __builtin_amdgcn_rsqf(s2);
+              const float rr = (float) bn.nmu * fabsf(dz) * inv_s;
+              const int n = (int) rr;
+              const float fr = rr - (float) n;
+              const float dr = __builtin_fmaf(rr, a.e_rrel, inv_s * __builtin_fmaf(rr, a.e_rabs, a.e_rz));
+              if (!(fr >= dr && fr <= 1.0f - dr)) amb = true;
+              if (!amb && n >= bn.nmu) continue;                       /* mu = 1 */
+              mb = n;
+            }
+            if (!amb) idx = (BIN == FCFC_BIN_ISO) ? sb : sb + bn.ns * mb;
+            else {
+              /* exact FP64 re-evaluation of this pair */
+              nrecheck += 1;
+              const int code = cc[j];
+              const int c3[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+              const double bo[3] = {a.B.x[jg], a.B.y[jg], a.B.z[jg]};
+              const double ex = bo[0] + c3[0] * a.g.Lx;
+              const double ey = bo[1] + c3[1] * a.g.Ly;
+              const double ez = bo[2] + c3[2] * a.g.Lz;
+              idx = fk_pair_box64<BIN>(ax[r], ay[r], az[r], ex, ey, ez, bn, stab, ptab, mutab, s2e, pe);
+              if (a.census) {
+                const double av3[3] = {ax[r], ay[r], az[r]};
+                /* a-side shift (0), per-pair wrap (2), unfused scalar (3) */
+                const int iv0 = fk_pair_box_variant<BIN>(0, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
+                const int iv2 = fk_pair_box_variant<BIN>(2, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
+                const int iv3 = fk_pair_box_variant<BIN>(3, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
+                const bool dif = (iv0 != idx) || (iv2 != idx) || (iv3 != idx);
+                if (dif) ncensus += 1;
+              }
+            }
+          }
+          if (idx < 0) continue;
+          if (MODE == MODE_GRAD) gacc[r] += hist_d[idx] * bw;
+          else if (WT) atomicAdd(myhist_d + idx, aw[r] * bw);
+          else atomicAdd(myhist_u + idx, 1u);
+        }
+      }
+    };
+
+    auto guard = [&](int cnt) {
+      if (MODE == MODE_GRAD || WT) return;
+      since_flush += (double) cnt * nva;
+      if (since_flush > 2.0e9) { flush(); since_flush = 0; }
+    };
+
+    /* ---- auto counts: pairs inside the tile (j > i) ---- */
+    if (MODE != MODE_GRAD && a.autoc) {
+      for (int c0 = t.p0; c0 < t.p1; c0 += FK_CH) {
+        const int cnt = min(FK_CH, t.p1 - c0);
+        __syncthreads();
+        for (int q = tid; q < cnt; q += FK_NT) {
+          const int jg = c0 + q;
+          const double x = a.A.x[jg], y = a.A.y[jg], z = a.A.z[jg];
+          if (F32) { fx[q] = (float) (x - tcx); fy[q] = (float) (y - tcy); fz[q] = (float) (z - tcz); cc[q] = 13; }
+          else { cx[q] = x; cy[q] = y; cz[q] = z; }
+          if (NEED_S) cs_[q] = a.A.s[jg];
+          if (NEED_W) cw[q] = a.A.w[jg];
+          cj[q] = jg;
+        }
+        __syncthreads();
+        run_chunk(cnt, true);
+        evals += (double) cnt * nva * 0.5;
+        guard(cnt);
+      }
+    }
+
+    /* ---- strips in batches of one wavefront ---- */
+    for (int sb0 = 0; sb0 < a.nstrips; sb0 += FK_NSB) {
+      __syncthreads();
+      if (wave == 0) {
+        const int l = tid;
+        int tot = 0;
+        int b0[3], len[3], code[3];
+        if (sb0 + l < a.nstrips) {
+          const int *sp = a.strips + 3 * (sb0 + l);
+          fk_strip_segments(a, t, sp[0], sp[1], sp[2], b0, len, code);
+          tot = len[0] + len[1] + len[2];
+        }
+        else { b0[0] = b0[1] = b0[2] = 0; len[0] = len[1] = len[2] = 0; code[0] = code[1] = code[2] = 13; }
+        for (int k = 0; k < 3; k++) { st->b0[3 * l + k] = b0[k]; st->len[3 * l + k] = len[k]; st->code[3 * l + k] = code[k]; }
+        /* inclusive wave scan */
+        int inc = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+          int v = __shfl_up(inc, o, 64);
+          if (l >= o) inc += v;
+        }
+        st->off[l] = inc - tot;
+        if (l == 63) st->off[64] = inc;
+      }
+      __syncthreads();
+      const int total = st->off[FK_NSB];
+      for (int c0 = 0; c0 < total; c0 += FK_CH) {
+        const int cnt = min(FK_CH, total - c0);
+        if (c0 > 0) __syncthreads();
+        for (int q = tid; q < cnt; q += FK_NT) {
+          const int pos = c0 + q;
+          int lo = 0, hi = FK_NSB;          /* off[lo] <= pos < off[hi] */
+          while (hi - lo > 1) { int mid = (lo + hi) >> 1; if (st->off[mid] <= pos) lo = mid; else hi = mid; }
+          int rem = pos - st->off[lo];
+          int sgi = 3 * lo;
+          if (rem >= st->len[sgi]) { rem -= st->len[sgi]; sgi++; if (rem >= st->len[sgi]) { rem -= st->len[sgi]; sgi++; } }
+          const int jg = st->b0[sgi] + rem;
+          const int code = st->code[sgi];
+          double x = a.B.x[jg], y = a.B.y[jg], z = a.B.z[jg];
+          if (GEOM == FCFC_GEOM_BOX && code != 13) {
+            x = x + (double) (code % 3 - 1) * a.g.Lx;
+            y = y + (double) ((code / 3) % 3 - 1) * a.g.Ly;
+            z = z + (double) (code / 9 - 1) * a.g.Lz;
+          }
+          if (F32) { fx[q] = (float) (x - tcx); fy[q] = (float) (y - tcy); fz[q] = (float) (z - tcz); cc[q] = code; }
+          else { cx[q] = x; cy[q] = y; cz[q] = z; }
+          if (NEED_S) cs_[q] = a.B.s[jg];
+          if (NEED_W) cw[q] = a.B.w[jg];
+          cj[q] = jg;
+        }
+        __syncthreads();
+        run_chunk(cnt, false);
+        evals += (double) cnt * nva;
+        guard(cnt);
+      }
+    }
+
+    if (MODE == MODE_GRAD) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (av[r]) a.gA[a.A.idx[ai[r]]] += gacc[r];
+    }
+  }
+
+  if (MODE != MODE_GRAD) flush();
+  if (tid == 0) {
+    atomicAdd(a.stats + 0, (unsigned long long) evals);
+  }
+  if (F32) {
+    /* per-lane counters */
+    unsigned long long rc = (unsigned long long) nrecheck, ce = (unsigned long long) ncensus;
+    if (rc) atomicAdd(a.stats + 1, rc);
+    if (ce) atomicAdd(a.stats + 2, ce);
+  }
+}
+
+/* add a device histogram into the caller's device array */
+__global__ void k_add_u64(long long *dst, const unsigned long long *src, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] += (long long) src[i];
+}
+__global__ void k_add_f64(double *dst, const double *src, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] += src[i];
+}
+
+/* ======================================================================== */
+/*                                host side                                 */
+/* ======================================================================== */
+
+struct DevBuf { void *p = nullptr; size_t cap = 0; };
+
+struct fcfc_gpu_ctx {
+  int device = 0;
+  int ncu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  char err[512] = {0};
+  fcfc_gpu_stats stats{};
+  int t_cell_div = 0, t_tile = 0, t_zdiv = 0, t_nblock = 0;
+  std::vector<DevBuf> bufs;
+};
+
+static void set_err(fcfc_gpu_ctx *c, const char *fmt, ...) {
+  if (!c) return;
+  va_list ap; va_start(ap, fmt);
+  vsnprintf(c->err, sizeof c->err, fmt, ap);
+  va_end(ap);
+}
+
+#define HIPCHK(ctx, call) do { hipError_t _e = (call); if (_e != hipSuccess) { \
+    set_err(ctx, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, __LINE__); \
+    return FCFC_ERR_GPU; } } while (0)
+
+/* workspace slots */
+enum {
+  WS_IN_X, WS_IN_Y, WS_IN_Z, WS_IN_S, WS_IN_W,          /* uploads, catalogue A */
+  WS_IN2_X, WS_IN2_Y, WS_IN2_Z, WS_IN2_S, WS_IN2_W,     /* uploads, catalogue B */
+  WS_A_X, WS_A_Y, WS_A_Z, WS_A_S, WS_A_W, WS_A_IDX, WS_A_CS,
+  WS_B_X, WS_B_Y, WS_B_Z, WS_B_S, WS_B_W, WS_B_IDX, WS_B_CS,
+  WS_CID, WS_RANK, WS_COUNT, WS_ROWN, WS_ROWOFF, WS_TILES,
+  WS_STRIPS, WS_STAB, WS_PTAB, WS_S2E, WS_PE, WS_HIST, WS_MISC, WS_G, WS_GA,
+  WS_NSLOT
+};
+
+static void *ws(fcfc_gpu_ctx *c, int slot, size_t bytes, int *err) {
+  if ((int) c->bufs.size() < WS_NSLOT) c->bufs.resize(WS_NSLOT);
+  DevBuf &b = c->bufs[slot];
+  if (bytes == 0) bytes = 8;
+  if (b.cap < bytes) {
+    if (b.p) { (void) hipFree(b.p); b.p = nullptr; b.cap = 0; }
+    size_t nb = bytes + bytes / 8 + 256;
+    if (hipMalloc(&b.p, nb) != hipSuccess) {
+      set_err(c, "hipMalloc of %zu bytes failed", nb);
+      *err = FCFC_ERR_MEMORY;
+      return nullptr;
+    }
+    b.cap = nb;
+  }
+  return b.p;
+}
+
+extern "C" int fcfc_gpu_init(fcfc_gpu_ctx **out, int device) {
+  if (!out) return FCFC_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FCFC_ERR_GPU;
+  if (device < 0 || device >= ndev) return FCFC_ERR_ARG;
+  fcfc_gpu_ctx *c = new (std::nothrow) fcfc_gpu_ctx();
+  if (!c) return FCFC_ERR_MEMORY;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return FCFC_ERR_GPU; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return FCFC_ERR_GPU; }
+  c->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c; return FCFC_ERR_GPU;
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" int fcfc_gpu_destroy(fcfc_gpu_ctx *c) {
+  if (!c) return 0;
+  (void) hipSetDevice(c->device);
+  if (c->stream) (void) hipStreamSynchronize(c->stream);
+  for (auto &b : c->bufs) if (b.p) (void) hipFree(b.p);
+  if (c->ev0) (void) hipEventDestroy(c->ev0);
+  if (c->ev1) (void) hipEventDestroy(c->ev1);
+  if (c->stream) (void) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+extern "C" const char *fcfc_gpu_last_error(const fcfc_gpu_ctx *c) {
+  return c ? c->err : "no context";
+}
+
+extern "C" int fcfc_gpu_last_stats(const fcfc_gpu_ctx *c, fcfc_gpu_stats *st) {
+  if (!c || !st) return FCFC_ERR_ARG;
+  *st = c->stats;
+  return 0;
+}
+
+extern "C" int fcfc_gpu_set_tuning(fcfc_gpu_ctx *c, int cell_div, int tile,
+    int zdiv, int nblock) {
+  if (!c) return FCFC_ERR_ARG;
+  if (cell_div < 0 || cell_div > 32 || tile < 0 || tile > 2 * FK_NT || zdiv < 0 || zdiv > 64 || nblock < 0)
+    return FCFC_ERR_ARG;
+  c->t_cell_div = cell_div; c->t_tile = tile; c->t_zdiv = zdiv; c->t_nblock = nblock;
+  return 0;
+}
+
+/* ---- per-job plan ------------------------------------------------------ */
+
+struct JobPlan {
+  FkGrid g;
+  int T;            /* objects per tile */
+  int zmaxc;        /* max z-cells per tile */
+  int R;            /* objects per lane */
+  std::vector<int> strips;
+  int nstrips;
+};
+
+static double max_sep2(const fcfc_bins *b, double *rxy2, double *hzc) {
+  double s2max = b->s2bin[b->ns];
+  *hzc = -1;
+  if (b->bintype == FCFC_BIN_SPI) {
+    double pm = b->pbin[b->np];
+    if (b->geometry == FCFC_GEOM_BOX) {
+      *rxy2 = s2max;
+      *hzc = pm;
+      return s2max + pm * pm;
+    }
+    /* survey: s^2 = s_perp^2 + pi^2 with pbin holding pi^2 */
+    *rxy2 = s2max + pm;
+    return s2max + pm;
+  }
+  *rxy2 = s2max;
+  return s2max;
+}
+
+/* Build the grid geometry and strip list.  lo/hi: bounding box (survey). */
+static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
+    const double lo[3], const double hi[3], int autoc, int mode, int64_t nA,
+    int64_t nB, JobPlan *P) {
+  double rxy2, hzc;
+  double r2 = max_sep2(b, &rxy2, &hzc);
+  /* pad: the survey formula carries ~1e-10 absolute rounding in s^2 */
+  const double pad = (b->geometry == FCFC_GEOM_SURVEY) ? 1e-6 : 1e-9;
+  r2 = r2 * (1 + pad) + pad;
+  rxy2 = rxy2 * (1 + pad) + pad;
+  if (hzc >= 0) hzc = hzc * (1 + pad) + pad;
+  const double R = sqrt(r2), Rxy = sqrt(rxy2);
+  FkGrid &g = P->g;
+  memset(&g, 0, sizeof g);
+  g.R2 = r2; g.Rxy2 = rxy2; g.hz_const = hzc;
+  /* objects per unit volume decides the default cell division */
+  int cd = c->t_cell_div;
+  int zd = c->t_zdiv;
+  double vol;
+  double ext[3];
+  if (b->geometry == FCFC_GEOM_BOX) {
+    for (int k = 0; k < 3; k++) ext[k] = bsize[k];
+  }
+  else {
+    for (int k = 0; k < 3; k++) { ext[k] = hi[k] - lo[k]; if (!(ext[k] > 0)) ext[k] = 1e-6 * (1 + fabs(hi[k])); }
+  }
+  vol = ext[0] * ext[1] * ext[2];
+  const double nden = (double) std::max<int64_t>(nB, 1) / vol;
+  if (cd <= 0) {
+    /* about 24..64 B objects per xy cell per unit of R along z is a good
+     * balance between pruning and segment overhead */
+    double objs_per_col = nden * Rxy * Rxy;   /* per unit z, for cd = 1 */
+    cd = 4;
+    while (cd > 1 && objs_per_col / (cd * cd) < 4.0) cd--;
+    while (cd < 8 && objs_per_col / (cd * cd) > 256.0) cd++;
+  }
+  if (zd <= 0) zd = 4;
+  P->R = (mode == MODE_GRAD) ? 1 : 1;
+  P->T = c->t_tile > 0 ? c->t_tile : FK_NT * P->R;
+  if (P->T > FK_NT * P->R) P->T = FK_NT * P->R;
+  double w = Rxy / cd;
+  if (b->geometry == FCFC_GEOM_BOX) {
+    g.periodic = 1;
+    g.Lx = bsize[0]; g.Ly = bsize[1]; g.Lz = bsize[2];
+    int n[2];
+    for (int k = 0; k < 2; k++) {
+      int nc = (int) floor(ext[k] / w);
+      if (nc < 1) nc = 1;
+      for (;;) {
+        double wk = ext[k] / nc;
+        int reach = (int) ceil(Rxy / wk);
+        if (2 * reach + 1 <= nc) break;
+        nc++;
+        if (nc > 4096) { set_err(c, "separation range too close to half the box"); return FCFC_ERR_CFG; }
+      }
+      n[k] = nc;
+    }
+    g.ncx = n[0]; g.ncy = n[1];
+    g.wx = ext[0] / g.ncx; g.wy = ext[1] / g.ncy;
+    double wzt = std::min(g.wx, g.wy) / zd;
+    /* z windows must stay shorter than the box: Lz - 2 R >= 4 wz */
+    double zroom = ext[2] - 2 * (hzc >= 0 ? hzc : R);
+    if (zroom <= 0) { set_err(c, "separation range not below half the box"); return FCFC_ERR_CFG; }
+    if (wzt > zroom / 6) wzt = zroom / 6;
+    g.ncz = std::max(1, (int) floor(ext[2] / wzt));
+    g.wz = ext[2] / g.ncz;
+    g.ox = g.oy = g.oz = 0;
+    P->zmaxc = (int) floor(zroom / g.wz) - 4;
+    if (P->zmaxc < 1) P->zmaxc = 1;
+    P->zmaxc = std::min(P->zmaxc, std::max(1, (int) ceil(2 * zd * 1.0)));
+  }
+  else {
+    g.periodic = 0;
+    g.ncx = std::max(1, (int) ceil(ext[0] / w));
+    g.ncy = std::max(1, (int) ceil(ext[1] / w));
+    g.wx = w * (1 + 1e-12); g.wy = w * (1 + 1e-12);
+    g.wz = w / zd;
+    g.ncz = std::max(1, (int) ceil(ext[2] / g.wz) + 1);
+    g.ox = lo[0]; g.oy = lo[1]; g.oz = lo[2];
+    P->zmaxc = std::max(1, 2 * zd);
+    if ((double) g.ncx * g.ncy * g.ncz > 2.0e8) {
+      set_err(c, "grid too large (%d x %d x %d cells)", g.ncx, g.ncy, g.ncz);
+      return FCFC_ERR_CFG;
+    }
+  }
+  /* strips */
+  P->strips.clear();
+  const int kx = (int) ceil(Rxy / g.wx), ky = (int) ceil(Rxy / g.wy);
+  for (int dx = -kx; dx <= kx; dx++) {
+    for (int dy = -ky; dy <= ky; dy++) {
+      double gx = std::max(abs(dx) - 1, 0) * g.wx, gy = std::max(abs(dy) - 1, 0) * g.wy;
+      if (gx * gx + gy * gy >= rxy2) continue;
+      int diag = 0;
+      if (autoc && mode != MODE_GRAD) {
+        if (dx < 0 || (dx == 0 && dy < 0)) continue;
+        if (dx == 0 && dy == 0) diag = 1;
+      }
+      P->strips.push_back(dx); P->strips.push_back(dy); P->strips.push_back(diag);
+    }
+  }
+  P->nstrips = (int) P->strips.size() / 3;
+  return 0;
+}
+
+/* sorted catalogue on the device */
+struct DevCat {
+  int n = 0;
+  double *x = nullptr, *y = nullptr, *z = nullptr, *s = nullptr, *w = nullptr;
+  int *idx = nullptr, *cs = nullptr;
+};
+
+static int upload(fcfc_gpu_ctx *c, const fcfc_cat *cat, int base, int need_s,
+    int need_w, const double **px, const double **py, const double **pz,
+    const double **ps, const double **pw) {
+  int err = 0;
+  const size_t nb = sizeof(double) * (size_t) cat->n;
+  if (cat->on_device) {
+    *px = cat->x; *py = cat->y; *pz = cat->z;
+    *ps = need_s ? cat->s : nullptr;
+    *pw = need_w ? cat->w : nullptr;
+    return 0;
+  }
+  double *dx = (double *) ws(c, base + 0, nb, &err);
+  double *dy = (double *) ws(c, base + 1, nb, &err);
+  double *dz = (double *) ws(c, base + 2, nb, &err);
+  if (err) return err;
+  HIPCHK(c, hipMemcpyAsync(dx, cat->x, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dy, cat->y, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dz, cat->z, nb, hipMemcpyHostToDevice, c->stream));
+  *px = dx; *py = dy; *pz = dz;
+  *ps = nullptr; *pw = nullptr;
+  if (need_s && cat->s) {
+    double *ds = (double *) ws(c, base + 3, nb, &err);
+    if (err) return err;
+    HIPCHK(c, hipMemcpyAsync(ds, cat->s, nb, hipMemcpyHostToDevice, c->stream));
+    *ps = ds;
+  }
+  if (need_w && cat->w) {
+    double *dw = (double *) ws(c, base + 4, nb, &err);
+    if (err) return err;
+    HIPCHK(c, hipMemcpyAsync(dw, cat->w, nb, hipMemcpyHostToDevice, c->stream));
+    *pw = dw;
+  }
+  return 0;
+}
+
+static inline int nblk(int64_t n, int bs) {
+  int64_t b = (n + bs - 1) / bs;
+  if (b < 1) b = 1;
+  if (b > 65536) b = 65536;
+  return (int) b;
+}
+
+/* sort one catalogue into the grid */
+static int build_grid(fcfc_gpu_ctx *c, const FkGrid &g, int n, const double *x,
+    const double *y, const double *z, const double *s, const double *w,
+    int need_s, int slot, DevCat *out) {
+  int err = 0;
+  const size_t nb = sizeof(double) * (size_t) (n > 0 ? n : 1);
+  const int ncell = g.ncx * g.ncy * g.ncz;
+  out->n = n;
+  out->x = (double *) ws(c, slot + 0, nb, &err);
+  out->y = (double *) ws(c, slot + 1, nb, &err);
+  out->z = (double *) ws(c, slot + 2, nb, &err);
+  out->s = need_s ? (double *) ws(c, slot + 3, nb, &err) : nullptr;
+  out->w = w ? (double *) ws(c, slot + 4, nb, &err) : nullptr;
+  out->idx = (int *) ws(c, slot + 5, sizeof(int) * (size_t) (n > 0 ? n : 1), &err);
+  out->cs = (int *) ws(c, slot + 6, sizeof(int) * ((size_t) ncell + 1), &err);
+  int *cid = (int *) ws(c, WS_CID, sizeof(int) * (size_t) (n > 0 ? n : 1), &err);
+  int *rank = (int *) ws(c, WS_RANK, sizeof(int) * (size_t) (n > 0 ? n : 1), &err);
+  int *count = (int *) ws(c, WS_COUNT, sizeof(int) * (size_t) ncell, &err);
+  if (err) return err;
+  const double *sin = s;
+  if (need_s && !s && n > 0) {
+    /* compute |r|^2 into the rank buffer's neighbour: use the output slot
+     * temporarily, then scatter from a dedicated copy */
+    double *tmp = (double *) ws(c, WS_IN_S + (slot == WS_A_X ? 0 : 5), nb, &err);
+    if (err) return err;
+    hipLaunchKernelGGL(k_survey_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, x, y, z, tmp);
+    sin = tmp;
+  }
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(ncell, 256)), dim3(256), 0, c->stream, count, ncell, 0);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_assign, dim3(nblk(n, 256)), dim3(256), 0, c->stream, g, n, x, y, z, cid, rank, count);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, count, out->cs, ncell);
+  if (n > 0)
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, cid, rank, out->cs,
+        x, y, z, need_s ? sin : nullptr, w, out->x, out->y, out->z, out->s, out->w, out->idx);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+static int build_tiles(fcfc_gpu_ctx *c, const FkGrid &g, const DevCat &A,
+    int T, int zmaxc, FkTile **tiles, int *ntiles) {
+  int err = 0;
+  const int nrow = g.ncx * g.ncy;
+  int *rown = (int *) ws(c, WS_ROWN, sizeof(int) * (size_t) nrow, &err);
+  int *rowoff = (int *) ws(c, WS_ROWOFF, sizeof(int) * ((size_t) nrow + 1), &err);
+  if (err) return err;
+  hipLaunchKernelGGL(k_tile_count, dim3(nblk(nrow, 64)), dim3(64), 0, c->stream, nrow, g.ncz, A.cs, T, zmaxc, rown);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, rown, rowoff, nrow);
+  int nt = 0;
+  HIPCHK(c, hipMemcpyAsync(&nt, rowoff + nrow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  FkTile *t = (FkTile *) ws(c, WS_TILES, sizeof(FkTile) * (size_t) (nt > 0 ? nt : 1), &err);
+  if (err) return err;
+  if (nt > 0)
+    hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nrow, 64)), dim3(64), 0, c->stream, nrow, g.ncz, A.cs, T, zmaxc, rowoff, t);
+  HIPCHK(c, hipGetLastError());
+  *tiles = t; *ntiles = nt;
+  return 0;
+}
+
+/* LDS plan */
+struct LdsPlan { int off_hist, off_stab, off_ptab, off_mutab, off_edges, off_strip, off_chunk, total, nh; };
+
+static int align16(int v) { return (v + 15) & ~15; }
+
+static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh) {
+  LdsPlan L;
+  int o = 0;
+  L.nh = nh;
+  L.off_hist = o;
+  if (mode == MODE_GRAD) o += b->ntot * 8;
+  else o += nh * b->ntot * (wt ? 8 : 4);
+  o = align16(o);
+  L.off_stab = o; o = align16(o + (b->stab_len + 2) * 2);
+  L.off_ptab = o; if (b->bintype == FCFC_BIN_SPI) o = align16(o + (b->ptab_len + 2) * 2);
+  L.off_mutab = o;
+  if (b->bintype == FCFC_BIN_SMU && b->nmu * b->nmu <= 16384) o = align16(o + b->nmu * b->nmu);
+  L.off_edges = o;
+  if (b->tabtype) o = align16(o + 8 * (b->ns + 1 + (b->bintype == FCFC_BIN_SPI ? b->np + 1 : 0)));
+  L.off_strip = o; o = align16(o + (int) sizeof(FkStripLds));
+  L.off_chunk = o;
+  int per = (mode == MODE_C32) ? 12 : 24;
+  if (need_s) per += 8;
+  if (wt || mode == MODE_GRAD) per += 8;
+  per += 8;   /* index + code */
+  o = align16(o + FK_CH * per);
+  L.total = o;
+  return L;
+}
+
+typedef void (*kfn_t)(const FkArgs);
+
+template <int BIN, int GEOM, bool WT, int MODE>
+static kfn_t pick4() { return k_count<BIN, GEOM, WT, MODE, 1>; }
+
+template <int BIN, int GEOM>
+static kfn_t pick3(int wt, int mode) {
+  if (mode == MODE_GRAD) return pick4<BIN, GEOM, true, MODE_GRAD>();
+  if (mode == MODE_C32 && GEOM == FCFC_GEOM_BOX)
+    return wt ? pick4<BIN, GEOM, true, MODE_C32>() : pick4<BIN, GEOM, false, MODE_C32>();
+  return wt ? pick4<BIN, GEOM, true, MODE_C64>() : pick4<BIN, GEOM, false, MODE_C64>();
+}
+
+static kfn_t pick_kernel(int bintype, int geom, int wt, int mode) {
+  if (geom == FCFC_GEOM_BOX) {
+    if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_BOX>(wt, mode);
+    if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_BOX>(wt, mode);
+    return pick3<FCFC_BIN_SPI, FCFC_GEOM_BOX>(wt, mode);
+  }
+  if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_SURVEY>(wt, mode);
+  if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_SURVEY>(wt, mode);
+  return pick3<FCFC_BIN_SPI, FCFC_GEOM_SURVEY>(wt, mode);
+}
+
+/* FP32 error model, see the derivation in the FP32 path of k_count. */
+static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
+    FkArgs *a) {
+  const double u = ldexp(1.0, -24);
+  double s2max = b->s2bin[b->ns];
+  double pmx = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
+  double Rb = sqrt(s2max + pmx * pmx) * 2.0;           /* |d| bound used */
+  double Ex = g.wx, Ey = g.wy, Ez = (zmaxc + 2) * g.wz;
+  double E = std::max(Ex, std::max(Ey, Ez));
+  double tiny = ldexp(1.0, -40) * std::max(g.Lx, std::max(g.Ly, g.Lz));
+  /* per-axis error e_k <= u (2|d_k| + E_k) + tiny */
+  double ek = u * (2 * Rb + E) + tiny;
+  /* s^2 error: sum 2|d_k| e_k + e_k^2 + 3 u s^2 */
+  double es2 = 2 * sqrt(3.0) * Rb * ek + 3 * ek * ek + 4 * u * Rb * Rb;
+  a->e_s2 = (float) (es2 * 1.25 + 1e-6);
+  a->e_pi = (float) ((2 * ek + 2 * u * Rb) * 1.25 + 1e-7);
+  double nmu = b->nmu;
+  /* mu window: dr <= r (10 u) + inv_s u E (1.74 r + nmu) + small */
+  a->e_rrel = (float) (12 * u + 1e-12);
+  a->e_rabs = (float) (1.8 * (u * E + tiny) * 1.25);
+  a->e_rz = (float) (nmu * (u * E + tiny) * 1.25 * 1.1);
+  a->s2_tiny = (float) std::max(1e-4, 1e4 * ek * ek);
+}
+
+/* One counting (or gradient) job on device-resident, sorted catalogues. */
+static int run_count(fcfc_gpu_ctx *c, const fcfc_bins *b, const JobPlan &P,
+    const DevCat &A, const DevCat &B, int autoc, int wt, int mode,
+    const FkTile *tiles, int ntiles, const double *G, double *gA,
+    void *dhist, unsigned long long *dstats, unsigned int *dqueue) {
+  int err = 0;
+  const int need_s = (b->geometry == FCFC_GEOM_SURVEY && b->bintype != FCFC_BIN_ISO);
+  const int geom = b->geometry;
+  int m = mode;
+  if (m == MODE_C32 && geom != FCFC_GEOM_BOX) m = MODE_C64;    /* survey FP32: FP64 kernel */
+  /* histogram copies: one per wavefront if small, else one per workgroup */
+  int nh = 1;
+  if (m != MODE_GRAD) {
+    int per = b->ntot * (wt ? 8 : 4);
+    nh = (per * 4 <= 16384) ? 4 : ((per * 2 <= 24576) ? 2 : 1);
+  }
+  LdsPlan L = lds_plan(b, m, wt, need_s, nh);
+  if (L.total > 160 * 1024) {
+    set_err(c, "binning tables need %d bytes of LDS (limit 160 KiB)", L.total);
+    return FCFC_ERR_CFG;
+  }
+  kfn_t fn = pick_kernel(b->bintype, geom, wt, m);
+  HIPCHK(c, hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
+  /* device tables */
+  uint16_t *dstab = (uint16_t *) ws(c, WS_STAB, sizeof(uint16_t) * (b->stab_len + 1), &err);
+  uint16_t *dptab = (uint16_t *) ws(c, WS_PTAB, sizeof(uint16_t) * (b->ptab_len + 1), &err);
+  double *ds2e = (double *) ws(c, WS_S2E, sizeof(double) * (b->ns + 1), &err);
+  double *dpe = (double *) ws(c, WS_PE, sizeof(double) * (b->np + 1), &err);
+  int *dstrips = (int *) ws(c, WS_STRIPS, sizeof(int) * P.strips.size(), &err);
+  if (err) return err;
+  HIPCHK(c, hipMemcpyAsync(dstab, b->stab, sizeof(uint16_t) * b->stab_len, hipMemcpyHostToDevice, c->stream));
+  if (b->bintype == FCFC_BIN_SPI)
+    HIPCHK(c, hipMemcpyAsync(dptab, b->ptab, sizeof(uint16_t) * b->ptab_len, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ds2e, b->s2bin, sizeof(double) * (b->ns + 1), hipMemcpyHostToDevice, c->stream));
+  if (b->bintype == FCFC_BIN_SPI)
+    HIPCHK(c, hipMemcpyAsync(dpe, b->pbin, sizeof(double) * (b->np + 1), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dstrips, P.strips.data(), sizeof(int) * P.strips.size(), hipMemcpyHostToDevice, c->stream));
+
+  FkArgs a;
+  memset(&a, 0, sizeof a);
+  a.A = FkCat{A.n, A.x, A.y, A.z, A.s, A.w, A.idx, A.cs};
+  a.B = FkCat{B.n, B.x, B.y, B.z, B.s, B.w, B.idx, B.cs};
+  a.g = P.g;
+  FkBins &fb = a.bins;
+  fb.bintype = b->bintype; fb.geometry = geom;
+  fb.ns = b->ns; fb.nmu = b->nmu; fb.np = b->np; fb.ntot = b->ntot;
+  fb.tabtype = b->tabtype; fb.soff = b->soff; fb.poff = b->poff;
+  fb.stab_len = b->stab_len; fb.ptab_len = b->ptab_len;
+  fb.mutab_lds = (b->bintype == FCFC_BIN_SMU && b->nmu * b->nmu <= 16384);
+  fb.s2min = b->s2bin[0]; fb.s2max = b->s2bin[b->ns];
+  fb.pmin = (b->bintype == FCFC_BIN_SPI) ? b->pbin[0] : 0;
+  fb.pmax = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
+  fb.nmu2 = (double) b->nmu * b->nmu;
+  fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
+  a.tiles = tiles; a.ntiles = ntiles;
+  a.strips = dstrips; a.nstrips = P.nstrips;
+  a.autoc = autoc;
+  a.queue = dqueue;
+  a.gcnt_u = (unsigned long long *) dhist; a.gcnt_d = (double *) dhist;
+  a.stats = dstats;
+  a.nh = L.nh;
+  a.off_hist = L.off_hist; a.off_stab = L.off_stab; a.off_ptab = L.off_ptab;
+  a.off_mutab = L.off_mutab; a.off_edges = L.off_edges; a.off_strip = L.off_strip;
+  a.off_chunk = L.off_chunk;
+  if (m == MODE_C32) fp32_windows(b, P.g, P.zmaxc, &a);
+  a.census = 0;
+  a.G = G; a.gA = gA;
+  int occ = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *) fn, FK_NT, L.total));
+  if (occ < 1) occ = 1;
+  int nblock = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
+  if (nblock > ntiles) nblock = std::max(ntiles, 1);
+  if (ntiles > 0)
+    hipLaunchKernelGGL(fn, dim3(nblock), dim3(FK_NT), L.total, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+static int cat_ok(const fcfc_cat *k) {
+  return k && k->n >= 0 && k->n < 0x7fffffff && (k->n == 0 || (k->x && k->y && k->z));
+}
+
+/* bounding box of host or device coordinates (survey grids) */
+__global__ void k_minmax(int n, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, double *out) {
+  __shared__ double sm[6][256];
+  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v[0] = fmin(v[0], x[i]); v[1] = fmin(v[1], y[i]); v[2] = fmin(v[2], z[i]);
+    v[3] = fmax(v[3], x[i]); v[4] = fmax(v[4], y[i]); v[5] = fmax(v[5], z[i]);
+  }
+  for (int k = 0; k < 6; k++) sm[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int) threadIdx.x < s)
+      for (int k = 0; k < 6; k++)
+        sm[k][threadIdx.x] = (k < 3) ? fmin(sm[k][threadIdx.x], sm[k][threadIdx.x + s])
+                                     : fmax(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = 0;   /* unused slot */
+  if (threadIdx.x < 6) out[8 + blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+static int bbox(fcfc_gpu_ctx *c, const double *x, const double *y, const double *z,
+    int n, double lo[3], double hi[3]) {
+  int err = 0;
+  if (n <= 0) return 0;
+  const int nbk = 64;
+  double *d = (double *) ws(c, WS_MISC, sizeof(double) * (8 + 6 * nbk), &err);
+  if (err) return err;
+  hipLaunchKernelGGL(k_minmax, dim3(nbk), dim3(256), 0, c->stream, n, x, y, z, d);
+  std::vector<double> h(8 + 6 * nbk);
+  HIPCHK(c, hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int bk = 0; bk < nbk; bk++)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fmin(lo[k], h[8 + bk * 6 + k]);
+      hi[k] = fmax(hi[k], h[8 + bk * 6 + 3 + k]);
+    }
+  return 0;
+}
+
+/* Shared front half of count / backward: upload, grid, tiles. */
+struct Prepared {
+  JobPlan P;
+  DevCat A, B;
+  FkTile *tiles = nullptr;
+  int ntiles = 0;
+};
+
+static int prepare(fcfc_gpu_ctx *c, const fcfc_cat *a, const fcfc_cat *b,
+    const fcfc_bins *bins, const double *bsize, int autoc, int wt, int mode,
+    Prepared *pp) {
+  const int need_s = (bins->geometry == FCFC_GEOM_SURVEY && bins->bintype != FCFC_BIN_ISO);
+  const double *ax, *ay, *az, *as, *aw, *bx = nullptr, *by = nullptr, *bz = nullptr, *bs = nullptr, *bw = nullptr;
+  int e = upload(c, a, WS_IN_X, need_s, wt, &ax, &ay, &az, &as, &aw);
+  if (e) return e;
+  if (!autoc) { e = upload(c, b, WS_IN2_X, need_s, wt, &bx, &by, &bz, &bs, &bw); if (e) return e; }
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  if (bins->geometry == FCFC_GEOM_SURVEY) {
+    if ((e = bbox(c, ax, ay, az, (int) a->n, lo, hi))) return e;
+    if (!autoc && (e = bbox(c, bx, by, bz, (int) b->n, lo, hi))) return e;
+    if (!(lo[0] <= hi[0])) { for (int k = 0; k < 3; k++) { lo[k] = 0; hi[k] = 1; } }
+  }
+  e = make_plan(c, bins, bsize, lo, hi, autoc, mode, a->n, autoc ? a->n : b->n, &pp->P);
+  if (e) return e;
+  if ((e = build_grid(c, pp->P.g, (int) a->n, ax, ay, az, as, wt ? aw : nullptr, need_s, WS_A_X, &pp->A))) return e;
+  if (wt && !aw && a->n > 0) {
+    /* unit weights for an unweighted catalogue in a weighted count
+     * (build_tree.c:143-156) */
+    int err = 0;
+    double *one = (double *) ws(c, WS_A_W, sizeof(double) * a->n, &err);
+    if (err) return err;
+    std::vector<double> ones(a->n, 1.0);
+    HIPCHK(c, hipMemcpyAsync(one, ones.data(), sizeof(double) * a->n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    pp->A.w = one;
+  }
+  if (autoc) pp->B = pp->A;
+  else {
+    if ((e = build_grid(c, pp->P.g, (int) b->n, bx, by, bz, bs, wt ? bw : nullptr, need_s, WS_B_X, &pp->B))) return e;
+    if (wt && !bw && b->n > 0) {
+      int err = 0;
+      double *one = (double *) ws(c, WS_B_W, sizeof(double) * b->n, &err);
+      if (err) return err;
+      std::vector<double> ones(b->n, 1.0);
+      HIPCHK(c, hipMemcpyAsync(one, ones.data(), sizeof(double) * b->n, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      pp->B.w = one;
+    }
+  }
+  return build_tiles(c, pp->P.g, pp->A, pp->P.T, pp->P.zmaxc, &pp->tiles, &pp->ntiles);
+}
+
+static int bins_ok(const fcfc_bins *b) {
+  return b && b->ns > 0 && b->s2bin && b->stab && b->ntot > 0 &&
+      (b->bintype != FCFC_BIN_SPI || (b->pbin && b->ptab));
+}
+
+extern "C" int fcfc_gpu_count(fcfc_gpu_ctx *c, const fcfc_cat *a,
+    const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
+    uint32_t flags, void *cnt) {
+  if (!c) return FCFC_ERR_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  const int autoc = (flags & FCFC_GPU_AUTO) ? 1 : 0;
+  const int wt = (flags & FCFC_GPU_WEIGHTED) ? 1 : 0;
+  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 : MODE_C64;
+  if (!cat_ok(a) || (!autoc && !cat_ok(b)) || !bins_ok(bins) || !cnt) {
+    set_err(c, "invalid argument"); return FCFC_ERR_ARG;
+  }
+  if (bins->geometry == FCFC_GEOM_BOX && !bsize) { set_err(c, "box size required"); return FCFC_ERR_ARG; }
+  HIPCHK(c, hipSetDevice(c->device));
+  memset(&c->stats, 0, sizeof c->stats);
+  Prepared pp;
+  int e = prepare(c, a, b, bins, bsize, autoc, wt, mode, &pp);
+  if (e) return e;
+  int err = 0;
+  void *dh = ws(c, WS_HIST, 8 * (size_t) bins->ntot, &err);
+  unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
+  if (err) return err;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *) dh, bins->ntot);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  FkArgs dummy; (void) dummy;
+  e = run_count(c, bins, pp.P, pp.A, pp.B, autoc, wt, mode, pp.tiles, pp.ntiles, nullptr, nullptr,
+      dh, misc + 8, (unsigned int *) (misc));
+  if (e) return e;
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  if (flags & FCFC_GPU_CNT_DEVICE) {
+    if (wt) hipLaunchKernelGGL(k_add_f64, dim3(nblk(bins->ntot, 256)), dim3(256), 0, c->stream, (double *) cnt, (const double *) dh, bins->ntot);
+    else hipLaunchKernelGGL(k_add_u64, dim3(nblk(bins->ntot, 256)), dim3(256), 0, c->stream, (long long *) cnt, (const unsigned long long *) dh, bins->ntot);
+    HIPCHK(c, hipGetLastError());
+  }
+  else {
+    std::vector<unsigned long long> h(bins->ntot);
+    HIPCHK(c, hipMemcpyAsync(h.data(), dh, 8 * (size_t) bins->ntot, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (wt) { double *o = (double *) cnt; const double *hd = (const double *) h.data(); for (int i = 0; i < bins->ntot; i++) o[i] += hd[i]; }
+    else { int64_t *o = (int64_t *) cnt; for (int i = 0; i < bins->ntot; i++) o[i] += (int64_t) h[i]; }
+  }
+  unsigned long long st[3] = {0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(st, misc + 8, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->stats.evals = (double) st[0];
+  c->stats.rechecked = (double) st[1];
+  c->stats.census = (double) st[2];
+  c->stats.kernel_ms = ms;
+  c->stats.cell_div = (int) lround(sqrt(pp.P.g.Rxy2) / pp.P.g.wx);
+  c->stats.tile = pp.P.T;
+  c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
+    const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
+    uint32_t flags, const double *grad_cnt, double *ga, double *gb) {
+  if (!c) return FCFC_ERR_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  const int autoc = (flags & FCFC_GPU_AUTO) ? 1 : 0;
+  if (!cat_ok(a) || (!autoc && !cat_ok(b)) || !bins_ok(bins) || !grad_cnt || !ga) {
+    set_err(c, "invalid argument"); return FCFC_ERR_ARG;
+  }
+  if (bins->geometry == FCFC_GEOM_BOX && !bsize) { set_err(c, "box size required"); return FCFC_ERR_ARG; }
+  HIPCHK(c, hipSetDevice(c->device));
+  memset(&c->stats, 0, sizeof c->stats);
+  const int dev = (flags & FCFC_GPU_CNT_DEVICE) ? 1 : 0;
+  int err = 0;
+  double *dG = (double *) ws(c, WS_G, sizeof(double) * bins->ntot, &err);
+  unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
+  if (err) return err;
+  HIPCHK(c, hipMemcpyAsync(dG, grad_cnt, sizeof(double) * bins->ntot,
+      dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+  float ktot = 0;
+  double evals = 0;
+  for (int pass = 0; pass < (autoc ? 1 : 2); pass++) {
+    if (pass == 1 && !gb) break;
+    const fcfc_cat *pa = pass ? b : a, *pb = pass ? a : b;
+    double *gout = pass ? gb : ga;
+    Prepared pp;
+    int e = prepare(c, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &pp);
+    if (e) return e;
+    double *dgA = (double *) ws(c, WS_GA, sizeof(double) * (size_t) (pa->n > 0 ? pa->n : 1), &err);
+    if (err) return err;
+    hipLaunchKernelGGL(k_zero_u64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, (unsigned long long *) dgA, (int) pa->n);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    e = run_count(c, bins, pp.P, pp.A, pp.B, autoc, 1, MODE_GRAD, pp.tiles, pp.ntiles, dG, dgA,
+        nullptr, misc + 8, (unsigned int *) misc);
+    if (e) return e;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    if (dev) {
+      hipLaunchKernelGGL(k_add_f64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, gout, dgA, (int) pa->n);
+      HIPCHK(c, hipGetLastError());
+    }
+    else {
+      std::vector<double> h((size_t) pa->n);
+      HIPCHK(c, hipMemcpyAsync(h.data(), dgA, sizeof(double) * (size_t) pa->n, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (int64_t i = 0; i < pa->n; i++) gout[i] += h[(size_t) i];
+    }
+    unsigned long long st[3];
+    HIPCHK(c, hipMemcpyAsync(st, misc + 8, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    ktot += ms;
+    evals += (double) st[0];
+  }
+  c->stats.evals = evals;
+  c->stats.kernel_ms = ktot;
+  c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+extern "C" int fcfc_gpu_count_batch(fcfc_gpu_ctx *c, int njob,
+    const fcfc_batch_job *jobs, const fcfc_bins *bins, uint32_t flags,
+    void *cnt) {
+  /* first version: sequential jobs (the persistent multi-job launch is
+   * added on top of the same kernels) */
+  if (!c || njob < 0 || (njob && !jobs) || !cnt) return FCFC_ERR_ARG;
+  double evals = 0, kms = 0, rc = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int k = 0; k < njob; k++) {
+    const fcfc_batch_job *j = jobs + k;
+    uint32_t f = (flags & ~(FCFC_GPU_AUTO | FCFC_GPU_WEIGHTED)) | j->flags;
+    const int wt = (f & FCFC_GPU_WEIGHTED) ? 1 : 0;
+    char *dst = (char *) cnt + (size_t) k * bins->ntot * 8;
+    (void) wt;
+    int e = fcfc_gpu_count(c, &j->a, &j->b, bins, j->bsize, f, dst);
+    if (e) return e;
+    evals += c->stats.evals; kms += c->stats.kernel_ms; rc += c->stats.rechecked;
+  }
+  c->stats.evals = evals; c->stats.kernel_ms = kms; c->stats.rechecked = rc;
+  c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
