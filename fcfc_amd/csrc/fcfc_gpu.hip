@@ -147,6 +147,15 @@ __global__ void k_tile_emit(int nrow, int ncz, const int *__restrict__ cstart,
 /*                             count kernel                                 */
 /* ======================================================================== */
 
+/* FP32 exactness windows (see fp32_windows on the host). */
+struct FkWin {
+  float e_s2;                 /* absolute window on s^2 (or s_perp^2)       */
+  float e_pi;                 /* absolute window on pi                      */
+  float e_rrel, e_rabs, e_rz; /* mu window: dt = t e_rrel + inv_s (t e_rabs + e_rz) */
+  float s2_tiny;              /* below this s^2 always re-evaluate          */
+  float nmu_f;
+};
+
 struct FkArgs {
   FkCat A, B;
   FkGrid g;
@@ -164,12 +173,7 @@ struct FkArgs {
   int nh;                     /* LDS histogram copies */
   /* LDS byte offsets */
   int off_hist, off_stab, off_ptab, off_mutab, off_edges, off_strip, off_chunk;
-  /* FP32 error model (MODE_C32) */
-  float e_s2;                 /* absolute window on s^2 (or s_perp^2)       */
-  float e_pi;                 /* absolute window on pi                      */
-  float e_rrel, e_rabs;       /* mu window: dr = r*e_rrel + inv_s*(r*e_rabs + c_rz) */
-  float e_rz;
-  float s2_tiny;              /* force recheck below this s^2                */
+  FkWin win;                  /* FP32 error model (MODE_C32) */
   int census;
   /* GRAD */
   const double *G;
@@ -243,61 +247,97 @@ __device__ void fk_strip_segments(const FkArgs &a, const FkTile &t, int dx,
 
 /* ---------------- FP64 pair evaluation (the arithmetic contract) -------- */
 
-template <int BIN>
-__device__ __forceinline__ int fk_sbin(double s2, const FkBins &bn,
+/* s^2 (or s_perp^2) bin from the LDS table, FK_OUT outside [s2min, s2max).
+ * Integer tables (setup_cf.c:385-410): index (int) s2 - (int) s2min, exactly
+ * the reference's cvtt lookup (metric_common.c:500-510); the two sentinel
+ * entries perform the range check.  Hybrid tables refine downwards against
+ * the edges (FCFC_LOOKUP_HYBRID, metric_common.c:56-64). */
+template <bool HYB>
+__device__ __forceinline__ unsigned fk_sbin(double s2, const FkBins &bn,
     const uint16_t *stab, const double *s2e) {
-  int k = (int) s2 - bn.soff + 1;            /* (int) truncates like cvtt */
-  k = k < 0 ? 0 : (k > bn.stab_len + 1 ? bn.stab_len + 1 : k);
-  int sb = stab[k];
-  if (bn.tabtype && sb >= bn.ns && sb != (int) FK_OUT) {   /* hybrid refine */
-    sb -= bn.ns;
-    while (sb != 0 && s2 < s2e[sb]) sb--;
+  if (HYB) {
+    if (!(s2 < bn.s2max) || (bn.s2min != 0 && s2 < bn.s2min)) return FK_OUT;
+    int k = (int) s2 - bn.soff + 1;
+    k = k < 1 ? 1 : (k > bn.stab_len ? bn.stab_len : k);
+    unsigned sb = stab[k];
+    if (sb >= (unsigned) bn.ns) {
+      sb -= bn.ns;
+      while (sb != 0 && s2 < s2e[sb]) sb--;
+    }
+    return sb;
   }
-  return sb;
+  unsigned k = (unsigned) ((int) s2 - bn.soff + 1);
+  k = min(k, (unsigned) (bn.stab_len + 1));
+  return stab[k];
 }
 
-__device__ __forceinline__ int fk_pbin(double p, const FkBins &bn,
+template <bool HYB>
+__device__ __forceinline__ unsigned fk_pbin(double p, const FkBins &bn,
     const uint16_t *ptab, const double *pe) {
-  int k = (int) p - bn.poff + 1;
-  k = k < 0 ? 0 : (k > bn.ptab_len + 1 ? bn.ptab_len + 1 : k);
-  int pb = ptab[k];
-  if (bn.tabtype && pb >= bn.np && pb != (int) FK_OUT) {
-    pb -= bn.np;
-    while (pb != 0 && p < pe[pb]) pb--;
+  if (HYB) {
+    if (!(p < bn.pmax) || p < bn.pmin) return FK_OUT;
+    int k = (int) p - bn.poff + 1;
+    k = k < 1 ? 1 : (k > bn.ptab_len ? bn.ptab_len : k);
+    unsigned pb = ptab[k];
+    if (pb >= (unsigned) bn.np) {
+      pb -= bn.np;
+      while (pb != 0 && p < pe[pb]) pb--;
+    }
+    return pb;
   }
-  return pb;
+  unsigned k = (unsigned) ((int) p - bn.poff + 1);
+  k = min(k, (unsigned) (bn.ptab_len + 1));
+  return ptab[k];
 }
 
-/* Box pair, B already shifted: returns the histogram index or -1. */
-template <int BIN>
-__device__ __forceinline__ int fk_pair_box64(double ax, double ay, double az,
-    double bx, double by, double bz, const FkBins &bn, const uint16_t *stab,
-    const uint16_t *ptab, const uint8_t *mutab, const double *s2e,
-    const double *pe) {
-  const double dx = bx - ax, dy = by - ay, dz = bz - az;
+/* mu bin of the reference's AVX-512 path: m = floor of the exact quotient
+ * fl(num * nmu^2) / s2 (trunc(RZ(.)), metric_common.c:484-486), m = 0 when
+ * s2 < DBL_EPSILON, pair dropped (-1) when m >= nmu^2 (mu = 1), bin =
+ * mutab[m] = isqrt(m).  The quotient comes from the hardware reciprocal
+ * plus one residual correction; only when it lies within 1e-6 of an integer
+ * is the floor settled with exact-sign fma residuals. */
+template <bool MUT>
+__device__ __forceinline__ int fk_mubin(double num, double s2, const FkBins &bn,
+    const uint8_t *mutab) {
+  const double a = num * bn.nmu2;
+  const double r = __builtin_amdgcn_rcp(s2);
+  const double q0 = a * r;
+  const double q1 = __builtin_fma(__builtin_fma(-q0, s2, a), r, q0);
+  int m = (int) q1;
+  const double f = q1 - (double) m;
+  if (f < 1e-6 || f > 1.0 - 1e-6) {
+    double mm = (double) m;
+    if (__builtin_fma(-mm, s2, a) < 0.0) mm -= 1.0;
+    else if (__builtin_fma(-(mm + 1.0), s2, a) >= 0.0) mm += 1.0;
+    m = (int) mm;
+  }
+  if (!(s2 >= 2.220446049250313e-16)) m = 0;
+  if (m >= bn.nmu * bn.nmu) return -1;
+  return MUT ? (int) mutab[m] : fk_isqrt(m);
+}
+
+/* Box pair, B already at its periodic image: histogram index or -1. */
+template <int BIN, bool HYB, bool MUT>
+__device__ __forceinline__ int fk_pair_box64(double dx, double dy, double dz,
+    const FkBins &bn, const uint16_t *stab, const uint16_t *ptab,
+    const uint8_t *mutab, const double *s2e, const double *pe) {
   if (BIN == FCFC_BIN_SPI) {
     const double pi = fabs(dz);
-    const double s2 = __builtin_fma(dy, dy, dx * dx);
-    if (!(s2 < bn.s2max) || s2 < bn.s2min || !(pi < bn.pmax) || pi < bn.pmin) return -1;
-    return fk_sbin<BIN>(s2, bn, stab, s2e) + bn.ns * fk_pbin(pi, bn, ptab, pe);
+    const double s2 = __builtin_fma(dy, dy, dx * dx);          /* :416-420 */
+    const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
+    const unsigned pb = fk_pbin<HYB>(pi, bn, ptab, pe);
+    return (sb == FK_OUT || pb == FK_OUT) ? -1 : (int) (sb + bn.ns * pb);
   }
   const double dz2 = dz * dz;
-  const double s2 = __builtin_fma(dy, dy, __builtin_fma(dx, dx, dz2));
-  if (!(s2 < bn.s2max) || s2 < bn.s2min) return -1;
-  const int sb = fk_sbin<BIN>(s2, bn, stab, s2e);
-  if (BIN == FCFC_BIN_ISO) return sb;
-  int m = 0;
-  if (s2 >= 2.220446049250313e-16) {
-    double mq = fk_floor_div(dz2 * bn.nmu2, s2);
-    if (mq >= bn.nmu2) return -1;            /* mu = 1 excluded */
-    m = (int) mq;
-  }
-  const int mb = bn.mutab_lds ? (int) mutab[m] : fk_isqrt(m);
-  return sb + bn.ns * mb;
+  const double s2 = __builtin_fma(dy, dy, __builtin_fma(dx, dx, dz2));  /* :426-430 */
+  const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
+  if (BIN == FCFC_BIN_ISO) return sb == FK_OUT ? -1 : (int) sb;
+  const int mb = fk_mubin<MUT>(dz2, s2, bn, mutab);
+  return (sb == FK_OUT || mb < 0) ? -1 : (int) sb + bn.ns * mb;
 }
 
 /* Survey pair (src/fcfc/2pt/metric_common.c:283-460, FMA vector formulas). */
-template <int BIN>
+template <int BIN, bool HYB, bool MUT>
 __device__ __forceinline__ int fk_pair_sv64(double ax, double ay, double az,
     double as, double bx, double by, double bz, double bs, const FkBins &bn,
     const uint16_t *stab, const uint16_t *ptab, const uint8_t *mutab,
@@ -305,8 +345,8 @@ __device__ __forceinline__ int fk_pair_sv64(double ax, double ay, double az,
   if (BIN == FCFC_BIN_ISO) {
     const double dx = ax - bx, dy = ay - by, dz = az - bz;
     const double s2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-    if (!(s2 < bn.s2max) || s2 < bn.s2min) return -1;
-    return fk_sbin<BIN>(s2, bn, stab, s2e);
+    const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
+    return sb == FK_OUT ? -1 : (int) sb;
   }
   const double t = __builtin_fma(az, bz, __builtin_fma(ay, by, ax * bx)) * 2.0;
   const double s = as + bs;
@@ -314,29 +354,22 @@ __device__ __forceinline__ int fk_pair_sv64(double ax, double ay, double az,
   const double p2 = (d * d) / (t + s);
   if (BIN == FCFC_BIN_SMU) {
     const double s2 = s - t;
-    if (!(s2 < bn.s2max) || s2 < bn.s2min) return -1;
-    const int sb = fk_sbin<BIN>(s2, bn, stab, s2e);
-    int m = 0;
-    if (s2 >= 2.220446049250313e-16) {
-      double mq = fk_floor_div(p2 * bn.nmu2, s2);
-      if (mq >= bn.nmu2) return -1;
-      m = (int) mq;
-    }
-    const int mb = bn.mutab_lds ? (int) mutab[m] : fk_isqrt(m);
-    return sb + bn.ns * mb;
+    const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
+    const int mb = fk_mubin<MUT>(p2, s2, bn, mutab);
+    return (sb == FK_OUT || mb < 0) ? -1 : (int) sb + bn.ns * mb;
   }
-  /* (s_perp, pi): pbin holds pi^2 edges */
-  if (!(p2 < bn.pmax) || p2 < bn.pmin) return -1;
+  /* (s_perp, pi): pbin holds pi^2 edges; with s2min == 0 the reference has
+   * no lower bound on s_perp^2 (a rounding-negative value lands in bin 0) */
   const double sp2 = (s - t) - p2;
-  if (!(sp2 < bn.s2max)) return -1;
-  if (bn.s2min != 0 && sp2 < bn.s2min) return -1;
-  return fk_sbin<BIN>(sp2, bn, stab, s2e) + bn.ns * fk_pbin(p2, bn, ptab, pe);
+  const unsigned sb = fk_sbin<HYB>(sp2, bn, stab, s2e);
+  const unsigned pb = fk_pbin<HYB>(p2, bn, ptab, pe);
+  return (sb == FK_OUT || pb == FK_OUT) ? -1 : (int) (sb + bn.ns * pb);
 }
 
 /* The other arithmetic paths of the reference, for the rounding census
- * (box only): vectorised a-side ((a+S)-b), per-pair periodic wrap, and the
- * unfused scalar remainder path.  Returns histogram index or -1. */
-template <int BIN>
+ * (box only): vectorised a-side ((a+S)-b, var 0), per-pair periodic wrap
+ * (var 2) and the unfused scalar remainder path (var 3). */
+template <int BIN, bool HYB>
 __device__ int fk_pair_box_variant(int var, const double a[3], const double bo[3],
     const int dcode[3], const FkGrid &g, const FkBins &bn, const uint16_t *stab,
     const uint16_t *ptab, const double *s2e, const double *pe) {
@@ -344,8 +377,8 @@ __device__ int fk_pair_box_variant(int var, const double a[3], const double bo[3
   double d[3];
   for (int k = 0; k < 3; k++) {
     const double D = dcode[k] * L[k];        /* b image = b + D, S = -D */
-    if (var == 0 || var == 3) d[k] = (a[k] - D) - bo[k];          /* (a + S) - b */
-    else {                                                        /* wrap */
+    if (var == 0 || var == 3) d[k] = (a[k] - D) - bo[k];
+    else {
       double v = a[k] - bo[k];
       const double h = L[k] * 0.5;
       if (v > h) v = v - L[k]; else if (v < -h) v = v + L[k];
@@ -353,38 +386,126 @@ __device__ int fk_pair_box_variant(int var, const double a[3], const double bo[3
     }
   }
   const int scalar = (var == 3);
-  double s2, dz2 = 0, pi = 0;
   if (BIN == FCFC_BIN_SPI) {
-    pi = fabs(d[2]);
-    s2 = scalar ? (d[0] * d[0] + d[1] * d[1]) : __builtin_fma(d[1], d[1], d[0] * d[0]);
-    if (!(s2 < bn.s2max) || s2 < bn.s2min || !(pi < bn.pmax) || pi < bn.pmin) return -1;
-    return fk_sbin<BIN>(s2, bn, stab, s2e) + bn.ns * fk_pbin(pi, bn, ptab, pe);
+    const double pi = fabs(d[2]);
+    const double s2 = scalar ? (d[0] * d[0] + d[1] * d[1]) : __builtin_fma(d[1], d[1], d[0] * d[0]);
+    const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
+    const unsigned pb = fk_pbin<HYB>(pi, bn, ptab, pe);
+    return (sb == FK_OUT || pb == FK_OUT) ? -1 : (int) (sb + bn.ns * pb);
   }
-  dz2 = d[2] * d[2];
+  const double dz2 = d[2] * d[2];
+  double s2;
   if (scalar) { double t = d[0] * d[0]; t = t + d[1] * d[1]; s2 = t + dz2; }
   else s2 = __builtin_fma(d[1], d[1], __builtin_fma(d[0], d[0], dz2));
-  if (!(s2 < bn.s2max) || s2 < bn.s2min) return -1;
-  const int sb = fk_sbin<BIN>(s2, bn, stab, s2e);
-  if (BIN == FCFC_BIN_ISO) return sb;
+  const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
+  if (sb == FK_OUT) return -1;
+  if (BIN == FCFC_BIN_ISO) return (int) sb;
   int m = 0;
   if (s2 >= 2.220446049250313e-16) {
     if (scalar) {
-      double q = (dz2 / s2) * bn.nmu2;
+      const double q = (dz2 / s2) * bn.nmu2;           /* :184 */
       if (q >= bn.nmu2) return -1;
       m = (int) q;
     }
     else {
-      double mq = fk_floor_div(dz2 * bn.nmu2, s2);
+      const double mq = fk_floor_div(dz2 * bn.nmu2, s2);
       if (mq >= bn.nmu2) return -1;
       m = (int) mq;
     }
   }
-  return sb + bn.ns * fk_isqrt(m);
+  return (int) sb + bn.ns * fk_isqrt(m);
+}
+
+/* ---------------- FP32 pair evaluation with exactness windows ----------- */
+/* Coordinates relative to the A tile centre in FP32.  Returns the histogram
+ * index, -1 (certainly outside) or -2 (within the rounding bound of a bin
+ * edge: the caller re-evaluates the pair in FP64). */
+template <int BIN>
+__device__ __forceinline__ int fk_pair_box32(float dx, float dy, float dz,
+    const FkBins &bn, const FkWin &wn, const uint16_t *stab,
+    const uint16_t *ptab) {
+  float s2;
+  if (BIN == FCFC_BIN_SPI) s2 = __builtin_fmaf(dy, dy, dx * dx);
+  else s2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, dz * dz));
+  /* s bin: integer part indexes the table, the fraction decides whether an
+   * edge is within the window */
+  const float fl = __builtin_floorf(s2);
+  const int kraw = (int) fl - bn.soff + 1;
+  const unsigned k = min((unsigned) kraw, (unsigned) (bn.stab_len + 1));
+  const unsigned sb = stab[k];
+  const float f = s2 - fl;
+  /* beyond the table (kraw outside [0, len+1]) the pair is out whatever the
+   * rounding: the window is far below one unit of s^2 */
+  bool near = (kraw >= 0 && kraw <= bn.stab_len + 1) && fabsf(f - 0.5f) > 0.5f - wn.e_s2;
+  if (near) {
+    const unsigned kn = (f < 0.5f) ? (k > 0 ? k - 1 : 0) : min(k + 1, (unsigned) (bn.stab_len + 1));
+    near = (stab[kn] != sb);
+  }
+  int res;
+  bool amb = near || (sb >= (unsigned) bn.ns && sb != FK_OUT);   /* hybrid entry */
+  bool out = (sb == FK_OUT);
+  if (BIN == FCFC_BIN_ISO) {
+    res = (int) sb;
+  }
+  else if (BIN == FCFC_BIN_SPI) {
+    const float pi = fabsf(dz);
+    const float fp = __builtin_floorf(pi);
+    const int kpr = (int) fp - bn.poff + 1;
+    const unsigned kp = min((unsigned) kpr, (unsigned) (bn.ptab_len + 1));
+    const unsigned pb = ptab[kp];
+    const float g = pi - fp;
+    bool nearp = (kpr >= 0 && kpr <= bn.ptab_len + 1) && fabsf(g - 0.5f) > 0.5f - wn.e_pi;
+    if (nearp) {
+      const unsigned kn = (g < 0.5f) ? (kp > 0 ? kp - 1 : 0) : min(kp + 1, (unsigned) (bn.ptab_len + 1));
+      nearp = (ptab[kn] != pb);
+    }
+    amb = amb || nearp || (pb >= (unsigned) bn.np && pb != FK_OUT);
+    out = out || (pb == FK_OUT);
+    res = (int) (sb + bn.ns * pb);
+  }
+  else {
+    /* mu bin = floor(nmu |dz| / s) = isqrt(floor(nmu^2 dz^2 / s^2)) */
+    const float inv_s = __builtin_amdgcn_rsqf(s2);
+    const float t = wn.nmu_f * fabsf(dz) * inv_s;
+    const float ft = __builtin_floorf(t);
+    const float dt = __builtin_fmaf(t, wn.e_rrel, inv_s * __builtin_fmaf(t, wn.e_rabs, wn.e_rz));
+    const bool nearm = !(fabsf(t - ft - 0.5f) <= 0.5f - dt) || s2 < wn.s2_tiny;
+    amb = amb || nearm;
+    out = out || (ft >= wn.nmu_f);
+    res = (int) sb + bn.ns * (int) ft;
+  }
+  return amb ? -2 : (out ? -1 : res);
+}
+
+/* Exact FP64 re-evaluation of a pair flagged by the FP32 windows (rare, kept
+ * out of line so that the FP32 loop stays small).  With census enabled the
+ * other rounding paths of the reference are evaluated as well and pairs
+ * whose bin depends on the path are counted. */
+template <int BIN, bool HYB, bool MUT>
+__device__ __noinline__ int fk_recheck(const FkArgs &a, double ax, double ay,
+    double az, int jg, int code, const uint16_t *stab, const uint16_t *ptab,
+    const uint8_t *mutab, const double *s2e, const double *pe,
+    unsigned int *ncensus) {
+  const FkBins &bn = a.bins;
+  const int c3[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+  const double bo[3] = {a.B.x[jg], a.B.y[jg], a.B.z[jg]};
+  const double ex = bo[0] + c3[0] * a.g.Lx;
+  const double ey = bo[1] + c3[1] * a.g.Ly;
+  const double ez = bo[2] + c3[2] * a.g.Lz;
+  const int idx = fk_pair_box64<BIN, HYB, MUT>(ex - ax, ey - ay, ez - az, bn, stab, ptab, mutab, s2e, pe);
+  if (a.census) {
+    const double av3[3] = {ax, ay, az};
+    const int iv0 = fk_pair_box_variant<BIN, HYB>(0, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
+    const int iv2 = fk_pair_box_variant<BIN, HYB>(2, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
+    const int iv3 = fk_pair_box_variant<BIN, HYB>(3, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
+    if (iv0 != idx || iv2 != idx || iv3 != idx) (*ncensus)++;
+  }
+  return idx;
 }
 
 /* ---------------------------- the kernel --------------------------------- */
 
-template <int BIN, int GEOM, bool WT, int MODE, int R>
+template <int BIN, int GEOM, bool WT, int MODE, bool HYB, bool MUT, int R>
 __global__ void __launch_bounds__(FK_NT)
 k_count(const FkArgs a) {
   extern __shared__ __attribute__((aligned(16))) char lds[];
@@ -405,7 +526,6 @@ k_count(const FkArgs a) {
   double *pe = s2e + (bn.ns + 1);
   FkStripLds *st = (FkStripLds *) (lds + a.off_strip);
   char *chunk = lds + a.off_chunk;
-  /* chunk arrays */
   double *cx = (double *) chunk, *cy = cx + FK_CH, *cz = cy + FK_CH;
   float *fx = (float *) chunk, *fy = fx + FK_CH, *fz = fy + FK_CH;
   char *after = F32 ? (char *) (fz + FK_CH) : (char *) (cz + FK_CH);
@@ -427,16 +547,17 @@ k_count(const FkArgs a) {
   if (BIN == FCFC_BIN_SPI)
     for (int i = tid; i < bn.ptab_len + 2; i += FK_NT)
       ptab[i] = (i == 0 || i == bn.ptab_len + 1) ? (uint16_t) FK_OUT : bn.ptab[i - 1];
-  if (BIN == FCFC_BIN_SMU && bn.mutab_lds) {
+  if (BIN == FCFC_BIN_SMU && MUT) {
     const int nm2 = bn.nmu * bn.nmu;
     for (int i = tid; i < nm2; i += FK_NT) mutab[i] = (uint8_t) fk_isqrt(i);
   }
-  if (bn.tabtype) {
+  if (HYB) {
     for (int i = tid; i <= bn.ns; i += FK_NT) s2e[i] = bn.s2bin[i];
     if (BIN == FCFC_BIN_SPI) for (int i = tid; i <= bn.np; i += FK_NT) pe[i] = bn.pbin[i];
   }
 
-  double evals = 0, nrecheck = 0, ncensus = 0;
+  double evals = 0;
+  unsigned int nrecheck = 0, ncensus = 0;
   double since_flush = 0;
   unsigned int *myhist_u = hist_u + (wave % a.nh) * bn.ntot;
   double *myhist_d = hist_d + (wave % a.nh) * bn.ntot;
@@ -481,7 +602,7 @@ k_count(const FkArgs a) {
       const int p = t.p0 + tid + r * FK_NT;
       av[r] = p < t.p1;
       const int q = av[r] ? p : t.p0;
-      ai[r] = p;
+      ai[r] = av[r] ? p : 0x7fffffff;
       ax[r] = a.A.x[q]; ay[r] = a.A.y[q]; az[r] = a.A.z[q];
       as[r] = NEED_S ? a.A.s[q] : 0.0;
       aw[r] = WT ? a.A.w[q] : 1.0;
@@ -490,99 +611,50 @@ k_count(const FkArgs a) {
     }
     const int nva = min(t.p1 - t.p0, R * FK_NT);
 
-    /* process one staged chunk of `cnt` B objects */
-    auto run_chunk = [&](int cnt, bool self) {
-      for (int j = 0; j < cnt; j++) {
-        double bx = 0, by = 0, bz = 0, bs = 0, bw = 1;
-        float gx = 0, gy = 0, gz = 0;
-        if (F32) { gx = fx[j]; gy = fy[j]; gz = fz[j]; }
-        else { bx = cx[j]; by = cy[j]; bz = cz[j]; }
-        if (NEED_S) bs = cs_[j];
-        if (NEED_W) bw = cw[j];
-        int jg = 0;
-        if (self || MODE == MODE_GRAD || F32) jg = cj[j];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          if (!av[r]) continue;
-          if (self && !(jg > ai[r])) continue;
-          if (MODE == MODE_GRAD && a.autoc && jg == ai[r]) continue;
-          int idx;
-          if (!F32) {
-            if (GEOM == FCFC_GEOM_BOX)
-              idx = fk_pair_box64<BIN>(ax[r], ay[r], az[r], bx, by, bz, bn, stab, ptab, mutab, s2e, pe);
-            else
-              idx = fk_pair_sv64<BIN>(ax[r], ay[r], az[r], as[r], bx, by, bz, bs, bn, stab, ptab, mutab, s2e, pe);
-          }
-          else {
-            /* ---------- FP32 fast path with exactness windows ---------- */
-            const float dx = gx - fax[r], dy = gy - fay[r], dz = gz - faz[r];
-            bool amb = false;
-            int sb, mb = 0;
-            float s2;
-            if (BIN == FCFC_BIN_SPI) s2 = __builtin_fmaf(dy, dy, dx * dx);
-            else { const float dz2 = dz * dz; s2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, dz2)); }
-            if (!(s2 < (float) bn.s2max + a.e_s2)) continue;          /* certainly out */
-            {
-              const int k = (int) s2;
-              const float f = s2 - (float) k;
-              int e = k - bn.soff + 1;
-              e = e < 0 ? 0 : (e > bn.stab_len + 1 ? bn.stab_len + 1 : e);
-              sb = stab[e];
-              if (f < a.e_s2) { int e2 = e > 0 ? e - 1 : 0; if (stab[e2] != sb) amb = true; }
-              if (f > 1.0f - a.e_s2) { int e2 = e < bn.stab_len + 1 ? e + 1 : e; if (stab[e2] != sb) amb = true; }
-              if (sb >= bn.ns && sb != (int) FK_OUT) amb = true;       /* hybrid entry */
-              if (!amb && sb == (int) FK_OUT) continue;               /* certainly out */
-            }
-            if (BIN == FCFC_BIN_SPI) {
-              const float pi = fabsf(dz);
-              const int k = (int) pi;
-              const float f = pi - (float) k;
-              int e = k - bn.poff + 1;
-              e = e < 0 ? 0 : (e > bn.ptab_len + 1 ? bn.ptab_len + 1 : e);
-              mb = ptab[e];
-              if (f < a.e_pi) { int e2 = e > 0 ? e - 1 : 0; if (ptab[e2] != mb) amb = true; }
-              if (f > 1.0f - a.e_pi) { int e2 = e < bn.ptab_len + 1 ? e + 1 : e; if (ptab[e2] != mb) amb = true; }
-              if (mb >= bn.np && mb != (int) FK_OUT) amb = true;
-              if (!amb && mb == (int) FK_OUT) continue;
-            }
-            if (BIN == FCFC_BIN_SMU) {
-              if (s2 < a.s2_tiny) amb = true;
-              const float inv_s = __builtin_amdgcn_rsqf(s2);
-              const float rr = (float) bn.nmu * fabsf(dz) * inv_s;
-              const int n = (int) rr;
-              const float fr = rr - (float) n;
-              const float dr = __builtin_fmaf(rr, a.e_rrel, inv_s * __builtin_fmaf(rr, a.e_rabs, a.e_rz));
-              if (!(fr >= dr && fr <= 1.0f - dr)) amb = true;
-              if (!amb && n >= bn.nmu) continue;                       /* mu = 1 */
-              mb = n;
-            }
-            if (!amb) idx = (BIN == FCFC_BIN_ISO) ? sb : sb + bn.ns * mb;
-            else {
-              /* exact FP64 re-evaluation of this pair */
-              nrecheck += 1;
-              const int code = cc[j];
-              const int c3[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
-              const double bo[3] = {a.B.x[jg], a.B.y[jg], a.B.z[jg]};
-              const double ex = bo[0] + c3[0] * a.g.Lx;
-              const double ey = bo[1] + c3[1] * a.g.Ly;
-              const double ez = bo[2] + c3[2] * a.g.Lz;
-              idx = fk_pair_box64<BIN>(ax[r], ay[r], az[r], ex, ey, ez, bn, stab, ptab, mutab, s2e, pe);
-              if (a.census) {
-                const double av3[3] = {ax[r], ay[r], az[r]};
-                /* a-side shift (0), per-pair wrap (2), unfused scalar (3) */
-                const int iv0 = fk_pair_box_variant<BIN>(0, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
-                const int iv2 = fk_pair_box_variant<BIN>(2, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
-                const int iv3 = fk_pair_box_variant<BIN>(3, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
-                const bool dif = (iv0 != idx) || (iv2 != idx) || (iv3 != idx);
-                if (dif) ncensus += 1;
-              }
-            }
-          }
-          if (idx < 0) continue;
-          if (MODE == MODE_GRAD) gacc[r] += hist_d[idx] * bw;
-          else if (WT) atomicAdd(myhist_d + idx, aw[r] * bw);
-          else atomicAdd(myhist_u + idx, 1u);
+    /* one pair: lane's A object r against staged B object j */
+    auto pair = [&](int r, int j, bool self) -> int {
+      int idx;
+      if (!F32) {
+        if (GEOM == FCFC_GEOM_BOX)
+          idx = fk_pair_box64<BIN, HYB, MUT>(cx[j] - ax[r], cy[j] - ay[r], cz[j] - az[r], bn, stab, ptab, mutab, s2e, pe);
+        else
+          idx = fk_pair_sv64<BIN, HYB, MUT>(ax[r], ay[r], az[r], as[r], cx[j], cy[j], cz[j],
+              NEED_S ? cs_[j] : 0.0, bn, stab, ptab, mutab, s2e, pe);
+      }
+      else {
+        idx = fk_pair_box32<BIN>(fx[j] - fax[r], fy[j] - fay[r], fz[j] - faz[r], bn, a.win, stab, ptab);
+        if (idx == -2) {
+          nrecheck++;
+          idx = fk_recheck<BIN, HYB, MUT>(a, ax[r], ay[r], az[r], cj[j], cc[j], stab, ptab,
+              mutab, s2e, pe, &ncensus);
         }
+      }
+      if (!av[r]) idx = -1;
+      if (self && !(cj[j] > ai[r])) idx = -1;
+      if (MODE == MODE_GRAD && a.autoc && cj[j] == ai[r]) idx = -1;
+      return idx;
+    };
+
+    auto accum = [&](int r, int j, int idx) {
+      if (idx < 0) return;
+      if (MODE == MODE_GRAD) gacc[r] += hist_d[idx] * cw[j];
+      else if (WT) atomicAdd(myhist_d + idx, aw[r] * cw[j]);
+      else atomicAdd(myhist_u + idx, 1u);
+    };
+
+    /* process one staged chunk of `cnt` B objects, two at a time */
+    auto run_chunk = [&](int cnt, bool self) {
+      int j = 0;
+      for (; j + 1 < cnt; j += 2) {
+        int id[2][R];
+#pragma unroll
+        for (int r = 0; r < R; r++) { id[0][r] = pair(r, j, self); id[1][r] = pair(r, j + 1, self); }
+#pragma unroll
+        for (int r = 0; r < R; r++) { accum(r, j, id[0][r]); accum(r, j + 1, id[1][r]); }
+      }
+      if (j < cnt) {
+#pragma unroll
+        for (int r = 0; r < R; r++) accum(r, j, pair(r, j, self));
       }
     };
 
@@ -627,7 +699,6 @@ k_count(const FkArgs a) {
         }
         else { b0[0] = b0[1] = b0[2] = 0; len[0] = len[1] = len[2] = 0; code[0] = code[1] = code[2] = 13; }
         for (int k = 0; k < 3; k++) { st->b0[3 * l + k] = b0[k]; st->len[3 * l + k] = len[k]; st->code[3 * l + k] = code[k]; }
-        /* inclusive wave scan */
         int inc = tot;
         for (int o = 1; o < 64; o <<= 1) {
           int v = __shfl_up(inc, o, 64);
@@ -677,14 +748,10 @@ k_count(const FkArgs a) {
   }
 
   if (MODE != MODE_GRAD) flush();
-  if (tid == 0) {
-    atomicAdd(a.stats + 0, (unsigned long long) evals);
-  }
+  if (tid == 0) atomicAdd(a.stats + 0, (unsigned long long) evals);
   if (F32) {
-    /* per-lane counters */
-    unsigned long long rc = (unsigned long long) nrecheck, ce = (unsigned long long) ncensus;
-    if (rc) atomicAdd(a.stats + 1, rc);
-    if (ce) atomicAdd(a.stats + 2, ce);
+    if (nrecheck) atomicAdd(a.stats + 1, (unsigned long long) nrecheck);
+    if (ncensus) atomicAdd(a.stats + 2, (unsigned long long) ncensus);
   }
 }
 
@@ -872,7 +939,7 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
     while (cd < 8 && objs_per_col / (cd * cd) > 256.0) cd++;
   }
   if (zd <= 0) zd = 4;
-  P->R = (mode == MODE_GRAD) ? 1 : 1;
+  P->R = (mode == MODE_GRAD) ? 1 : 2;
   P->T = c->t_tile > 0 ? c->t_tile : FK_NT * P->R;
   if (P->T > FK_NT * P->R) P->T = FK_NT * P->R;
   double w = Rxy / cd;
@@ -1080,50 +1147,66 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
 
 typedef void (*kfn_t)(const FkArgs);
 
+/* Kernel variants: binning x geometry x weights x mode x table type x mu
+ * table in LDS; two A objects per lane for counts, one for the gradient. */
 template <int BIN, int GEOM, bool WT, int MODE>
-static kfn_t pick4() { return k_count<BIN, GEOM, WT, MODE, 1>; }
+static kfn_t pick5(int hyb, int mut) {
+  constexpr int R = (MODE == MODE_GRAD) ? 1 : 2;
+  if (BIN == FCFC_BIN_SMU) {
+    if (hyb) return mut ? k_count<BIN, GEOM, WT, MODE, true, true, R> : k_count<BIN, GEOM, WT, MODE, true, false, R>;
+    return mut ? k_count<BIN, GEOM, WT, MODE, false, true, R> : k_count<BIN, GEOM, WT, MODE, false, false, R>;
+  }
+  return hyb ? k_count<BIN, GEOM, WT, MODE, true, false, R> : k_count<BIN, GEOM, WT, MODE, false, false, R>;
+}
 
 template <int BIN, int GEOM>
-static kfn_t pick3(int wt, int mode) {
-  if (mode == MODE_GRAD) return pick4<BIN, GEOM, true, MODE_GRAD>();
-  if (mode == MODE_C32 && GEOM == FCFC_GEOM_BOX)
-    return wt ? pick4<BIN, GEOM, true, MODE_C32>() : pick4<BIN, GEOM, false, MODE_C32>();
-  return wt ? pick4<BIN, GEOM, true, MODE_C64>() : pick4<BIN, GEOM, false, MODE_C64>();
+static kfn_t pick3(int wt, int mode, int hyb, int mut) {
+  if (mode == MODE_GRAD) return pick5<BIN, GEOM, true, MODE_GRAD>(hyb, mut);
+  if (GEOM == FCFC_GEOM_BOX && mode == MODE_C32)
+    return wt ? pick5<BIN, GEOM, true, (GEOM == FCFC_GEOM_BOX ? MODE_C32 : MODE_C64)>(hyb, mut)
+              : pick5<BIN, GEOM, false, (GEOM == FCFC_GEOM_BOX ? MODE_C32 : MODE_C64)>(hyb, mut);
+  return wt ? pick5<BIN, GEOM, true, MODE_C64>(hyb, mut) : pick5<BIN, GEOM, false, MODE_C64>(hyb, mut);
 }
 
-static kfn_t pick_kernel(int bintype, int geom, int wt, int mode) {
+static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int mut) {
   if (geom == FCFC_GEOM_BOX) {
-    if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_BOX>(wt, mode);
-    if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_BOX>(wt, mode);
-    return pick3<FCFC_BIN_SPI, FCFC_GEOM_BOX>(wt, mode);
+    if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_BOX>(wt, mode, hyb, mut);
+    if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_BOX>(wt, mode, hyb, mut);
+    return pick3<FCFC_BIN_SPI, FCFC_GEOM_BOX>(wt, mode, hyb, mut);
   }
-  if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_SURVEY>(wt, mode);
-  if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_SURVEY>(wt, mode);
-  return pick3<FCFC_BIN_SPI, FCFC_GEOM_SURVEY>(wt, mode);
+  if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_SURVEY>(wt, mode, hyb, mut);
+  if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_SURVEY>(wt, mode, hyb, mut);
+  return pick3<FCFC_BIN_SPI, FCFC_GEOM_SURVEY>(wt, mode, hyb, mut);
 }
 
-/* FP32 error model, see the derivation in the FP32 path of k_count. */
+/* FP32 exactness windows.  With coordinates relative to the A tile centre
+ * c, each FP32 difference satisfies |d32 - d| <= u (2|d| + E) + tiny
+ * (u = 2^-24, E = tile extent), so
+ *   |s2_32 - s2| <= 7 u s2 + 2 sqrt(3) u E s + 3 (u (2 s + E))^2,
+ * evaluated at the largest in-range s^2 (plus the first integer above it:
+ * beyond that the table answers "outside" without a window).  The mu bin
+ * floor(t), t = nmu |dz| / s, has |t32 - t| <= t 10u + (sqrt(3) u E t +
+ * nmu u E) / s.  All bounds are inflated by 25%; pairs with s^2 below
+ * s2_tiny (where second-order terms matter) are always re-evaluated. */
 static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
     FkArgs *a) {
   const double u = ldexp(1.0, -24);
-  double s2max = b->s2bin[b->ns];
-  double pmx = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
-  double Rb = sqrt(s2max + pmx * pmx) * 2.0;           /* |d| bound used */
-  double Ex = g.wx, Ey = g.wy, Ez = (zmaxc + 2) * g.wz;
-  double E = std::max(Ex, std::max(Ey, Ez));
-  double tiny = ldexp(1.0, -40) * std::max(g.Lx, std::max(g.Ly, g.Lz));
-  /* per-axis error e_k <= u (2|d_k| + E_k) + tiny */
-  double ek = u * (2 * Rb + E) + tiny;
-  /* s^2 error: sum 2|d_k| e_k + e_k^2 + 3 u s^2 */
-  double es2 = 2 * sqrt(3.0) * Rb * ek + 3 * ek * ek + 4 * u * Rb * Rb;
-  a->e_s2 = (float) (es2 * 1.25 + 1e-6);
-  a->e_pi = (float) ((2 * ek + 2 * u * Rb) * 1.25 + 1e-7);
-  double nmu = b->nmu;
-  /* mu window: dr <= r (10 u) + inv_s u E (1.74 r + nmu) + small */
-  a->e_rrel = (float) (12 * u + 1e-12);
-  a->e_rabs = (float) (1.8 * (u * E + tiny) * 1.25);
-  a->e_rz = (float) (nmu * (u * E + tiny) * 1.25 * 1.1);
-  a->s2_tiny = (float) std::max(1e-4, 1e4 * ek * ek);
+  const double s2max = b->s2bin[b->ns];
+  const double pmx = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
+  const double S2 = s2max + pmx * pmx + 2.0;
+  const double s = sqrt(S2);
+  const double E = std::max(g.wx, std::max(g.wy, (zmaxc + 1) * g.wz));
+  const double tiny = ldexp(1.0, -44) * std::max(g.Lx, std::max(g.Ly, g.Lz)) * 4;
+  const double ek = u * (2 * s + E) + tiny;
+  const double es2 = 7 * u * S2 + 2 * sqrt(3.0) * u * E * s + 3 * ek * ek + 6 * ek * tiny;
+  FkWin &w = a->win;
+  w.e_s2 = (float) (es2 * 1.25 + 1e-7);
+  w.e_pi = (float) ((u * (2 * pmx + E) + tiny) * 1.25 + 1e-7);
+  w.e_rrel = (float) (10 * u * 1.25);
+  w.e_rabs = (float) (sqrt(3.0) * u * E * 1.25 + 1e-9);
+  w.e_rz = (float) (b->nmu * (u * E + tiny) * 1.25 + 1e-9);
+  w.s2_tiny = (float) std::max(1e-6, pow(1e4 * u * E, 2));
+  w.nmu_f = (float) b->nmu;
 }
 
 /* One counting (or gradient) job on device-resident, sorted catalogues. */
@@ -1147,7 +1230,9 @@ static int run_count(fcfc_gpu_ctx *c, const fcfc_bins *b, const JobPlan &P,
     set_err(c, "binning tables need %d bytes of LDS (limit 160 KiB)", L.total);
     return FCFC_ERR_CFG;
   }
-  kfn_t fn = pick_kernel(b->bintype, geom, wt, m);
+  const int hyb = b->tabtype ? 1 : 0;
+  const int mut = (b->bintype == FCFC_BIN_SMU && b->nmu * b->nmu <= 16384) ? 1 : 0;
+  kfn_t fn = pick_kernel(b->bintype, geom, wt, m, hyb, mut);
   HIPCHK(c, hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
   /* device tables */
   uint16_t *dstab = (uint16_t *) ws(c, WS_STAB, sizeof(uint16_t) * (b->stab_len + 1), &err);

@@ -1,0 +1,145 @@
+"""Writes the golden pair-count fixtures under tests/golden/.
+
+Runs the upstream FCFC executables (built from the unmodified sources into
+oracle/_ref/ by `make -C oracle ref`) on catalogues from the project's seeded
+generator and stores, per case, the input checksum and the reference's
+outputs (counts, normalisation, bin edges, box size as written in its binary
+pair-count file).  The inputs themselves are not stored: the tests
+regenerate them from the seeds and check the checksum first.
+
+Each PAIR_COUNT runs in its own invocation: requesting DD, DR and RR in one
+run with a weighted D and an unweighted R makes the upstream binary crash.
+
+Usage:  python tests/golden/make_golden.py      (needs oracle/_ref built)
+"""
+import hashlib
+import json
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import fcfc_amd as F                     # noqa: E402  (generator only)
+from oracle import oracle as O           # noqa: E402
+
+L = 1000.0
+
+
+def checksum(*arrs):
+    h = hashlib.sha256()
+    for a in arrs:
+        if a is not None:
+            h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()
+
+
+def catalogs(kind):
+    if kind == "box_clustered":
+        x, y, z, w = F.gen_clustered(11, 3000, 30, 40, 2.0, L, 0.3)
+        xr, yr, zr = F.gen_uniform(12, 6000, L)
+        return [(x, y, z, w), (xr, yr, zr)], [True, False]
+    if kind == "box_uniform20k":
+        return [F.gen_uniform(31, 20000, L)], [False]
+    if kind == "box_tiny300":
+        return [F.gen_uniform(41, 300, L)], [False]
+    if kind == "survey":
+        x, y, z, w = F.gen_survey(21, 4000)
+        xr, yr, zr, wr = F.gen_survey(22, 6000)
+        return [(x, y, z, w), (xr, yr, zr, wr)], [True, True]
+    raise KeyError(kind)
+
+
+CASES = [
+    # name, code, catalogue kind, pairs, binning kwargs, flavours
+    ("box_iso_uniform20k", "FCFC_2PT_BOX", "box_uniform20k", ["DD"],
+     dict(bintype=0, smin=0, smax=200, ds=5), ["avx512", "scalar"]),
+    ("box_iso_tiny300", "FCFC_2PT_BOX", "box_tiny300", ["DD"],
+     dict(bintype=0, smin=0, smax=200, ds=5), ["scalar"]),
+    ("box_iso_clustered", "FCFC_2PT_BOX", "box_clustered", ["DD", "DR", "RR"],
+     dict(bintype=0, smin=0, smax=200, ds=5), ["avx512", "scalar"]),
+    ("box_smu_clustered", "FCFC_2PT_BOX", "box_clustered", ["DD", "DR", "RR"],
+     dict(bintype=1, smin=0, smax=200, ds=5, nmu=100), ["avx512", "scalar"]),
+    ("box_spi_clustered", "FCFC_2PT_BOX", "box_clustered", ["DD", "DR", "RR"],
+     dict(bintype=2, smin=0, smax=200, ds=5, pmin=0, pmax=200, dpi=5), ["avx512", "scalar"]),
+    ("box_smu_hybrid", "FCFC_2PT_BOX", "box_clustered", ["DD", "DR"],
+     dict(bintype=1, smin=0.5, smax=146.5, ds=7.3, nmu=7), ["avx512"]),
+    ("box_smu_smin", "FCFC_2PT_BOX", "box_clustered", ["DR", "RR"],
+     dict(bintype=1, smin=20, smax=150, ds=10, nmu=10), ["avx512"]),
+    ("box_spi_pmin", "FCFC_2PT_BOX", "box_clustered", ["DD", "DR"],
+     dict(bintype=2, smin=10, smax=100, ds=10, pmin=20, pmax=120, dpi=20), ["avx512"]),
+    ("survey_iso", "FCFC_2PT", "survey", ["DD", "DR"],
+     dict(bintype=0, smin=0, smax=200, ds=5), ["avx512", "scalar"]),
+    ("survey_smu", "FCFC_2PT", "survey", ["DD", "DR"],
+     dict(bintype=1, smin=0, smax=200, ds=5, nmu=100), ["avx512", "scalar"]),
+    ("survey_spi", "FCFC_2PT", "survey", ["DD", "DR"],
+     dict(bintype=2, smin=0, smax=200, ds=5, pmin=0, pmax=200, dpi=5), ["avx512", "scalar"]),
+]
+
+
+def run_case(name, code, kind, pairs, bk, flavours):
+    cats, wts = catalogs(kind)
+    out = {"name": name, "code": code, "catalogs": kind, "binning": bk,
+           "checksums": [checksum(*c) for c in cats], "results": {}}
+    for fl in flavours:
+        for p in pairs:
+            idx = sorted(set("DR".index(ch) for ch in p))
+            sub = [cats[i] for i in idx]
+            labels = ["DR"[i] for i in idx]
+            res, phase, wall, _ = O.run_reference(
+                code, fl, sub, labels, [p], bk["bintype"], bk["smin"], bk["smax"], bk["ds"],
+                nmu=bk.get("nmu"), pmin=bk.get("pmin"), pmax=bk.get("pmax"), dpi=bk.get("dpi"),
+                bsize=L if code == "FCFC_2PT_BOX" else None,
+                weights=[wts[i] for i in idx], threads=1 if fl == "scalar" else 8)
+            r = res[p]
+            ent = {"header": r["header"], "spec": r["spec"].tolist(),
+                   "sbin": r["sbin"].tolist(), "norm": float(r["norm"]),
+                   "num": r["num"].tolist(),
+                   "cnt": r["cnt"].tolist()}
+            if "bsize" in r:
+                ent["bsize"] = r["bsize"].tolist()
+            if "pbin" in r:
+                ent["pbin"] = r["pbin"].tolist()
+            out["results"][f"{fl}:{p}"] = ent
+    return out
+
+
+def main():
+    for f in [c for c in CASES]:
+        name = f[0]
+        print("case", name, flush=True)
+        data = run_case(*f)
+        with open(os.path.join(HERE, name + ".json"), "w") as fp:
+            json.dump(data, fp, separators=(",", ":"))
+    # one reference binary and one ASCII pair-count file, for the writer test
+    cats, wts = catalogs("box_clustered")
+    for ofmt, ext in ((0, "bin"), (1, "txt")):
+        tmp = os.path.join(HERE, "_tmp")
+        os.makedirs(tmp, exist_ok=True)
+        res, _, _, _ = O.run_reference("FCFC_2PT_BOX", "scalar", [cats[0], cats[1]], ["D", "R"],
+                                       ["DR"], 1, 0, 200, 5, nmu=100, bsize=L,
+                                       weights=wts, threads=1, workdir=tmp)
+        src = os.path.join(tmp, "pc_DR.bin")
+        if ofmt == 1:
+            # rerun with the ASCII output format
+            exe = O.ref_binary("FCFC_2PT_BOX", "scalar")
+            import subprocess
+            args = [exe, "-i", f"[{tmp}/cat0.txt,{tmp}/cat1.txt]", "-l", "[D,R]",
+                    "-f", '["%lf %lf %lf %lf","%lf %lf %lf"]',
+                    "-x", "[${1},${2},${3},${1},${2},${3}]", "-w", "[${4},1]",
+                    "-B", "1", "-p", "[DR]", "-P", f"[{tmp}/pc_DR.txt]",
+                    "--s-min", "0", "--s-max", "200", "--s-step", "5", "--mu-num", "100",
+                    "-b", "1000", "-F", "1", "-O", "2"]
+            subprocess.run(args, cwd=tmp, check=True, capture_output=True)
+            src = os.path.join(tmp, "pc_DR.txt")
+        shutil.copy(src, os.path.join(HERE, f"ref_box_smu_DR.{ext}"))
+        shutil.rmtree(tmp)
+    print("done")
+
+
+if __name__ == "__main__":
+    main()
