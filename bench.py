@@ -1,0 +1,203 @@
+"""Benchmark of the FCFC pair-count step on one MI355X.
+
+Workload (the headline configuration): periodic box L = 1000 Mpc/h,
+D = 2e6 clustered objects (1e6 uniform + 2000 Gaussian halos of 500 members,
+sigma = 2 Mpc/h) with lognormal(0, 0.3) weights, R = 2e7 uniform randoms;
+DD (weighted), DR (weighted, unit random weights) and RR (unweighted) in
+(s, mu) bins: 40 linear s bins in [0, 200) x 100 mu bins.  Synthetic data
+from the project's seeded generator, resident in HBM before timing.
+
+One step = the three counts (grid build + pair counting) with the FP32
+distance kernel whose pairs near a bin edge are re-evaluated in FP64.  The
+same step is then run once with the plain FP64 kernel; counts of the two are
+compared (integer counts must be identical, weighted ones within 1e-12).
+
+Prints ONE JSON line.  `value` = candidate pair-distance evaluations per
+second of the FP32 + recheck step.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pair distance evals/s and mocks/s vs plain FP64 GPU, counts bit-exact"
+# FLOPs per evaluated pair (+ - x count 1, FMA 2, division 1): box (s,mu) 10,
+# +2 for the weight product and its accumulation (weighted counts)
+FLOPS_PAIR = {"smu": 10, "smu_w": 12}
+PEAK_FP32_VALU = 157.3e12
+PEAK_FP64_VALU = 78.6e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(scale):
+    import fcfc_amd as F
+    L = 1000.0
+    nh = max(1, int(round(2000 * scale)))
+    x, y, z, w = F.gen_clustered(20261, int(1e6 * scale), nh, 500, 2.0, L, 0.3)
+    xr, yr, zr = F.gen_uniform(20262, int(2e7 * scale), L)
+    return L, (x, y, z, w), (xr, yr, zr)
+
+
+def to_device(b, cat, F, torch):
+    f = b.rescale
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a * f)).to("cuda")
+    w = torch.from_numpy(np.ascontiguousarray(cat[3])).to("cuda") if len(cat) > 3 and cat[3] is not None else None
+    return F.Catalog(b, dev(cat[0]), dev(cat[1]), dev(cat[2]), w)
+
+
+def run_step(eng, b, D, R, prec):
+    """DD, DR, RR; returns (counts, evals, kernel ms per count, rechecked, inrange)."""
+    out, ev, kms, rc = {}, 0.0, {}, 0.0
+    for name, a, c, au, wt in (("DD", D, None, True, True), ("DR", D, R, False, True),
+                               ("RR", R, None, True, False)):
+        out[name] = eng.count_pairs(a, c, b, isauto=au, withwt=wt, precision=prec)
+        s = eng.stats()
+        ev += s["evals"]
+        kms[name] = s["kernel_ms"]
+        rc += s["rechecked"]
+        out[name + "_evals"] = s["evals"]
+    return out, ev, kms, rc
+
+
+def cpu_baseline(target_s=15.0):
+    """The upstream FCFC_2PT_BOX (OpenMP + SIMD build from the unmodified
+    sources, oracle/_ref) on a bounded sample of the same workload: RR of a
+    uniform subsample, (s,mu) 40x100; count-phase time from its own progress
+    lines.  Returns in-range pairs per second."""
+    from oracle import oracle as O
+    import fcfc_amd as F
+    flav = O.host_flavour()
+    exe = O.ref_binary("FCFC_2PT_BOX", flav)
+    if not os.path.exists(exe):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    L = 1000.0
+
+    def one(n):
+        x, y, z = F.gen_uniform(777, n, L)
+        res, phase, wall, _ = O.run_reference("FCFC_2PT_BOX", flav, [(x, y, z)], ["R"], ["RR"],
+                                              1, 0, 200, 5, nmu=100, bsize=L, threads=threads)
+        # FCFC doubles auto counts: unordered in-range pairs = sum / 2
+        return float(res["RR"]["cnt"].sum()) / 2, phase["RR"], wall
+
+    n = 400000
+    pairs, t, _ = one(n)
+    rate = pairs / max(t, 1e-6)
+    n2 = int(min(8e6, max(n, n * np.sqrt(target_s / max(t, 1e-3)))))
+    if n2 > 1.3 * n:
+        pairs, t, _ = one(n2)
+        n = n2
+        rate = pairs / max(t, 1e-6)
+    return {"value": rate, "unit": "unordered in-range pairs/s", "cores": threads, "kind": "reference",
+            "sample": f"upstream FCFC_2PT_BOX ({flav} build, OpenMP {threads} threads), RR auto "
+                      f"count of {n} uniform points in the L=1000 box, (s,mu) 40x100; "
+                      f"count phase {t:.2f} s, {pairs:.4g} unordered in-range pairs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="fraction of the workload sizes (1 = the headline configuration)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this "
+              "build (the pair count would shard by cell-pair ranges; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fcfc_amd as F
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    t0 = time.time()
+    L, dcat, rcat = make_workload(args.scale)
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=100, bsize=L)
+    D = to_device(b, dcat, F, torch)
+    R = to_device(b, rcat, F, torch)
+    torch.cuda.synchronize()
+    log(f"workload ready in {time.time() - t0:.1f} s: D={D.n} R={R.n}")
+
+    eng = F.Engine(0)
+    for i in range(args.warmup):
+        run_step(eng, b, D, R, "fp32")
+        log(f"warmup step {i} done")
+    torch.cuda.synchronize()
+    t1 = time.time()
+    evals = 0.0
+    kms = {"DD": 0.0, "DR": 0.0, "RR": 0.0}
+    rechk = 0.0
+    res = None
+    for i in range(args.steps):
+        res, ev, km, rc = run_step(eng, b, D, R, "fp32")
+        evals += ev
+        rechk += rc
+        for k in kms:
+            kms[k] += km[k]
+        log(f"step {i} done")
+    torch.cuda.synchronize()
+    dt = time.time() - t1
+    value = evals / dt
+    line = {
+        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "box L=1000 Mpc/h; D=2e6 clustered (1e6 uniform + 2000 halos x 500, "
+                               "sigma=2) lognormal(0,0.3) weights; R=2e7 uniform; DD+DR+RR "
+                               "(s,mu) 40x100 in [0,200)", "scale": args.scale,
+                   "n_data": D.n, "n_random": R.n, "precision": "fp32 distances, fp64 recheck "
+                   "near bin edges, fp64 weighted sums"},
+    }
+    step_evals = evals / args.steps
+    line["evals_per_step"] = step_evals
+    line["rechecked_fraction"] = rechk / max(evals, 1)
+    # dominant kernel: RR (the largest count), FP32 path, unweighted
+    rr_ms = kms["RR"] / args.steps
+    rr_evals = res["RR_evals"]
+    ach = rr_evals * FLOPS_PAIR["smu"] / (rr_ms * 1e-3)
+    line["roofline"] = {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32_VALU / 1e12,
+                        "unit": "TFLOP/s", "frac": ach / PEAK_FP32_VALU, "traffic": None,
+                        "kernel": "k_count (s,mu) FP32+recheck, RR", "kernel_ms": rr_ms,
+                        "flops_per_pair": FLOPS_PAIR["smu"], "evals": rr_evals}
+    line["kernel_ms"] = {k: v / args.steps for k, v in kms.items()}
+
+    if not args.no_fp64:
+        t2 = time.time()
+        res64, ev64, km64, _ = run_step(eng, b, D, R, "fp64")
+        torch.cuda.synchronize()
+        dt64 = time.time() - t2
+        line["fp64"] = {"evals_per_s": ev64 / dt64, "ms_per_step": dt64 * 1e3,
+                        "kernel_ms": km64}
+        line["speedup_vs_fp64"] = (ev64 / dt64) and value / (ev64 / dt64)
+        rr_equal = bool(np.array_equal(res["RR"], res64["RR"]))
+        wrel = max(float(np.max(np.abs(res[k] - res64[k]) / np.maximum(np.abs(res64[k]), 1e-300)))
+                   for k in ("DD", "DR"))
+        line["counts_bit_exact"] = {"RR_fp32_eq_fp64": rr_equal, "weighted_max_rel": wrel}
+        # in-range pairs: exact unweighted totals of RR; DD and DR pair
+        # numbers from an unweighted FP64 count would cost another step, so
+        # the in-range rate is quoted for RR alone
+        rr_pairs = float(res64["RR"].sum())
+        line["inrange_pairs_per_s_RR"] = rr_pairs / (km64["RR"] * 1e-3)
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
+    eng.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,710 @@
+/*
+ * fcfc_cli.c -- drop-in executables FCFC_2PT_BOX_GPU (periodic box) and
+ * FCFC_2PT_GPU (survey geometry) over the C ABI of libfcfc_gpu.
+ *
+ * Mirrors the command line and configuration keys of FCFC's FCFC_2PT_BOX /
+ * FCFC_2PT (src/fcfc/2pt_box/load_conf.c:485-519, src/fcfc/2pt/load_conf.c:
+ * 521-558; CLI options override the file, like libcfg's priorities) and the
+ * flow of eval_cf (src/fcfc/2pt_box/eval_cf.c:50-430): per PAIR_COUNT read
+ * the catalogues, count pairs on the GPU, double auto counts, normalise,
+ * write the pair-count file (same binary/ASCII layout), then evaluate the
+ * CF_ESTIMATOR expressions, multipoles and projected CFs.
+ *
+ * Supported subset: ASCII catalogues (CATALOG_TYPE 0) whose POSITION /
+ * WEIGHT entries are plain column references ${n}; SELECTION and
+ * COORD_CONVERT are not supported (an error is reported).  Extra key with
+ * no counterpart in the reference: GPU_PRECISION = 0 (FP64 kernel, default)
+ * or 1 (FP32 distances with FP64 re-evaluation near bin edges); CLI --gpu-prec.
+ * Compiled twice with -DFCFC_CLI_GEOM=0 (box) or 1 (survey).
+ */
+#include <ctype.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include "fcfc_gpu.h"
+
+#ifndef FCFC_CLI_GEOM
+#define FCFC_CLI_GEOM 0
+#endif
+#if FCFC_CLI_GEOM == 0
+#define CODE_NAME "FCFC_2PT_BOX"
+#define DEFAULT_CONF "fcfc_2pt_box.conf"
+#else
+#define CODE_NAME "FCFC_2PT"
+#define DEFAULT_CONF "fcfc_2pt.conf"
+#endif
+
+#define MAXKEY 64
+#ifndef M_PI
+#define M_PI 0x1.921fb54442d18p+1
+#endif
+#define P_ERR(...) fprintf(stderr, "Error: " __VA_ARGS__)
+
+/* ---------------------------------------------------------------------- */
+/* configuration store: KEY -> raw value, with a priority (CLI > file)     */
+typedef struct { char key[32]; char *val; int prior; } kv_t;
+static kv_t kvs[MAXKEY];
+static int nkv = 0;
+
+static void kv_set(const char *key, const char *val, int prior) {
+  for (int i = 0; i < nkv; i++) {
+    if (!strcmp(kvs[i].key, key)) {
+      if (prior >= kvs[i].prior) { free(kvs[i].val); kvs[i].val = strdup(val); kvs[i].prior = prior; }
+      return;
+    }
+  }
+  if (nkv < MAXKEY) {
+    snprintf(kvs[nkv].key, sizeof kvs[nkv].key, "%s", key);
+    kvs[nkv].val = strdup(val);
+    kvs[nkv].prior = prior;
+    nkv++;
+  }
+}
+static const char *kv_get(const char *key) {
+  for (int i = 0; i < nkv; i++) if (!strcmp(kvs[i].key, key)) return kvs[i].val;
+  return NULL;
+}
+
+static char *trim(char *s) {
+  while (isspace((unsigned char) *s)) s++;
+  char *e = s + strlen(s);
+  while (e > s && isspace((unsigned char) e[-1])) *--e = '\0';
+  return s;
+}
+
+/* options of load_conf.c:485-519 (box) / 2pt/load_conf.c:521-558 */
+typedef struct { char s; const char *l; const char *key; } opt_t;
+static const opt_t OPTS[] = {
+  {'c', "conf", "CONFIG_FILE"}, {'i', "input", "CATALOG"}, {'l', "label", "CATALOG_LABEL"},
+  {'T', "type", "CATALOG_TYPE"}, {0, "skip", "ASCII_SKIP"}, {0, "comment", "ASCII_COMMENT"},
+  {'f', "formatter", "ASCII_FORMATTER"}, {'x', "position", "POSITION"}, {'w', "weight", "WEIGHT"},
+  {'s', "select", "SELECTION"}, {'b', "box", "BOX_SIZE"}, {'S', "data-struct", "DATA_STRUCT"},
+  {'B', "bin", "BINNING_SCHEME"}, {'p', "pair", "PAIR_COUNT"}, {'P', "pair-output", "PAIR_COUNT_FILE"},
+  {'e', "cf", "CF_ESTIMATOR"}, {'E', "cf-output", "CF_OUTPUT_FILE"}, {'m', "multipole", "MULTIPOLE"},
+  {'M', "mp-output", "MULTIPOLE_FILE"}, {'u', "wp", "PROJECTED_CF"}, {'U', "wp-output", "PROJECTED_FILE"},
+  {0, "s-file", "SEP_BIN_FILE"}, {0, "s-min", "SEP_BIN_MIN"}, {0, "s-max", "SEP_BIN_MAX"},
+  {0, "s-step", "SEP_BIN_SIZE"}, {0, "mu-num", "MU_BIN_NUM"}, {0, "pi-file", "PI_BIN_FILE"},
+  {0, "pi-min", "PI_BIN_MIN"}, {0, "pi-max", "PI_BIN_MAX"}, {0, "pi-step", "PI_BIN_SIZE"},
+  {'F', "out-format", "OUTPUT_FORMAT"}, {'O', "overwrite", "OVERWRITE"}, {'v', "verbose", "VERBOSE"},
+  {0, "convert", "COORD_CONVERT"}, {0, "gpu-prec", "GPU_PRECISION"},
+  {0, NULL, NULL}
+};
+
+static int read_conf_file(const char *fname, int must) {
+  FILE *fp = fopen(fname, "r");
+  if (!fp) {
+    if (must) { P_ERR("cannot open the configuration file `%s'\n", fname); return FCFC_ERR_FILE; }
+    return 0;
+  }
+  char line[8192];
+  while (fgets(line, sizeof line, fp)) {
+    char *h = strchr(line, '#');
+    if (h) *h = '\0';
+    char *eq = strchr(line, '=');
+    if (!eq) continue;
+    *eq = '\0';
+    char *k = trim(line), *v = trim(eq + 1);
+    if (*k && *v) kv_set(k, v, 1);
+  }
+  fclose(fp);
+  return 0;
+}
+
+/* split "[a, b, c]" or "a" into items; quotes removed */
+static int split_list(const char *v, char ***out) {
+  char *s = strdup(v ? v : "");
+  char *p = trim(s);
+  size_t n = strlen(p);
+  if (n && p[0] == '[' && p[n - 1] == ']') { p[n - 1] = '\0'; p++; }
+  char **items = NULL;
+  int cnt = 0;
+  char *q = p;
+  while (1) {
+    char *start = q;
+    int inq = 0; char qc = 0;
+    while (*q && (inq || *q != ',')) {
+      if (!inq && (*q == '"' || *q == '\'')) { inq = 1; qc = *q; }
+      else if (inq && *q == qc) inq = 0;
+      q++;
+    }
+    char save = *q;
+    *q = '\0';
+    char *it = trim(start);
+    size_t m = strlen(it);
+    if (m >= 2 && (it[0] == '"' || it[0] == '\'') && it[m - 1] == it[0]) { it[m - 1] = '\0'; it++; }
+    if (*it || save == ',') {
+      items = realloc(items, sizeof(char *) * (cnt + 1));
+      items[cnt++] = strdup(it);
+    }
+    if (!save) break;
+    q++;
+  }
+  free(s);
+  *out = items;
+  return cnt;
+}
+static void free_list(char **l, int n) { for (int i = 0; i < n; i++) free(l[i]); free(l); }
+
+static int get_bool(const char *key, int def) {
+  const char *v = kv_get(key);
+  if (!v) return def;
+  return (v[0] == 'T' || v[0] == 't' || v[0] == '1' || v[0] == 'Y' || v[0] == 'y');
+}
+static int get_double(const char *key, double *d) {
+  const char *v = kv_get(key);
+  if (!v) return 1;
+  char *e;
+  *d = strtod(v, &e);
+  return (e == v);
+}
+static int get_int(const char *key, int def) {
+  const char *v = kv_get(key);
+  return v ? atoi(v) : def;
+}
+
+/* ---------------------------------------------------------------------- */
+/* ASCII catalogues                                                        */
+typedef struct { int64_t n; double *x, *y, *z, *w; double sumw; int loaded; } cat_t;
+
+/* column number from "${n}" or "$n" */
+static int colref(const char *s) {
+  const char *p = s;
+  while (isspace((unsigned char) *p)) p++;
+  if (*p != '$') return -1;
+  p++;
+  int brace = (*p == '{');
+  if (brace) p++;
+  char *e;
+  long c = strtol(p, &e, 10);
+  if (e == p || c < 1) return -1;
+  if (brace) { if (*e != '}') return -1; e++; }
+  while (isspace((unsigned char) *e)) e++;
+  return *e ? -1 : (int) c;
+}
+
+/* number of conversions of a C99 formatter, and which are suppressed */
+static int fmt_fields(const char *fmt, int *supp, int max) {
+  int n = 0;
+  for (const char *p = fmt; *p; p++) {
+    if (*p != '%') continue;
+    if (p[1] == '%') { p++; continue; }
+    p++;
+    int s = 0;
+    if (*p == '*') { s = 1; p++; }
+    while (*p && !isalpha((unsigned char) *p) && *p != '[') p++;
+    while (*p == 'l' || *p == 'h' || *p == 'L' || *p == 'j' || *p == 'z' || *p == 't') p++;
+    if (*p == '[') { while (*p && *p != ']') p++; }
+    if (n < max) supp[n] = s;
+    n++;
+    if (!*p) break;
+  }
+  return n;
+}
+
+static int read_ascii(const char *fname, long skip, char cmt, const char *fmt,
+    const int col[4], int ncol, cat_t *c) {
+  FILE *fp = fopen(fname, "r");
+  if (!fp) { P_ERR("cannot open catalogue `%s'\n", fname); return FCFC_ERR_FILE; }
+  int supp[256];
+  int nf = fmt ? fmt_fields(fmt, supp, 256) : 0;
+  /* token index of each requested column */
+  int tok[4];
+  for (int k = 0; k < ncol; k++) {
+    if (nf == 0) { tok[k] = col[k] - 1; continue; }
+    int seen = 0; tok[k] = -1;
+    for (int i = 0; i < nf && i < 256; i++) {
+      if (supp[i]) continue;
+      if (++seen == col[k]) { tok[k] = i; break; }
+    }
+    if (tok[k] < 0) { P_ERR("column $%d not in ASCII_FORMATTER\n", col[k]); fclose(fp); return FCFC_ERR_CFG; }
+  }
+  size_t cap = 1 << 16;
+  c->x = malloc(sizeof(double) * cap); c->y = malloc(sizeof(double) * cap);
+  c->z = malloc(sizeof(double) * cap); c->w = (ncol > 3) ? malloc(sizeof(double) * cap) : NULL;
+  c->n = 0;
+  char *line = NULL; size_t lcap = 0;
+  long lno = 0;
+  ssize_t len;
+  while ((len = getline(&line, &lcap, fp)) >= 0) {
+    lno++;
+    if (lno <= skip) continue;
+    char *p = line;
+    while (isspace((unsigned char) *p)) p++;
+    if (!*p) continue;
+    if (cmt && *p == cmt) continue;
+    double v[4];
+    int got = 0, ti = 0;
+    char *save;
+    for (char *t = strtok_r(p, " \t\r\n,", &save); t; t = strtok_r(NULL, " \t\r\n,", &save), ti++) {
+      for (int k = 0; k < ncol; k++) if (tok[k] == ti) { v[k] = strtod(t, NULL); got++; }
+      if (got == ncol) break;
+    }
+    if (got < ncol) { P_ERR("line %ld of `%s' has too few columns\n", lno, fname); free(line); fclose(fp); return FCFC_ERR_ASCII; }
+    if ((size_t) c->n == cap) {
+      cap *= 2;
+      c->x = realloc(c->x, sizeof(double) * cap); c->y = realloc(c->y, sizeof(double) * cap);
+      c->z = realloc(c->z, sizeof(double) * cap);
+      if (c->w) c->w = realloc(c->w, sizeof(double) * cap);
+    }
+    c->x[c->n] = v[0]; c->y[c->n] = v[1]; c->z[c->n] = v[2];
+    if (c->w) c->w[c->n] = v[3];
+    c->n++;
+  }
+  free(line);
+  fclose(fp);
+  c->loaded = 1;
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* pair-count files: reader for the resume path (read_res.c:256-...)       */
+static int read_pair_file(const char *fname, const fcfc_bins *b, int weighted, void *cnt, double *norm) {
+  FILE *fp = fopen(fname, "rb");
+  if (!fp) return FCFC_ERR_FILE;
+  int ch;
+  while ((ch = fgetc(fp)) != EOF && ch != '\n') {}
+  if (ch == EOF) { fclose(fp); return FCFC_ERR_FILE; }
+  uint64_t sz, sz2;
+  int blk = 0, e = 0;
+  char *buf = NULL;
+  while (fread(&sz, sizeof sz, 1, fp) == 1) {
+    buf = realloc(buf, sz ? sz : 1);
+    if ((sz && fread(buf, sz, 1, fp) != 1) || fread(&sz2, sizeof sz2, 1, fp) != 1 || sz2 != sz) { e = FCFC_ERR_FILE; break; }
+    if (blk == 0) {
+      const int32_t *spec = (const int32_t *) buf;
+      if (spec[5] != b->bintype || spec[6] != b->ns || spec[4] != weighted) { e = FCFC_ERR_FILE; break; }
+    }
+    blk++;
+    if (sz == sizeof(double) && norm) memcpy(norm, buf, sizeof(double));
+    if (sz == (uint64_t) b->ntot * 8) memcpy(cnt, buf, sz);   /* last ntot-block: raw counts */
+  }
+  free(buf);
+  fclose(fp);
+  return e;
+}
+
+/* ---------------------------------------------------------------------- */
+/* CF estimator expressions: + - * / ^ ( ), numbers, pair labels, @@        */
+typedef struct { const char *p; const double *val; char (*lab)[3]; int npc; double rr; int err; } ex_t;
+static double ex_sum(ex_t *e);
+static double ex_atom(ex_t *e) {
+  while (isspace((unsigned char) *e->p)) e->p++;
+  if (*e->p == '(') { e->p++; double v = ex_sum(e); while (isspace((unsigned char) *e->p)) e->p++; if (*e->p == ')') e->p++; else e->err = 1; return v; }
+  if (*e->p == '-') { e->p++; return -ex_atom(e); }
+  if (*e->p == '+') { e->p++; return ex_atom(e); }
+  if (e->p[0] == '@' && e->p[1] == '@') { e->p += 2; return e->rr; }
+  if (isupper((unsigned char) e->p[0]) && isupper((unsigned char) e->p[1])) {
+    for (int i = 0; i < e->npc; i++)
+      if (e->lab[i][0] == e->p[0] && e->lab[i][1] == e->p[1]) { e->p += 2; return e->val[i]; }
+    e->err = 2; return 0;
+  }
+  char *end;
+  double v = strtod(e->p, &end);
+  if (end == e->p) { e->err = 1; return 0; }
+  e->p = end;
+  return v;
+}
+static double ex_pow(ex_t *e) {
+  double v = ex_atom(e);
+  while (isspace((unsigned char) *e->p)) e->p++;
+  if (*e->p == '^') { e->p++; return pow(v, ex_pow(e)); }
+  return v;
+}
+static double ex_prod(ex_t *e) {
+  double v = ex_pow(e);
+  for (;;) {
+    while (isspace((unsigned char) *e->p)) e->p++;
+    if (*e->p == '*') { e->p++; v *= ex_pow(e); }
+    else if (*e->p == '/') { e->p++; v /= ex_pow(e); }
+    else return v;
+  }
+}
+static double ex_sum(ex_t *e) {
+  double v = ex_prod(e);
+  for (;;) {
+    while (isspace((unsigned char) *e->p)) e->p++;
+    if (*e->p == '+') { e->p++; v += ex_prod(e); }
+    else if (*e->p == '-') { e->p++; v -= ex_prod(e); }
+    else return v;
+  }
+}
+
+static double legp(int ell, double x) {      /* src/math/legpoly.h:47-61 */
+  const double x2 = x * x;
+  switch (ell) {
+    case 0: return 1;
+    case 1: return x;
+    case 2: return 1.5 * x2 - 0.5;
+    case 3: return 2.5 * x * (x2 - 0.6);
+    case 4: return 4.375 * x2 * x2 - 3.75 * x2 + 0.375;
+    case 5: return 8.75 * x * (0.9 * x2 * x2 - x2 + 0x1.b6db6db6db6dbp-3);
+    case 6: return 14.4375 * x2 * x2 * x2 - 19.6875 * x2 * x2 + 6.5625 * x2 - 0.3125;
+    default: return 0;
+  }
+}
+
+static int file_exists(const char *f) { struct stat st; return stat(f, &st) == 0; }
+
+static void usage(void) {
+  printf("Usage: %s_GPU [OPTION]\n"
+         "Pair counts and 2-point correlation functions with the MI355X engine.\n"
+         "Options and configuration keys as %s (-c CONFIG_FILE, -i CATALOG, ...);\n"
+         "extra: --gpu-prec GPU_PRECISION (0: FP64 kernel, 1: FP32 + FP64 edge re-evaluation)\n",
+         CODE_NAME, CODE_NAME);
+}
+
+#define FAIL(code) do { ret = (code); goto done; } while (0)
+
+int main(int argc, char **argv) {
+  int ret = 0;
+  const char *conf_file = DEFAULT_CONF;
+  int conf_given = 0;
+  /* command line */
+  for (int i = 1; i < argc; i++) {
+    const char *a = argv[i];
+    if (!strcmp(a, "-h") || !strcmp(a, "--help")) { usage(); return 0; }
+    if (!strcmp(a, "-V") || !strcmp(a, "--version")) { printf("%s_GPU (FCFC v1.0.1 compatible)\n", CODE_NAME); return 0; }
+    const opt_t *o = NULL;
+    const char *val = NULL;
+    if (a[0] == '-' && a[1] == '-') {
+      const char *name = a + 2;
+      const char *eqp = strchr(name, '=');
+      size_t nl = eqp ? (size_t) (eqp - name) : strlen(name);
+      for (const opt_t *q = OPTS; q->key; q++) if (q->l && strlen(q->l) == nl && !strncmp(q->l, name, nl)) { o = q; break; }
+      if (eqp) val = eqp + 1;
+    }
+    else if (a[0] == '-' && a[1] && !a[2]) {
+      for (const opt_t *q = OPTS; q->key; q++) if (q->s == a[1]) { o = q; break; }
+    }
+    if (!o) { P_ERR("unknown option `%s'\n", a); return FCFC_ERR_ARG; }
+    if (!val) {
+      if (!strcmp(o->key, "VERBOSE") && (i + 1 >= argc || argv[i + 1][0] == '-')) val = "T";
+      else if (!strcmp(o->key, "PROJECTED_CF") && (i + 1 >= argc || argv[i + 1][0] == '-')) val = "T";
+      else if (i + 1 < argc) val = argv[++i];
+      else { P_ERR("missing value for `%s'\n", a); return FCFC_ERR_ARG; }
+    }
+    if (!strcmp(o->key, "CONFIG_FILE")) { conf_file = val; conf_given = 1; }
+    else kv_set(o->key, val, 5);
+  }
+  if (read_conf_file(conf_file, conf_given)) return FCFC_ERR_FILE;
+  const int verbose = get_bool("VERBOSE", 1);
+
+  char **cats = NULL, **labels = NULL, **pos = NULL, **wts = NULL, **fmts = NULL, **pcs = NULL,
+       **pcout = NULL, **cfs = NULL, **cfout = NULL, **mpout = NULL, **wpout = NULL, **poles = NULL,
+       **skips = NULL, **cmts = NULL, **types = NULL;
+  int ncat = 0, nlab = 0, npos = 0, nwt = 0, nfmt = 0, npc = 0, npco = 0, ncf = 0, ncfo = 0,
+      nmpo = 0, nwpo = 0, npole = 0, nskip = 0, ncmt = 0, ntype = 0;
+  fcfc_gpu_ctx *ctx = NULL;
+  fcfc_bins bins;
+  memset(&bins, 0, sizeof bins);
+  cat_t *cat = NULL;
+  double **cnt = NULL, **ncnt = NULL, *norm = NULL;
+  int *weighted = NULL;
+
+  ncat = split_list(kv_get("CATALOG"), &cats);
+  if (ncat < 1) { P_ERR("CATALOG is not set\n"); return FCFC_ERR_CFG; }
+  nlab = split_list(kv_get("CATALOG_LABEL"), &labels);
+  npos = split_list(kv_get("POSITION"), &pos);
+  nwt = split_list(kv_get("WEIGHT"), &wts);
+  nfmt = split_list(kv_get("ASCII_FORMATTER"), &fmts);
+  nskip = split_list(kv_get("ASCII_SKIP"), &skips);
+  ncmt = split_list(kv_get("ASCII_COMMENT"), &cmts);
+  ntype = split_list(kv_get("CATALOG_TYPE"), &types);
+  npc = split_list(kv_get("PAIR_COUNT"), &pcs);
+  npco = split_list(kv_get("PAIR_COUNT_FILE"), &pcout);
+  ncf = split_list(kv_get("CF_ESTIMATOR"), &cfs);
+  ncfo = split_list(kv_get("CF_OUTPUT_FILE"), &cfout);
+  npole = split_list(kv_get("MULTIPOLE"), &poles);
+  nmpo = split_list(kv_get("MULTIPOLE_FILE"), &mpout);
+  nwpo = split_list(kv_get("PROJECTED_FILE"), &wpout);
+  if (npos != 3 * ncat) { P_ERR("POSITION needs 3 entries per catalogue\n"); FAIL(FCFC_ERR_CFG); }
+  if (nfmt != ncat) { P_ERR("ASCII_FORMATTER is not set for every catalogue\n"); FAIL(FCFC_ERR_CFG); }
+  for (int i = 0; i < ntype; i++) if (atoi(types[i]) != 0) { P_ERR("only ASCII catalogues (CATALOG_TYPE 0) are supported\n"); FAIL(FCFC_ERR_CFG); }
+  if (kv_get("SELECTION")) {
+    char **sel; int ns = split_list(kv_get("SELECTION"), &sel);
+    for (int i = 0; i < ns; i++) if (sel[i][0]) { P_ERR("SELECTION is not supported by this build\n"); free_list(sel, ns); FAIL(FCFC_ERR_CFG); }
+    free_list(sel, ns);
+  }
+  if (get_bool("COORD_CONVERT", 0)) { P_ERR("COORD_CONVERT is not supported by this build\n"); FAIL(FCFC_ERR_CFG); }
+  if (npc < 1 || npco != npc) { P_ERR("PAIR_COUNT and PAIR_COUNT_FILE must be set with equal lengths\n"); FAIL(FCFC_ERR_CFG); }
+  if (ncf != ncfo) { P_ERR("CF_ESTIMATOR and CF_OUTPUT_FILE must have equal lengths\n"); FAIL(FCFC_ERR_CFG); }
+  const int bintype = get_int("BINNING_SCHEME", 0);
+  const int ofmt = get_int("OUTPUT_FORMAT", 0);
+  const int overwrite = get_int("OVERWRITE", 0);
+  const int prec = get_int("GPU_PRECISION", 0);
+  const int want_wp = get_bool("PROJECTED_CF", 0);
+
+  /* box */
+  double bsize[3] = {0, 0, 0};
+#if FCFC_CLI_GEOM == 0
+  {
+    char **bl; int nb = split_list(kv_get("BOX_SIZE"), &bl);
+    if (nb != 1 && nb != 3) { P_ERR("BOX_SIZE must have 1 or 3 values\n"); free_list(bl, nb); FAIL(FCFC_ERR_CFG); }
+    for (int k = 0; k < 3; k++) bsize[k] = atof(bl[nb == 1 ? 0 : k]);
+    free_list(bl, nb);
+  }
+#endif
+  /* bins */
+  {
+    int nmu = get_int("MU_BIN_NUM", 1);
+    const char *sf = kv_get("SEP_BIN_FILE");
+    const char *pf = kv_get("PI_BIN_FILE");
+    int e;
+    if (sf || pf) {
+      double *se = NULL, *pe = NULL; int nse = 0, npe = 0;
+      const char *files[2] = {sf, pf};
+      for (int f = 0; f < 2; f++) {
+        if (!files[f]) continue;
+        FILE *fp = fopen(files[f], "r");
+        if (!fp) { P_ERR("cannot open `%s'\n", files[f]); FAIL(FCFC_ERR_FILE); }
+        char ln[1024]; double *edges = NULL; int ne = 0;
+        while (fgets(ln, sizeof ln, fp)) {
+          char *p = trim(ln);
+          if (!*p || *p == '#') continue;
+          double lo, hi;
+          if (sscanf(p, "%lf %lf", &lo, &hi) != 2) continue;
+          edges = realloc(edges, sizeof(double) * (ne + 2));
+          if (ne == 0) edges[ne++] = lo;
+          edges[ne++] = hi;
+        }
+        fclose(fp);
+        if (f == 0) { se = edges; nse = ne; } else { pe = edges; npe = ne; }
+      }
+      if (!sf) {
+        double smin, smax, ds;
+        if (get_double("SEP_BIN_MIN", &smin) || get_double("SEP_BIN_MAX", &smax) || get_double("SEP_BIN_SIZE", &ds)) { P_ERR("separation bins not set\n"); FAIL(FCFC_ERR_CFG); }
+        nse = (int) floor((smax - smin) / ds + 0.5) + 1;
+        se = malloc(sizeof(double) * nse);
+        for (int i = 0; i < nse; i++) se[i] = smin + ds * i;
+      }
+      if (bintype == FCFC_BIN_SPI && !pf) {
+        double pmin, pmax, dpi;
+        if (get_double("PI_BIN_MIN", &pmin) || get_double("PI_BIN_MAX", &pmax) || get_double("PI_BIN_SIZE", &dpi)) { P_ERR("pi bins not set\n"); FAIL(FCFC_ERR_CFG); }
+        npe = (int) floor((pmax - pmin) / dpi + 0.5) + 1;
+        pe = malloc(sizeof(double) * npe);
+        for (int i = 0; i < npe; i++) pe[i] = pmin + dpi * i;
+      }
+      e = fcfc_bins_edges(&bins, FCFC_CLI_GEOM, bintype, se, nse - 1, nmu, pe, npe - 1,
+          FCFC_CLI_GEOM == 0 ? bsize : NULL);
+      free(se); free(pe);
+    }
+    else {
+      double smin, smax, ds, pmin = 0, pmax = 0, dpi = 1;
+      if (get_double("SEP_BIN_MIN", &smin) || get_double("SEP_BIN_MAX", &smax) || get_double("SEP_BIN_SIZE", &ds)) { P_ERR("separation bins not set\n"); FAIL(FCFC_ERR_CFG); }
+      if (bintype == FCFC_BIN_SPI && (get_double("PI_BIN_MIN", &pmin) || get_double("PI_BIN_MAX", &pmax) || get_double("PI_BIN_SIZE", &dpi))) { P_ERR("pi bins not set\n"); FAIL(FCFC_ERR_CFG); }
+      e = fcfc_bins_linear(&bins, FCFC_CLI_GEOM, bintype, smin, smax, ds, nmu, pmin, pmax, dpi,
+          FCFC_CLI_GEOM == 0 ? bsize : NULL);
+    }
+    if (e) { P_ERR("invalid binning configuration (%d)\n", e); FAIL(FCFC_ERR_CFG); }
+  }
+  double bres[3] = {bsize[0] * bins.rescale, bsize[1] * bins.rescale, bsize[2] * bins.rescale};
+
+  /* labels */
+  char lab[26];
+  for (int i = 0; i < ncat; i++) lab[i] = (i < nlab && labels[i][0]) ? labels[i][0] : (char) ('A' + i);
+  cat = calloc(ncat, sizeof(cat_t));
+  cnt = calloc(npc, sizeof(double *));
+  ncnt = calloc(npc, sizeof(double *));
+  norm = calloc(npc, sizeof(double));
+  weighted = calloc(npc, sizeof(int));
+  char (*pclab)[3] = calloc(npc, 3);
+  int *pci[2];
+  pci[0] = calloc(npc, sizeof(int)); pci[1] = calloc(npc, sizeof(int));
+  for (int i = 0; i < npc; i++) {
+    if (strlen(pcs[i]) != 2) { P_ERR("invalid PAIR_COUNT `%s'\n", pcs[i]); FAIL(FCFC_ERR_CFG); }
+    pclab[i][0] = pcs[i][0]; pclab[i][1] = pcs[i][1];
+    for (int s = 0; s < 2; s++) {
+      pci[s][i] = -1;
+      for (int k = 0; k < ncat; k++) if (lab[k] == pcs[i][s]) pci[s][i] = k;
+      if (pci[s][i] < 0) { P_ERR("catalog not found for pair count: %s\n", pcs[i]); FAIL(FCFC_ERR_CFG); }
+    }
+  }
+  int *has_wt = calloc(ncat, sizeof(int));
+  for (int k = 0; k < ncat; k++) has_wt[k] = (k < nwt && wts[k][0] && strcmp(wts[k], "1"));
+  if (fcfc_gpu_init(&ctx, 0)) { P_ERR("no usable GPU\n"); FAIL(FCFC_ERR_UNKNOWN); }
+
+  for (int i = 0; i < npc; i++) {
+    const int a = pci[0][i], b = pci[1][i], isauto = (a == b);
+    weighted[i] = has_wt[a] || has_wt[b];
+    cnt[i] = calloc(bins.ntot, sizeof(double));
+    ncnt[i] = calloc(bins.ntot, sizeof(double));
+    /* reuse an existing pair-count file (OVERWRITE 1, eval_cf.c:64-85) */
+    if (file_exists(pcout[i]) && overwrite < 2) {
+      if (overwrite == 1 && read_pair_file(pcout[i], &bins, weighted[i], cnt[i], &norm[i]) == 0) {
+        printf("Reading %s pairs ...\n", pcs[i]);
+        for (int k = 0; k < bins.ntot; k++)
+          ncnt[i][k] = (weighted[i] ? cnt[i][k] : (double) ((int64_t *) cnt[i])[k]) / norm[i];
+        continue;
+      }
+      P_ERR("output file exists: `%s' (set OVERWRITE)\n", pcout[i]);
+      FAIL(FCFC_ERR_FILE);
+    }
+    for (int s = 0; s < 2; s++) {
+      int k = pci[s][i];
+      if (cat[k].loaded) continue;
+      int col[4], nc = 3;
+      for (int d = 0; d < 3; d++) {
+        col[d] = colref(pos[3 * k + d]);
+        if (col[d] < 0) { P_ERR("POSITION `%s' must be a column reference ${n}\n", pos[3 * k + d]); FAIL(FCFC_ERR_CFG); }
+      }
+      if (has_wt[k]) {
+        col[3] = colref(wts[k]);
+        if (col[3] < 0) { P_ERR("WEIGHT `%s' must be a column reference ${n}\n", wts[k]); FAIL(FCFC_ERR_CFG); }
+        nc = 4;
+      }
+      printf("Construct the grid for catalog '%c' ...\n", lab[k]);
+      long skip = (k < nskip) ? atol(skips[k]) : 0;
+      char cm = (k < ncmt && cmts[k][0]) ? cmts[k][0] : '\0';
+      if ((ret = read_ascii(cats[k], skip, cm, fmts[k], col, nc, &cat[k]))) goto done;
+      if (verbose) printf("  Number of recorded objects: %lld\n", (long long) cat[k].n);
+    }
+    /* catalogue preparation (rescale, survey |r|^2) */
+    fcfc_cat ca, cb;
+    double *sv[2] = {NULL, NULL};
+    for (int s = 0; s < 2; s++) {
+      int k = pci[s][i];
+      cat_t *c = &cat[k];
+      if (c->loaded == 1) {
+        if (fcfc_prepare_catalog(&bins, c->n, c->x, c->y, c->z, NULL, c->w, &c->sumw)) FAIL(FCFC_ERR_DATA);
+        c->loaded = 2;   /* rescaled once */
+      }
+      if (FCFC_CLI_GEOM == 1 && bintype != FCFC_BIN_ISO) {
+        sv[s] = malloc(sizeof(double) * (c->n ? c->n : 1));
+        fcfc_bins nb = bins; nb.rescale = 1;
+        fcfc_prepare_catalog(&nb, c->n, c->x, c->y, c->z, sv[s], NULL, NULL);
+      }
+      fcfc_cat *t = s ? &cb : &ca;
+      t->n = c->n; t->x = c->x; t->y = c->y; t->z = c->z; t->s = sv[s];
+      t->w = weighted[i] ? c->w : NULL; t->on_device = 0;
+    }
+    printf("Counting %c%c pairs ...\n", pclab[i][0], pclab[i][1]);
+    fflush(stdout);
+    uint32_t flags = (isauto ? FCFC_GPU_AUTO : 0) | (weighted[i] ? FCFC_GPU_WEIGHTED : 0) |
+        (prec ? FCFC_GPU_PREC_FP32 : 0);
+    int e = fcfc_gpu_count(ctx, &ca, &cb, &bins, FCFC_CLI_GEOM == 0 ? bres : NULL, flags, cnt[i]);
+    free(sv[0]); free(sv[1]);
+    if (e) { P_ERR("GPU pair counting failed: %s\n", fcfc_gpu_last_error(ctx)); FAIL(e); }
+    /* caller conventions of eval_pairs (eval_cf.c:117-167) */
+    const cat_t *A = &cat[a], *B = &cat[b];
+    double wa = has_wt[a] ? A->sumw : (double) A->n, wb = has_wt[b] ? B->sumw : (double) B->n;
+    if (isauto) {
+      if (weighted[i]) { for (int k = 0; k < bins.ntot; k++) cnt[i][k] *= 2; norm[i] = wa * (wa - 1); }
+      else { for (int k = 0; k < bins.ntot; k++) ((int64_t *) cnt[i])[k] *= 2; norm[i] = (double) A->n * (A->n - 1); }
+    }
+    else norm[i] = weighted[i] ? wa * wb : (double) A->n * B->n;
+    for (int k = 0; k < bins.ntot; k++)
+      ncnt[i][k] = (weighted[i] ? cnt[i][k] : (double) ((int64_t *) cnt[i])[k]) / norm[i];
+    if ((e = fcfc_save_pair_count(pcout[i], &bins, ofmt, bres, isauto, weighted[i], A->n, B->n, wa, wb, cnt[i]))) {
+      P_ERR("cannot write `%s'\n", pcout[i]); FAIL(e);
+    }
+    if (verbose) printf("  Results written to file: `%s'\n", pcout[i]);
+    printf("[DONE]\n");
+  }
+
+  /* correlation functions (eval_cf.c:258-388) */
+  if (ncf) {
+    printf("Evaluate correlation function estimators ...\n");
+    double *rr = NULL;
+#if FCFC_CLI_GEOM == 0
+    rr = calloc(bins.ntot, sizeof(double));
+    if (bins.bintype == FCFC_BIN_SPI) {
+      double fac = 2 * M_PI / (bsize[0] * bsize[1] * bsize[2]);
+      for (int j = 0; j < bins.np; j++) {
+        double dpi = bins.pbin_raw[j + 1] - bins.pbin_raw[j];
+        for (int k = 0; k < bins.ns; k++)
+          rr[k + j * bins.ns] = fac * dpi * (bins.sbin_raw[k + 1] * bins.sbin_raw[k + 1] - bins.sbin_raw[k] * bins.sbin_raw[k]);
+      }
+    }
+    else {
+      double fac = 4 * M_PI / (3 * bins.nmu * bsize[0] * bsize[1] * bsize[2]);
+      for (int k = 0; k < bins.ns; k++) {
+        double v = fac * (pow(bins.sbin_raw[k + 1], 3) - pow(bins.sbin_raw[k], 3));
+        for (int j = 0; j < bins.nmu; j++) rr[k + j * bins.ns] = v;
+      }
+    }
+#endif
+    double *vals = malloc(sizeof(double) * npc);
+    double *xi = malloc(sizeof(double) * bins.ntot);
+    for (int c = 0; c < ncf; c++) {
+      for (int k = 0; k < bins.ntot; k++) {
+        for (int p = 0; p < npc; p++) vals[p] = ncnt[p][k];
+        ex_t ex = {cfs[c], vals, pclab, npc, rr ? rr[k] : NAN, 0};
+        xi[k] = ex_sum(&ex);
+        if (ex.err || *ex.p) { P_ERR("cannot evaluate CF_ESTIMATOR `%s'\n", cfs[c]); free(vals); free(xi); free(rr); FAIL(FCFC_ERR_CFG); }
+      }
+      FILE *fp = fopen(cfout[c], "w");
+      if (!fp) { P_ERR("cannot write `%s'\n", cfout[c]); free(vals); free(xi); free(rr); FAIL(FCFC_ERR_FILE); }
+      if (bins.bintype == FCFC_BIN_SMU) {
+        fprintf(fp, "# Columns: s_min(1) s_max(2) mu_min(3) mu_max(4) xi(5)\n");
+        for (int j = 0; j < bins.nmu; j++)
+          for (int k = 0; k < bins.ns; k++)
+            fprintf(fp, "%.10lg %.10lg %.10lg %.10lg %.10lg\n", bins.sbin_raw[k], bins.sbin_raw[k + 1],
+                j / (double) bins.nmu, (j + 1) / (double) bins.nmu, xi[k + j * bins.ns]);
+      }
+      else if (bins.bintype == FCFC_BIN_SPI) {
+        fprintf(fp, "# Columns: s_perp_min(1) s_perp_max(2) pi_min(3) pi_max(4) xi(5)\n");
+        for (int j = 0; j < bins.np; j++)
+          for (int k = 0; k < bins.ns; k++)
+            fprintf(fp, "%.10lg %.10lg %.10lg %.10lg %.10lg\n", bins.sbin_raw[k], bins.sbin_raw[k + 1],
+                bins.pbin_raw[j], bins.pbin_raw[j + 1], xi[k + j * bins.ns]);
+      }
+      else {
+        fprintf(fp, "# Columns: s_cen(1) s_min(2) s_max(3) xi_iso(4)\n");
+        for (int k = 0; k < bins.ns; k++)
+          fprintf(fp, "%.10lg %.10lg %.10lg %.10lg\n", (bins.sbin_raw[k] + bins.sbin_raw[k + 1]) * 0.5,
+              bins.sbin_raw[k], bins.sbin_raw[k + 1], xi[k]);
+      }
+      fclose(fp);
+      /* multipoles (eval_cf.c:326-354) */
+      if (bins.bintype == FCFC_BIN_SMU && npole && c < nmpo) {
+        fp = fopen(mpout[c], "w");
+        if (!fp) { free(vals); free(xi); free(rr); FAIL(FCFC_ERR_FILE); }
+        fprintf(fp, "# Columns: s_cen(1) s_min(2) s_max(3)");
+        for (int l = 0; l < npole; l++) fprintf(fp, " xi_%d(%d)", atoi(poles[l]), 4 + l);
+        fprintf(fp, "\n");
+        for (int k = 0; k < bins.ns; k++) {
+          fprintf(fp, "%.10lg %.10lg %.10lg", (bins.sbin_raw[k] + bins.sbin_raw[k + 1]) * 0.5,
+              bins.sbin_raw[k], bins.sbin_raw[k + 1]);
+          for (int l = 0; l < npole; l++) {
+            int ell = atoi(poles[l]);
+            double fac = (2 * ell + 1) / (double) bins.nmu, v = 0;
+            for (int j = 0; j < bins.nmu; j++) v += xi[k + j * bins.ns] * fac * legp(ell, (j + 0.5) / (double) bins.nmu);
+            fprintf(fp, " %.10lg", v);
+          }
+          fprintf(fp, "\n");
+        }
+        fclose(fp);
+      }
+      /* projected CF (eval_cf.c:365-388) */
+      if (bins.bintype == FCFC_BIN_SPI && want_wp && c < nwpo) {
+        fp = fopen(wpout[c], "w");
+        if (!fp) { free(vals); free(xi); free(rr); FAIL(FCFC_ERR_FILE); }
+        fprintf(fp, "# Columns: s_perp_cen(1) s_perp_min(2) s_perp_max(3) wp(4)\n");
+        for (int k = 0; k < bins.ns; k++) {
+          double v = 0;
+          for (int j = 0; j < bins.np; j++) v += 2 * xi[k + j * bins.ns] * (bins.pbin_raw[j + 1] - bins.pbin_raw[j]);
+          fprintf(fp, "%.10lg %.10lg %.10lg %.10lg\n", (bins.sbin_raw[k] + bins.sbin_raw[k + 1]) * 0.5,
+              bins.sbin_raw[k], bins.sbin_raw[k + 1], v);
+        }
+        fclose(fp);
+      }
+    }
+    free(vals); free(xi); free(rr);
+    printf("[DONE]\n");
+  }
+
+done:
+  if (ctx) fcfc_gpu_destroy(ctx);
+  fcfc_bins_free(&bins);
+  if (cat) for (int k = 0; k < ncat; k++) { free(cat[k].x); free(cat[k].y); free(cat[k].z); free(cat[k].w); }
+  free(cat);
+  if (cnt) for (int i = 0; i < npc; i++) { free(cnt[i]); free(ncnt[i]); }
+  free(cnt); free(ncnt); free(norm); free(weighted);
+  free_list(cats, ncat); free_list(labels, nlab); free_list(pos, npos); free_list(wts, nwt);
+  free_list(fmts, nfmt); free_list(pcs, npc); free_list(pcout, npco); free_list(cfs, ncf);
+  free_list(cfout, ncfo); free_list(mpout, nmpo); free_list(wpout, nwpo); free_list(poles, npole);
+  free_list(skips, nskip); free_list(cmts, ncmt); free_list(types, ntype);
+  return ret;
+}

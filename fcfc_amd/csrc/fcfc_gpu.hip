@@ -109,18 +109,18 @@ __global__ void k_survey_norm(int n, const double *__restrict__ x,
 /* Tiles of one row: consecutive cells, at most T objects, at most zmaxc
  * z-cells; a cell with more than T objects is split into T-sized tiles. */
 __device__ int row_tiles(const int *__restrict__ cs, int ncz, int T, int zmaxc,
-    int row, FkTile *out) {
+    int row, int job, FkTile *out) {
   int nt = 0, open = 0, tp0 = 0, tcnt = 0, tz0 = 0, tz1 = 0;
   for (int iz = 0; iz < ncz; iz++) {
     const int c = cs[iz + 1] - cs[iz];
     if (c == 0) continue;
     if (open && (tcnt + c > T || iz - tz0 + 1 > zmaxc)) {
-      if (out) out[nt] = FkTile{row, tp0, tp0 + tcnt, tz0, tz1};
+      if (out) out[nt] = FkTile{row, tp0, tp0 + tcnt, tz0, tz1, job};
       nt++; open = 0;
     }
     if (c > T) {
       for (int k = 0; k < c; k += T) {
-        if (out) out[nt] = FkTile{row, cs[iz] + k, cs[iz] + min(k + T, c), iz, iz};
+        if (out) out[nt] = FkTile{row, cs[iz] + k, cs[iz] + min(k + T, c), iz, iz, job};
         nt++;
       }
       continue;
@@ -128,19 +128,19 @@ __device__ int row_tiles(const int *__restrict__ cs, int ncz, int T, int zmaxc,
     if (!open) { open = 1; tp0 = cs[iz]; tcnt = 0; tz0 = iz; }
     tcnt += c; tz1 = iz;
   }
-  if (open) { if (out) out[nt] = FkTile{row, tp0, tp0 + tcnt, tz0, tz1}; nt++; }
+  if (open) { if (out) out[nt] = FkTile{row, tp0, tp0 + tcnt, tz0, tz1, job}; nt++; }
   return nt;
 }
 
 __global__ void k_tile_count(int nrow, int ncz, const int *__restrict__ cstart,
     int T, int zmaxc, int *__restrict__ ntile) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
-    ntile[r] = row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, nullptr);
+    ntile[r] = row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, 0, nullptr);
 }
 __global__ void k_tile_emit(int nrow, int ncz, const int *__restrict__ cstart,
-    int T, int zmaxc, const int *__restrict__ toff, FkTile *__restrict__ tiles) {
+    int T, int zmaxc, const int *__restrict__ toff, int job, FkTile *__restrict__ tiles) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
-    row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, tiles + toff[r]);
+    row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, job, tiles + toff[r]);
 }
 
 /* ======================================================================== */
@@ -156,16 +156,28 @@ struct FkWin {
   float nmu_f;
 };
 
-struct FkArgs {
+/* One counting job: a catalogue pair and its histogram.  A launch counts the
+ * tiles of one or many jobs (batched mocks); tiles carry their job index. */
+struct FkJob {
   FkCat A, B;
   FkGrid g;
-  FkBins bins;
-  const FkTile *tiles;
-  int ntiles;
   const int *strips;          /* (dx, dy, diag) triples */
   int nstrips;
   int autoc;                  /* COUNT: A == B, half shell + self tile;
                                  GRAD: A == B, exclude j == i          */
+  int wt;                     /* weighted histogram (double)          */
+  int pad;
+  long long hist_off;         /* histogram of this job: gcnt + hist_off */
+  double *gA;                 /* GRAD: output, original object order  */
+  FkWin win;                  /* FP32 error model (MODE_C32)          */
+};
+
+struct FkArgs {
+  const FkJob *jobs;
+  int njobs;
+  FkBins bins;
+  const FkTile *tiles;
+  int ntiles;
   unsigned int *queue;
   unsigned long long *gcnt_u;
   double *gcnt_d;
@@ -173,11 +185,8 @@ struct FkArgs {
   int nh;                     /* LDS histogram copies */
   /* LDS byte offsets */
   int off_hist, off_stab, off_ptab, off_mutab, off_edges, off_strip, off_chunk;
-  FkWin win;                  /* FP32 error model (MODE_C32) */
   int census;
-  /* GRAD */
-  const double *G;
-  double *gA;
+  const double *G;            /* GRAD: dL/dcnt */
 };
 
 /* LDS strip table */
@@ -189,9 +198,9 @@ struct FkStripLds {
 };
 
 /* Compute the (up to three) B segments of one strip for tile t. */
-__device__ void fk_strip_segments(const FkArgs &a, const FkTile &t, int dx,
+__device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
     int dy, int diag, int *b0, int *len, int *code) {
-  const FkGrid &g = a.g;
+  const FkGrid &g = J.g;
   b0[0] = b0[1] = b0[2] = 0;
   len[0] = len[1] = len[2] = 0;
   code[0] = code[1] = code[2] = 13;
@@ -210,7 +219,7 @@ __device__ void fk_strip_segments(const FkArgs &a, const FkTile &t, int dx,
   const double zc0 = (double) t.iz0 - hz / g.wz - 1e-6;
   const double zc1 = (double) (t.iz1 + 1) + hz / g.wz + 1e-6;
   int izlo = (int) floor(zc0), izhi = (int) floor(zc1);
-  const int *cs = a.B.cell_start + (size_t) (bx * g.ncy + by) * g.ncz;
+  const int *cs = J.B.cell_start + (size_t) (bx * g.ncy + by) * g.ncz;
   const int cxy = (cx + 1) + 3 * (cy + 1);
   int ns = 0;
   if (g.periodic) {
@@ -482,22 +491,21 @@ __device__ __forceinline__ int fk_pair_box32(float dx, float dy, float dz,
  * other rounding paths of the reference are evaluated as well and pairs
  * whose bin depends on the path are counted. */
 template <int BIN, bool HYB, bool MUT>
-__device__ __noinline__ int fk_recheck(const FkArgs &a, double ax, double ay,
-    double az, int jg, int code, const uint16_t *stab, const uint16_t *ptab,
-    const uint8_t *mutab, const double *s2e, const double *pe,
+__device__ __noinline__ int fk_recheck(const FkBins &bn, const FkJob &J, int census,
+    double ax, double ay, double az, int jg, int code, const uint16_t *stab,
+    const uint16_t *ptab, const uint8_t *mutab, const double *s2e, const double *pe,
     unsigned int *ncensus) {
-  const FkBins &bn = a.bins;
   const int c3[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
-  const double bo[3] = {a.B.x[jg], a.B.y[jg], a.B.z[jg]};
-  const double ex = bo[0] + c3[0] * a.g.Lx;
-  const double ey = bo[1] + c3[1] * a.g.Ly;
-  const double ez = bo[2] + c3[2] * a.g.Lz;
+  const double bo[3] = {J.B.x[jg], J.B.y[jg], J.B.z[jg]};
+  const double ex = bo[0] + c3[0] * J.g.Lx;
+  const double ey = bo[1] + c3[1] * J.g.Ly;
+  const double ez = bo[2] + c3[2] * J.g.Lz;
   const int idx = fk_pair_box64<BIN, HYB, MUT>(ex - ax, ey - ay, ez - az, bn, stab, ptab, mutab, s2e, pe);
-  if (a.census) {
+  if (census) {
     const double av3[3] = {ax, ay, az};
-    const int iv0 = fk_pair_box_variant<BIN, HYB>(0, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
-    const int iv2 = fk_pair_box_variant<BIN, HYB>(2, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
-    const int iv3 = fk_pair_box_variant<BIN, HYB>(3, av3, bo, c3, a.g, bn, stab, ptab, s2e, pe);
+    const int iv0 = fk_pair_box_variant<BIN, HYB>(0, av3, bo, c3, J.g, bn, stab, ptab, s2e, pe);
+    const int iv2 = fk_pair_box_variant<BIN, HYB>(2, av3, bo, c3, J.g, bn, stab, ptab, s2e, pe);
+    const int iv3 = fk_pair_box_variant<BIN, HYB>(3, av3, bo, c3, J.g, bn, stab, ptab, s2e, pe);
     if (iv0 != idx || iv2 != idx || iv3 != idx) (*ncensus)++;
   }
   return idx;
@@ -540,7 +548,7 @@ k_count(const FkArgs a) {
   if (MODE == MODE_GRAD) {
     for (int i = tid; i < bn.ntot; i += FK_NT) hist_d[i] = a.G[i];
   }
-  else if (WT) { for (int i = tid; i < hist_elems; i += FK_NT) hist_d[i] = 0.0; }
+  else if (WT) { for (int i = tid; i < hist_elems; i += FK_NT) hist_d[i] = 0.0; }   /* covers the u32 view */
   else { for (int i = tid; i < hist_elems; i += FK_NT) hist_u[i] = 0u; }
   for (int i = tid; i < bn.stab_len + 2; i += FK_NT)
     stab[i] = (i == 0 || i == bn.stab_len + 1) ? (uint16_t) FK_OUT : bn.stab[i - 1];
@@ -561,19 +569,26 @@ k_count(const FkArgs a) {
   double since_flush = 0;
   unsigned int *myhist_u = hist_u + (wave % a.nh) * bn.ntot;
   double *myhist_d = hist_d + (wave % a.nh) * bn.ntot;
+  __shared__ FkJob sJ;
+  const FkJob &J = sJ;
+  int cur = -1;
 
+  /* add the LDS histograms of the current job to its global histogram */
   auto flush = [&]() {
     __syncthreads();
-    for (int i = tid; i < bn.ntot; i += FK_NT) {
-      if (WT) {
-        double v = 0;
-        for (int c = 0; c < a.nh; c++) { v += hist_d[c * bn.ntot + i]; hist_d[c * bn.ntot + i] = 0.0; }
-        if (v != 0) unsafeAtomicAdd(a.gcnt_d + i, v);
-      }
-      else {
-        unsigned long long v = 0;
-        for (int c = 0; c < a.nh; c++) { v += hist_u[c * bn.ntot + i]; hist_u[c * bn.ntot + i] = 0u; }
-        if (v) atomicAdd(a.gcnt_u + i, v);
+    if (cur >= 0) {
+      const bool w = WT && J.wt;
+      for (int i = tid; i < bn.ntot; i += FK_NT) {
+        if (w) {
+          double v = 0;
+          for (int c = 0; c < a.nh; c++) { v += hist_d[c * bn.ntot + i]; hist_d[c * bn.ntot + i] = 0.0; }
+          if (v != 0) unsafeAtomicAdd(a.gcnt_d + J.hist_off + i, v);
+        }
+        else {
+          unsigned long long v = 0;
+          for (int c = 0; c < a.nh; c++) { v += hist_u[c * bn.ntot + i]; hist_u[c * bn.ntot + i] = 0u; }
+          if (v) atomicAdd(a.gcnt_u + J.hist_off + i, v);
+        }
       }
     }
     __syncthreads();
@@ -586,7 +601,21 @@ k_count(const FkArgs a) {
     const int tix = s_tile;
     if (tix >= a.ntiles) break;
     const FkTile t = a.tiles[tix];
-    const int ix = t.row / a.g.ncy, iy = t.row - ix * a.g.ncy;
+    if (t.job != cur) {
+      /* switch job: flush the histogram of the previous one, stage the new
+       * job descriptor in LDS */
+      if (MODE != MODE_GRAD) flush();
+      __syncthreads();
+      {
+        const int *src = (const int *) (a.jobs + t.job);
+        int *dst = (int *) &sJ;
+        for (int i = tid; i < (int) (sizeof(FkJob) / sizeof(int)); i += FK_NT) dst[i] = src[i];
+      }
+      __syncthreads();
+      cur = t.job;
+      since_flush = 0;
+    }
+    const int ix = t.row / J.g.ncy, iy = t.row - ix * J.g.ncy;
 
     /* A objects of this lane */
     double ax[R], ay[R], az[R], as[R], aw[R], gacc[R];
@@ -594,18 +623,18 @@ k_count(const FkArgs a) {
     int ai[R];
     bool av[R];
     /* tile centre for the FP32 relative coordinates */
-    const double tcx = a.g.ox + (ix + 0.5) * a.g.wx;
-    const double tcy = a.g.oy + (iy + 0.5) * a.g.wy;
-    const double tcz = a.g.oz + 0.5 * (t.iz0 + t.iz1 + 1) * a.g.wz;
+    const double tcx = J.g.ox + (ix + 0.5) * J.g.wx;
+    const double tcy = J.g.oy + (iy + 0.5) * J.g.wy;
+    const double tcz = J.g.oz + 0.5 * (t.iz0 + t.iz1 + 1) * J.g.wz;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int p = t.p0 + tid + r * FK_NT;
       av[r] = p < t.p1;
       const int q = av[r] ? p : t.p0;
       ai[r] = av[r] ? p : 0x7fffffff;
-      ax[r] = a.A.x[q]; ay[r] = a.A.y[q]; az[r] = a.A.z[q];
-      as[r] = NEED_S ? a.A.s[q] : 0.0;
-      aw[r] = WT ? a.A.w[q] : 1.0;
+      ax[r] = J.A.x[q]; ay[r] = J.A.y[q]; az[r] = J.A.z[q];
+      as[r] = NEED_S ? J.A.s[q] : 0.0;
+      aw[r] = WT ? J.A.w[q] : 1.0;
       gacc[r] = 0.0;
       fax[r] = (float) (ax[r] - tcx); fay[r] = (float) (ay[r] - tcy); faz[r] = (float) (az[r] - tcz);
     }
@@ -622,23 +651,23 @@ k_count(const FkArgs a) {
               NEED_S ? cs_[j] : 0.0, bn, stab, ptab, mutab, s2e, pe);
       }
       else {
-        idx = fk_pair_box32<BIN>(fx[j] - fax[r], fy[j] - fay[r], fz[j] - faz[r], bn, a.win, stab, ptab);
+        idx = fk_pair_box32<BIN>(fx[j] - fax[r], fy[j] - fay[r], fz[j] - faz[r], bn, J.win, stab, ptab);
         if (idx == -2) {
           nrecheck++;
-          idx = fk_recheck<BIN, HYB, MUT>(a, ax[r], ay[r], az[r], cj[j], cc[j], stab, ptab,
-              mutab, s2e, pe, &ncensus);
+          idx = fk_recheck<BIN, HYB, MUT>(bn, J, a.census, ax[r], ay[r], az[r], cj[j], cc[j],
+              stab, ptab, mutab, s2e, pe, &ncensus);
         }
       }
       if (!av[r]) idx = -1;
       if (self && !(cj[j] > ai[r])) idx = -1;
-      if (MODE == MODE_GRAD && a.autoc && cj[j] == ai[r]) idx = -1;
+      if (MODE == MODE_GRAD && J.autoc && cj[j] == ai[r]) idx = -1;
       return idx;
     };
 
     auto accum = [&](int r, int j, int idx) {
       if (idx < 0) return;
       if (MODE == MODE_GRAD) gacc[r] += hist_d[idx] * cw[j];
-      else if (WT) atomicAdd(myhist_d + idx, aw[r] * cw[j]);
+      else if (WT && J.wt) atomicAdd(myhist_d + idx, aw[r] * cw[j]);
       else atomicAdd(myhist_u + idx, 1u);
     };
 
@@ -659,23 +688,23 @@ k_count(const FkArgs a) {
     };
 
     auto guard = [&](int cnt) {
-      if (MODE == MODE_GRAD || WT) return;
+      if (MODE == MODE_GRAD || (WT && J.wt)) return;
       since_flush += (double) cnt * nva;
       if (since_flush > 2.0e9) { flush(); since_flush = 0; }
     };
 
     /* ---- auto counts: pairs inside the tile (j > i) ---- */
-    if (MODE != MODE_GRAD && a.autoc) {
+    if (MODE != MODE_GRAD && J.autoc) {
       for (int c0 = t.p0; c0 < t.p1; c0 += FK_CH) {
         const int cnt = min(FK_CH, t.p1 - c0);
         __syncthreads();
         for (int q = tid; q < cnt; q += FK_NT) {
           const int jg = c0 + q;
-          const double x = a.A.x[jg], y = a.A.y[jg], z = a.A.z[jg];
+          const double x = J.A.x[jg], y = J.A.y[jg], z = J.A.z[jg];
           if (F32) { fx[q] = (float) (x - tcx); fy[q] = (float) (y - tcy); fz[q] = (float) (z - tcz); cc[q] = 13; }
           else { cx[q] = x; cy[q] = y; cz[q] = z; }
-          if (NEED_S) cs_[q] = a.A.s[jg];
-          if (NEED_W) cw[q] = a.A.w[jg];
+          if (NEED_S) cs_[q] = J.A.s[jg];
+          if (NEED_W) cw[q] = J.A.w[jg];
           cj[q] = jg;
         }
         __syncthreads();
@@ -686,15 +715,15 @@ k_count(const FkArgs a) {
     }
 
     /* ---- strips in batches of one wavefront ---- */
-    for (int sb0 = 0; sb0 < a.nstrips; sb0 += FK_NSB) {
+    for (int sb0 = 0; sb0 < J.nstrips; sb0 += FK_NSB) {
       __syncthreads();
       if (wave == 0) {
         const int l = tid;
         int tot = 0;
         int b0[3], len[3], code[3];
-        if (sb0 + l < a.nstrips) {
-          const int *sp = a.strips + 3 * (sb0 + l);
-          fk_strip_segments(a, t, sp[0], sp[1], sp[2], b0, len, code);
+        if (sb0 + l < J.nstrips) {
+          const int *sp = J.strips + 3 * (sb0 + l);
+          fk_strip_segments(J, t, sp[0], sp[1], sp[2], b0, len, code);
           tot = len[0] + len[1] + len[2];
         }
         else { b0[0] = b0[1] = b0[2] = 0; len[0] = len[1] = len[2] = 0; code[0] = code[1] = code[2] = 13; }
@@ -721,16 +750,16 @@ k_count(const FkArgs a) {
           if (rem >= st->len[sgi]) { rem -= st->len[sgi]; sgi++; if (rem >= st->len[sgi]) { rem -= st->len[sgi]; sgi++; } }
           const int jg = st->b0[sgi] + rem;
           const int code = st->code[sgi];
-          double x = a.B.x[jg], y = a.B.y[jg], z = a.B.z[jg];
+          double x = J.B.x[jg], y = J.B.y[jg], z = J.B.z[jg];
           if (GEOM == FCFC_GEOM_BOX && code != 13) {
-            x = x + (double) (code % 3 - 1) * a.g.Lx;
-            y = y + (double) ((code / 3) % 3 - 1) * a.g.Ly;
-            z = z + (double) (code / 9 - 1) * a.g.Lz;
+            x = x + (double) (code % 3 - 1) * J.g.Lx;
+            y = y + (double) ((code / 3) % 3 - 1) * J.g.Ly;
+            z = z + (double) (code / 9 - 1) * J.g.Lz;
           }
           if (F32) { fx[q] = (float) (x - tcx); fy[q] = (float) (y - tcy); fz[q] = (float) (z - tcz); cc[q] = code; }
           else { cx[q] = x; cy[q] = y; cz[q] = z; }
-          if (NEED_S) cs_[q] = a.B.s[jg];
-          if (NEED_W) cw[q] = a.B.w[jg];
+          if (NEED_S) cs_[q] = J.B.s[jg];
+          if (NEED_W) cw[q] = J.B.w[jg];
           cj[q] = jg;
         }
         __syncthreads();
@@ -743,7 +772,7 @@ k_count(const FkArgs a) {
     if (MODE == MODE_GRAD) {
 #pragma unroll
       for (int r = 0; r < R; r++)
-        if (av[r]) a.gA[a.A.idx[ai[r]]] += gacc[r];
+        if (av[r]) J.gA[J.A.idx[ai[r]]] += gacc[r];
     }
   }
 
@@ -779,6 +808,7 @@ struct fcfc_gpu_ctx {
   char err[512] = {0};
   fcfc_gpu_stats stats{};
   int t_cell_div = 0, t_tile = 0, t_zdiv = 0, t_nblock = 0;
+  int census = 0;
   std::vector<DevBuf> bufs;
 };
 
@@ -800,7 +830,7 @@ enum {
   WS_A_X, WS_A_Y, WS_A_Z, WS_A_S, WS_A_W, WS_A_IDX, WS_A_CS,
   WS_B_X, WS_B_Y, WS_B_Z, WS_B_S, WS_B_W, WS_B_IDX, WS_B_CS,
   WS_CID, WS_RANK, WS_COUNT, WS_ROWN, WS_ROWOFF, WS_TILES,
-  WS_STRIPS, WS_STAB, WS_PTAB, WS_S2E, WS_PE, WS_HIST, WS_MISC, WS_G, WS_GA,
+  WS_STRIPS, WS_STAB, WS_PTAB, WS_S2E, WS_PE, WS_HIST, WS_MISC, WS_G, WS_GA, WS_GB, WS_JOBS,
   WS_NSLOT
 };
 
@@ -1013,41 +1043,6 @@ struct DevCat {
   int *idx = nullptr, *cs = nullptr;
 };
 
-static int upload(fcfc_gpu_ctx *c, const fcfc_cat *cat, int base, int need_s,
-    int need_w, const double **px, const double **py, const double **pz,
-    const double **ps, const double **pw) {
-  int err = 0;
-  const size_t nb = sizeof(double) * (size_t) cat->n;
-  if (cat->on_device) {
-    *px = cat->x; *py = cat->y; *pz = cat->z;
-    *ps = need_s ? cat->s : nullptr;
-    *pw = need_w ? cat->w : nullptr;
-    return 0;
-  }
-  double *dx = (double *) ws(c, base + 0, nb, &err);
-  double *dy = (double *) ws(c, base + 1, nb, &err);
-  double *dz = (double *) ws(c, base + 2, nb, &err);
-  if (err) return err;
-  HIPCHK(c, hipMemcpyAsync(dx, cat->x, nb, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(dy, cat->y, nb, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(dz, cat->z, nb, hipMemcpyHostToDevice, c->stream));
-  *px = dx; *py = dy; *pz = dz;
-  *ps = nullptr; *pw = nullptr;
-  if (need_s && cat->s) {
-    double *ds = (double *) ws(c, base + 3, nb, &err);
-    if (err) return err;
-    HIPCHK(c, hipMemcpyAsync(ds, cat->s, nb, hipMemcpyHostToDevice, c->stream));
-    *ps = ds;
-  }
-  if (need_w && cat->w) {
-    double *dw = (double *) ws(c, base + 4, nb, &err);
-    if (err) return err;
-    HIPCHK(c, hipMemcpyAsync(dw, cat->w, nb, hipMemcpyHostToDevice, c->stream));
-    *pw = dw;
-  }
-  return 0;
-}
-
 static inline int nblk(int64_t n, int bs) {
   int64_t b = (n + bs - 1) / bs;
   if (b < 1) b = 1;
@@ -1055,33 +1050,134 @@ static inline int nblk(int64_t n, int bs) {
   return (int) b;
 }
 
-/* sort one catalogue into the grid */
-static int build_grid(fcfc_gpu_ctx *c, const FkGrid &g, int n, const double *x,
-    const double *y, const double *z, const double *s, const double *w,
-    int need_s, int slot, DevCat *out) {
+/* Device memory of one call: the context's reusable workspace slots for a
+ * single job, or allocations owned by the call (freed at its end) when many
+ * jobs need their own sorted catalogues. */
+struct Alloc {
+  fcfc_gpu_ctx *c;
+  bool own;
+  std::vector<void *> blocks;
   int err = 0;
+  Alloc(fcfc_gpu_ctx *c_, bool own_) : c(c_), own(own_) {}
+  ~Alloc() { for (void *p : blocks) (void) hipFree(p); }
+  void *get(int slot, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (!own) return ws(c, slot, bytes, &err);
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      set_err(c, "hipMalloc of %zu bytes failed", bytes);
+      err = FCFC_ERR_MEMORY;
+      return nullptr;
+    }
+    blocks.push_back(p);
+    return p;
+  }
+};
+
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_minmax(int n, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, double *out) {
+  __shared__ double sm[6][256];
+  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v[0] = fmin(v[0], x[i]); v[1] = fmin(v[1], y[i]); v[2] = fmin(v[2], z[i]);
+    v[3] = fmax(v[3], x[i]); v[4] = fmax(v[4], y[i]); v[5] = fmax(v[5], z[i]);
+  }
+  for (int k = 0; k < 6; k++) sm[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int) threadIdx.x < s)
+      for (int k = 0; k < 6; k++)
+        sm[k][threadIdx.x] = (k < 3) ? fmin(sm[k][threadIdx.x], sm[k][threadIdx.x + s])
+                                     : fmax(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = 0;   /* unused slot */
+  if (threadIdx.x < 6) out[8 + blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+
+
+static int bbox(fcfc_gpu_ctx *c, const fcfc_cat *k, double lo[3], double hi[3]) {
+  if (k->n <= 0) return 0;
+  if (!k->on_device) {
+    for (int64_t i = 0; i < k->n; i++) {
+      const double v[3] = {k->x[i], k->y[i], k->z[i]};
+      for (int d = 0; d < 3; d++) { lo[d] = fmin(lo[d], v[d]); hi[d] = fmax(hi[d], v[d]); }
+    }
+    return 0;
+  }
+  int err = 0;
+  const int nbk = 64;
+  double *d = (double *) ws(c, WS_MISC, sizeof(double) * (8 + 6 * nbk), &err);
+  if (err) return err;
+  hipLaunchKernelGGL(k_minmax, dim3(nbk), dim3(256), 0, c->stream, (int) k->n, k->x, k->y, k->z, d);
+  std::vector<double> h(8 + 6 * nbk);
+  HIPCHK(c, hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int bk = 0; bk < nbk; bk++)
+    for (int q = 0; q < 3; q++) {
+      lo[q] = fmin(lo[q], h[8 + bk * 6 + q]);
+      hi[q] = fmax(hi[q], h[8 + bk * 6 + 3 + q]);
+    }
+  return 0;
+}
+
+/* Sort one catalogue into the grid of a job (cell counting sort), with the
+ * survey |r|^2 and unit weights filled in where needed.  slot: first of 7
+ * consecutive workspace slots (single-job calls). */
+static int stage_cat(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, int need_s,
+    int wt, const FkGrid &g, int slot, int upslot, DevCat *out) {
+  const int n = (int) k->n;
   const size_t nb = sizeof(double) * (size_t) (n > 0 ? n : 1);
   const int ncell = g.ncx * g.ncy * g.ncz;
+  int err = 0;
+  /* inputs on the device */
+  const double *x = k->x, *y = k->y, *z = k->z, *s = need_s ? k->s : nullptr, *w = wt ? k->w : nullptr;
+  if (!k->on_device && n > 0) {
+    double *dx = (double *) al.get(upslot + 0, nb), *dy = (double *) al.get(upslot + 1, nb),
+           *dz = (double *) al.get(upslot + 2, nb);
+    if (al.err) return al.err;
+    HIPCHK(c, hipMemcpyAsync(dx, k->x, nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dy, k->y, nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dz, k->z, nb, hipMemcpyHostToDevice, c->stream));
+    x = dx; y = dy; z = dz;
+    if (s) {
+      double *ds = (double *) al.get(upslot + 3, nb);
+      if (al.err) return al.err;
+      HIPCHK(c, hipMemcpyAsync(ds, k->s, nb, hipMemcpyHostToDevice, c->stream));
+      s = ds;
+    }
+    if (w) {
+      double *dw = (double *) al.get(upslot + 4, nb);
+      if (al.err) return al.err;
+      HIPCHK(c, hipMemcpyAsync(dw, k->w, nb, hipMemcpyHostToDevice, c->stream));
+      w = dw;
+    }
+  }
   out->n = n;
-  out->x = (double *) ws(c, slot + 0, nb, &err);
-  out->y = (double *) ws(c, slot + 1, nb, &err);
-  out->z = (double *) ws(c, slot + 2, nb, &err);
-  out->s = need_s ? (double *) ws(c, slot + 3, nb, &err) : nullptr;
-  out->w = w ? (double *) ws(c, slot + 4, nb, &err) : nullptr;
-  out->idx = (int *) ws(c, slot + 5, sizeof(int) * (size_t) (n > 0 ? n : 1), &err);
-  out->cs = (int *) ws(c, slot + 6, sizeof(int) * ((size_t) ncell + 1), &err);
+  out->x = (double *) al.get(slot + 0, nb);
+  out->y = (double *) al.get(slot + 1, nb);
+  out->z = (double *) al.get(slot + 2, nb);
+  out->s = need_s ? (double *) al.get(slot + 3, nb) : nullptr;
+  out->w = wt ? (double *) al.get(slot + 4, nb) : nullptr;
+  out->idx = (int *) al.get(slot + 5, sizeof(int) * (size_t) (n > 0 ? n : 1));
+  out->cs = (int *) al.get(slot + 6, sizeof(int) * ((size_t) ncell + 1));
   int *cid = (int *) ws(c, WS_CID, sizeof(int) * (size_t) (n > 0 ? n : 1), &err);
   int *rank = (int *) ws(c, WS_RANK, sizeof(int) * (size_t) (n > 0 ? n : 1), &err);
   int *count = (int *) ws(c, WS_COUNT, sizeof(int) * (size_t) ncell, &err);
+  if (al.err) return al.err;
   if (err) return err;
-  const double *sin = s;
+  double *stmp = nullptr;
   if (need_s && !s && n > 0) {
-    /* compute |r|^2 into the rank buffer's neighbour: use the output slot
-     * temporarily, then scatter from a dedicated copy */
-    double *tmp = (double *) ws(c, WS_IN_S + (slot == WS_A_X ? 0 : 5), nb, &err);
+    stmp = (double *) al.get(upslot + 3, nb);
+    if (al.err) return al.err;
     if (err) return err;
-    hipLaunchKernelGGL(k_survey_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, x, y, z, tmp);
-    sin = tmp;
+    hipLaunchKernelGGL(k_survey_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, x, y, z, stmp);
+    s = stmp;
   }
   hipLaunchKernelGGL(k_fill_int, dim3(nblk(ncell, 256)), dim3(256), 0, c->stream, count, ncell, 0);
   if (n > 0)
@@ -1089,33 +1185,13 @@ static int build_grid(fcfc_gpu_ctx *c, const FkGrid &g, int n, const double *x,
   hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, count, out->cs, ncell);
   if (n > 0)
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, cid, rank, out->cs,
-        x, y, z, need_s ? sin : nullptr, w, out->x, out->y, out->z, out->s, out->w, out->idx);
+        x, y, z, need_s ? s : nullptr, w, out->x, out->y, out->z, out->s, out->w, out->idx);
+  if (wt && !w && n > 0)     /* unit weights (build_tree.c:143-156) */
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n, 256)), dim3(256), 0, c->stream, out->w, (int64_t) n, 1.0);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
 
-static int build_tiles(fcfc_gpu_ctx *c, const FkGrid &g, const DevCat &A,
-    int T, int zmaxc, FkTile **tiles, int *ntiles) {
-  int err = 0;
-  const int nrow = g.ncx * g.ncy;
-  int *rown = (int *) ws(c, WS_ROWN, sizeof(int) * (size_t) nrow, &err);
-  int *rowoff = (int *) ws(c, WS_ROWOFF, sizeof(int) * ((size_t) nrow + 1), &err);
-  if (err) return err;
-  hipLaunchKernelGGL(k_tile_count, dim3(nblk(nrow, 64)), dim3(64), 0, c->stream, nrow, g.ncz, A.cs, T, zmaxc, rown);
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, rown, rowoff, nrow);
-  int nt = 0;
-  HIPCHK(c, hipMemcpyAsync(&nt, rowoff + nrow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  FkTile *t = (FkTile *) ws(c, WS_TILES, sizeof(FkTile) * (size_t) (nt > 0 ? nt : 1), &err);
-  if (err) return err;
-  if (nt > 0)
-    hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nrow, 64)), dim3(64), 0, c->stream, nrow, g.ncz, A.cs, T, zmaxc, rowoff, t);
-  HIPCHK(c, hipGetLastError());
-  *tiles = t; *ntiles = nt;
-  return 0;
-}
-
-/* LDS plan */
 struct LdsPlan { int off_hist, off_stab, off_ptab, off_mutab, off_edges, off_strip, off_chunk, total, nh; };
 
 static int align16(int v) { return (v + 15) & ~15; }
@@ -1189,7 +1265,7 @@ static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int m
  * nmu u E) / s.  All bounds are inflated by 25%; pairs with s^2 below
  * s2_tiny (where second-order terms matter) are always re-evaluated. */
 static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
-    FkArgs *a) {
+    FkWin *win) {
   const double u = ldexp(1.0, -24);
   const double s2max = b->s2bin[b->ns];
   const double pmx = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
@@ -1199,7 +1275,7 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
   const double tiny = ldexp(1.0, -44) * std::max(g.Lx, std::max(g.Ly, g.Lz)) * 4;
   const double ek = u * (2 * s + E) + tiny;
   const double es2 = 7 * u * S2 + 2 * sqrt(3.0) * u * E * s + 3 * ek * ek + 6 * ek * tiny;
-  FkWin &w = a->win;
+  FkWin &w = *win;
   w.e_s2 = (float) (es2 * 1.25 + 1e-7);
   w.e_pi = (float) ((u * (2 * pmx + E) + tiny) * 1.25 + 1e-7);
   w.e_rrel = (float) (10 * u * 1.25);
@@ -1210,180 +1286,18 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
 }
 
 /* One counting (or gradient) job on device-resident, sorted catalogues. */
-static int run_count(fcfc_gpu_ctx *c, const fcfc_bins *b, const JobPlan &P,
-    const DevCat &A, const DevCat &B, int autoc, int wt, int mode,
-    const FkTile *tiles, int ntiles, const double *G, double *gA,
-    void *dhist, unsigned long long *dstats, unsigned int *dqueue) {
-  int err = 0;
-  const int need_s = (b->geometry == FCFC_GEOM_SURVEY && b->bintype != FCFC_BIN_ISO);
-  const int geom = b->geometry;
-  int m = mode;
-  if (m == MODE_C32 && geom != FCFC_GEOM_BOX) m = MODE_C64;    /* survey FP32: FP64 kernel */
-  /* histogram copies: one per wavefront if small, else one per workgroup */
-  int nh = 1;
-  if (m != MODE_GRAD) {
-    int per = b->ntot * (wt ? 8 : 4);
-    nh = (per * 4 <= 16384) ? 4 : ((per * 2 <= 24576) ? 2 : 1);
-  }
-  LdsPlan L = lds_plan(b, m, wt, need_s, nh);
-  if (L.total > 160 * 1024) {
-    set_err(c, "binning tables need %d bytes of LDS (limit 160 KiB)", L.total);
-    return FCFC_ERR_CFG;
-  }
-  const int hyb = b->tabtype ? 1 : 0;
-  const int mut = (b->bintype == FCFC_BIN_SMU && b->nmu * b->nmu <= 16384) ? 1 : 0;
-  kfn_t fn = pick_kernel(b->bintype, geom, wt, m, hyb, mut);
-  HIPCHK(c, hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
-  /* device tables */
-  uint16_t *dstab = (uint16_t *) ws(c, WS_STAB, sizeof(uint16_t) * (b->stab_len + 1), &err);
-  uint16_t *dptab = (uint16_t *) ws(c, WS_PTAB, sizeof(uint16_t) * (b->ptab_len + 1), &err);
-  double *ds2e = (double *) ws(c, WS_S2E, sizeof(double) * (b->ns + 1), &err);
-  double *dpe = (double *) ws(c, WS_PE, sizeof(double) * (b->np + 1), &err);
-  int *dstrips = (int *) ws(c, WS_STRIPS, sizeof(int) * P.strips.size(), &err);
-  if (err) return err;
-  HIPCHK(c, hipMemcpyAsync(dstab, b->stab, sizeof(uint16_t) * b->stab_len, hipMemcpyHostToDevice, c->stream));
-  if (b->bintype == FCFC_BIN_SPI)
-    HIPCHK(c, hipMemcpyAsync(dptab, b->ptab, sizeof(uint16_t) * b->ptab_len, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(ds2e, b->s2bin, sizeof(double) * (b->ns + 1), hipMemcpyHostToDevice, c->stream));
-  if (b->bintype == FCFC_BIN_SPI)
-    HIPCHK(c, hipMemcpyAsync(dpe, b->pbin, sizeof(double) * (b->np + 1), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(dstrips, P.strips.data(), sizeof(int) * P.strips.size(), hipMemcpyHostToDevice, c->stream));
 
-  FkArgs a;
-  memset(&a, 0, sizeof a);
-  a.A = FkCat{A.n, A.x, A.y, A.z, A.s, A.w, A.idx, A.cs};
-  a.B = FkCat{B.n, B.x, B.y, B.z, B.s, B.w, B.idx, B.cs};
-  a.g = P.g;
-  FkBins &fb = a.bins;
-  fb.bintype = b->bintype; fb.geometry = geom;
-  fb.ns = b->ns; fb.nmu = b->nmu; fb.np = b->np; fb.ntot = b->ntot;
-  fb.tabtype = b->tabtype; fb.soff = b->soff; fb.poff = b->poff;
-  fb.stab_len = b->stab_len; fb.ptab_len = b->ptab_len;
-  fb.mutab_lds = (b->bintype == FCFC_BIN_SMU && b->nmu * b->nmu <= 16384);
-  fb.s2min = b->s2bin[0]; fb.s2max = b->s2bin[b->ns];
-  fb.pmin = (b->bintype == FCFC_BIN_SPI) ? b->pbin[0] : 0;
-  fb.pmax = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
-  fb.nmu2 = (double) b->nmu * b->nmu;
-  fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
-  a.tiles = tiles; a.ntiles = ntiles;
-  a.strips = dstrips; a.nstrips = P.nstrips;
-  a.autoc = autoc;
-  a.queue = dqueue;
-  a.gcnt_u = (unsigned long long *) dhist; a.gcnt_d = (double *) dhist;
-  a.stats = dstats;
-  a.nh = L.nh;
-  a.off_hist = L.off_hist; a.off_stab = L.off_stab; a.off_ptab = L.off_ptab;
-  a.off_mutab = L.off_mutab; a.off_edges = L.off_edges; a.off_strip = L.off_strip;
-  a.off_chunk = L.off_chunk;
-  if (m == MODE_C32) fp32_windows(b, P.g, P.zmaxc, &a);
-  a.census = 0;
-  a.G = G; a.gA = gA;
-  int occ = 0;
-  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *) fn, FK_NT, L.total));
-  if (occ < 1) occ = 1;
-  int nblock = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
-  if (nblock > ntiles) nblock = std::max(ntiles, 1);
-  if (ntiles > 0)
-    hipLaunchKernelGGL(fn, dim3(nblock), dim3(FK_NT), L.total, c->stream, a);
-  HIPCHK(c, hipGetLastError());
-  return 0;
-}
+/* A job ready to count: plan, sorted catalogues, output. */
+struct PJob {
+  JobPlan P;
+  DevCat A, B;
+  int autoc = 0, wt = 0;
+  double *gA = nullptr;
+  long long hist_off = 0;
+};
 
 static int cat_ok(const fcfc_cat *k) {
   return k && k->n >= 0 && k->n < 0x7fffffff && (k->n == 0 || (k->x && k->y && k->z));
-}
-
-/* bounding box of host or device coordinates (survey grids) */
-__global__ void k_minmax(int n, const double *__restrict__ x, const double *__restrict__ y,
-    const double *__restrict__ z, double *out) {
-  __shared__ double sm[6][256];
-  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    v[0] = fmin(v[0], x[i]); v[1] = fmin(v[1], y[i]); v[2] = fmin(v[2], z[i]);
-    v[3] = fmax(v[3], x[i]); v[4] = fmax(v[4], y[i]); v[5] = fmax(v[5], z[i]);
-  }
-  for (int k = 0; k < 6; k++) sm[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int) threadIdx.x < s)
-      for (int k = 0; k < 6; k++)
-        sm[k][threadIdx.x] = (k < 3) ? fmin(sm[k][threadIdx.x], sm[k][threadIdx.x + s])
-                                     : fmax(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[blockIdx.x] = 0;   /* unused slot */
-  if (threadIdx.x < 6) out[8 + blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
-}
-
-static int bbox(fcfc_gpu_ctx *c, const double *x, const double *y, const double *z,
-    int n, double lo[3], double hi[3]) {
-  int err = 0;
-  if (n <= 0) return 0;
-  const int nbk = 64;
-  double *d = (double *) ws(c, WS_MISC, sizeof(double) * (8 + 6 * nbk), &err);
-  if (err) return err;
-  hipLaunchKernelGGL(k_minmax, dim3(nbk), dim3(256), 0, c->stream, n, x, y, z, d);
-  std::vector<double> h(8 + 6 * nbk);
-  HIPCHK(c, hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int bk = 0; bk < nbk; bk++)
-    for (int k = 0; k < 3; k++) {
-      lo[k] = fmin(lo[k], h[8 + bk * 6 + k]);
-      hi[k] = fmax(hi[k], h[8 + bk * 6 + 3 + k]);
-    }
-  return 0;
-}
-
-/* Shared front half of count / backward: upload, grid, tiles. */
-struct Prepared {
-  JobPlan P;
-  DevCat A, B;
-  FkTile *tiles = nullptr;
-  int ntiles = 0;
-};
-
-static int prepare(fcfc_gpu_ctx *c, const fcfc_cat *a, const fcfc_cat *b,
-    const fcfc_bins *bins, const double *bsize, int autoc, int wt, int mode,
-    Prepared *pp) {
-  const int need_s = (bins->geometry == FCFC_GEOM_SURVEY && bins->bintype != FCFC_BIN_ISO);
-  const double *ax, *ay, *az, *as, *aw, *bx = nullptr, *by = nullptr, *bz = nullptr, *bs = nullptr, *bw = nullptr;
-  int e = upload(c, a, WS_IN_X, need_s, wt, &ax, &ay, &az, &as, &aw);
-  if (e) return e;
-  if (!autoc) { e = upload(c, b, WS_IN2_X, need_s, wt, &bx, &by, &bz, &bs, &bw); if (e) return e; }
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  if (bins->geometry == FCFC_GEOM_SURVEY) {
-    if ((e = bbox(c, ax, ay, az, (int) a->n, lo, hi))) return e;
-    if (!autoc && (e = bbox(c, bx, by, bz, (int) b->n, lo, hi))) return e;
-    if (!(lo[0] <= hi[0])) { for (int k = 0; k < 3; k++) { lo[k] = 0; hi[k] = 1; } }
-  }
-  e = make_plan(c, bins, bsize, lo, hi, autoc, mode, a->n, autoc ? a->n : b->n, &pp->P);
-  if (e) return e;
-  if ((e = build_grid(c, pp->P.g, (int) a->n, ax, ay, az, as, wt ? aw : nullptr, need_s, WS_A_X, &pp->A))) return e;
-  if (wt && !aw && a->n > 0) {
-    /* unit weights for an unweighted catalogue in a weighted count
-     * (build_tree.c:143-156) */
-    int err = 0;
-    double *one = (double *) ws(c, WS_A_W, sizeof(double) * a->n, &err);
-    if (err) return err;
-    std::vector<double> ones(a->n, 1.0);
-    HIPCHK(c, hipMemcpyAsync(one, ones.data(), sizeof(double) * a->n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    pp->A.w = one;
-  }
-  if (autoc) pp->B = pp->A;
-  else {
-    if ((e = build_grid(c, pp->P.g, (int) b->n, bx, by, bz, bs, wt ? bw : nullptr, need_s, WS_B_X, &pp->B))) return e;
-    if (wt && !bw && b->n > 0) {
-      int err = 0;
-      double *one = (double *) ws(c, WS_B_W, sizeof(double) * b->n, &err);
-      if (err) return err;
-      std::vector<double> ones(b->n, 1.0);
-      HIPCHK(c, hipMemcpyAsync(one, ones.data(), sizeof(double) * b->n, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      pp->B.w = one;
-    }
-  }
-  return build_tiles(c, pp->P.g, pp->A, pp->P.T, pp->P.zmaxc, &pp->tiles, &pp->ntiles);
 }
 
 static int bins_ok(const fcfc_bins *b) {
@@ -1391,60 +1305,276 @@ static int bins_ok(const fcfc_bins *b) {
       (b->bintype != FCFC_BIN_SPI || (b->pbin && b->ptab));
 }
 
+/* Plan a job and stage its catalogues.  Sorted copies are shared between
+ * jobs of one call when the catalogue (same coordinate array) and the grid
+ * coincide. */
+struct CopyKey { const double *x; int64_t n; int wt; FkGrid g; DevCat d; };
+
+static bool same_grid(const FkGrid &a, const FkGrid &b) {
+  return a.ncx == b.ncx && a.ncy == b.ncy && a.ncz == b.ncz && a.ox == b.ox && a.oy == b.oy &&
+      a.oz == b.oz && a.wx == b.wx && a.wy == b.wy && a.wz == b.wz;
+}
+
+static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
+    const fcfc_cat *a, const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
+    int autoc, int wt, int mode, PJob *J) {
+  const int need_s = (bins->geometry == FCFC_GEOM_SURVEY && bins->bintype != FCFC_BIN_ISO);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  int e;
+  if (bins->geometry == FCFC_GEOM_SURVEY) {
+    if ((e = bbox(c, a, lo, hi))) return e;
+    if (!autoc && (e = bbox(c, b, lo, hi))) return e;
+    if (!(lo[0] <= hi[0])) { for (int k = 0; k < 3; k++) { lo[k] = 0; hi[k] = 1; } }
+  }
+  if ((e = make_plan(c, bins, bsize, lo, hi, autoc, mode, a->n, autoc ? a->n : b->n, &J->P))) return e;
+  J->autoc = autoc;
+  J->wt = wt;
+  const fcfc_cat *cs[2] = {a, autoc ? a : b};
+  DevCat *dst[2] = {&J->A, &J->B};
+  for (int q = 0; q < (autoc ? 1 : 2); q++) {
+    const fcfc_cat *k = cs[q];
+    bool found = false;
+    for (auto &cp : copies)
+      if (cp.x == k->x && cp.n == k->n && cp.wt == wt && same_grid(cp.g, J->P.g)) { *dst[q] = cp.d; found = true; break; }
+    if (found) continue;
+    const int slot = (q == 0) ? WS_A_X : WS_B_X;
+    const int upslot = (q == 0) ? WS_IN_X : WS_IN2_X;
+    if ((e = stage_cat(c, al, k, need_s, wt, J->P.g, slot, upslot, dst[q]))) return e;
+    if (al.own) copies.push_back(CopyKey{k->x, k->n, wt, J->P.g, *dst[q]});
+  }
+  if (autoc) J->B = J->A;
+  return 0;
+}
+
+/* Tile all jobs, then count (or differentiate) them in one persistent launch. */
+static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
+    std::vector<PJob> &jobs, int mode, const double *G, void *dhist,
+    unsigned long long *dstats, unsigned int *dqueue, float *kms) {
+  const int geom = b->geometry;
+  int m = mode;
+  if (m == MODE_C32 && geom != FCFC_GEOM_BOX) m = MODE_C64;    /* survey FP32: FP64 kernel */
+  const int need_s = (geom == FCFC_GEOM_SURVEY && b->bintype != FCFC_BIN_ISO);
+  int anywt = 0;
+  for (auto &j : jobs) anywt |= j.wt;
+  if (m == MODE_GRAD) anywt = 1;
+  /* ---- tiles of all jobs ---- */
+  std::vector<int> rowoff(jobs.size() + 1, 0);
+  for (size_t k = 0; k < jobs.size(); k++) rowoff[k + 1] = rowoff[k] + jobs[k].P.g.ncx * jobs[k].P.g.ncy;
+  const int nrow = rowoff.back();
+  int err = 0;
+  int *rown = (int *) ws(c, WS_ROWN, sizeof(int) * (size_t) nrow, &err);
+  int *rowscan = (int *) ws(c, WS_ROWOFF, sizeof(int) * ((size_t) nrow + 1), &err);
+  if (err) return err;
+  for (size_t k = 0; k < jobs.size(); k++) {
+    const PJob &J = jobs[k];
+    const int nr = J.P.g.ncx * J.P.g.ncy;
+    hipLaunchKernelGGL(k_tile_count, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
+        J.P.T, J.P.zmaxc, rown + rowoff[k]);
+  }
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, rown, rowscan, nrow);
+  int ntiles = 0;
+  HIPCHK(c, hipMemcpyAsync(&ntiles, rowscan + nrow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  FkTile *tiles = (FkTile *) ws(c, WS_TILES, sizeof(FkTile) * (size_t) (ntiles > 0 ? ntiles : 1), &err);
+  if (err) return err;
+  for (size_t k = 0; k < jobs.size(); k++) {
+    const PJob &J = jobs[k];
+    const int nr = J.P.g.ncx * J.P.g.ncy;
+    hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
+        J.P.T, J.P.zmaxc, rowscan + rowoff[k], (int) k, tiles);
+  }
+  /* ---- LDS plan and kernel ---- */
+  int nh = 1;
+  if (m != MODE_GRAD) {
+    int per = b->ntot * (anywt ? 8 : 4);
+    nh = (per * 4 <= 16384) ? 4 : ((per * 2 <= 24576) ? 2 : 1);
+  }
+  LdsPlan L = lds_plan(b, m, anywt, need_s, nh);
+  if (L.total > 160 * 1024) {
+    set_err(c, "binning tables need %d bytes of LDS (limit 160 KiB)", L.total);
+    return FCFC_ERR_CFG;
+  }
+  const int hyb = b->tabtype ? 1 : 0;
+  const int mut = (b->bintype == FCFC_BIN_SMU && b->nmu * b->nmu <= 16384) ? 1 : 0;
+  kfn_t fn = pick_kernel(b->bintype, geom, anywt, m, hyb, mut);
+  HIPCHK(c, hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
+  /* ---- tables, strips, job descriptors ---- */
+  size_t nstr = 0;
+  for (auto &j : jobs) nstr += j.P.strips.size();
+  uint16_t *dstab = (uint16_t *) ws(c, WS_STAB, sizeof(uint16_t) * (b->stab_len + 1), &err);
+  uint16_t *dptab = (uint16_t *) ws(c, WS_PTAB, sizeof(uint16_t) * (b->ptab_len + 1), &err);
+  double *ds2e = (double *) ws(c, WS_S2E, sizeof(double) * (b->ns + 1), &err);
+  double *dpe = (double *) ws(c, WS_PE, sizeof(double) * (b->np + 1), &err);
+  int *dstrips = (int *) ws(c, WS_STRIPS, sizeof(int) * (nstr ? nstr : 1), &err);
+  FkJob *djobs = nullptr;
+  if (err) return err;
+  std::vector<int> hstr;
+  hstr.reserve(nstr);
+  std::vector<FkJob> hj(jobs.size());
+  for (size_t k = 0; k < jobs.size(); k++) {
+    const PJob &J = jobs[k];
+    FkJob &f = hj[k];
+    memset(&f, 0, sizeof f);
+    f.A = FkCat{J.A.n, J.A.x, J.A.y, J.A.z, J.A.s, J.A.w, J.A.idx, J.A.cs};
+    f.B = FkCat{J.B.n, J.B.x, J.B.y, J.B.z, J.B.s, J.B.w, J.B.idx, J.B.cs};
+    f.g = J.P.g;
+    f.strips = dstrips + hstr.size();
+    f.nstrips = J.P.nstrips;
+    f.autoc = J.autoc;
+    f.wt = J.wt;
+    f.hist_off = J.hist_off;
+    f.gA = J.gA;
+    if (m == MODE_C32) fp32_windows(b, J.P.g, J.P.zmaxc, &f.win);
+    hstr.insert(hstr.end(), J.P.strips.begin(), J.P.strips.end());
+  }
+  djobs = (FkJob *) al.get(WS_JOBS, sizeof(FkJob) * jobs.size());
+  if (al.err) return al.err;
+  HIPCHK(c, hipMemcpyAsync(dstab, b->stab, sizeof(uint16_t) * b->stab_len, hipMemcpyHostToDevice, c->stream));
+  if (b->bintype == FCFC_BIN_SPI)
+    HIPCHK(c, hipMemcpyAsync(dptab, b->ptab, sizeof(uint16_t) * b->ptab_len, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ds2e, b->s2bin, sizeof(double) * (b->ns + 1), hipMemcpyHostToDevice, c->stream));
+  if (b->bintype == FCFC_BIN_SPI)
+    HIPCHK(c, hipMemcpyAsync(dpe, b->pbin, sizeof(double) * (b->np + 1), hipMemcpyHostToDevice, c->stream));
+  if (nstr) HIPCHK(c, hipMemcpyAsync(dstrips, hstr.data(), sizeof(int) * nstr, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(djobs, hj.data(), sizeof(FkJob) * jobs.size(), hipMemcpyHostToDevice, c->stream));
+
+  FkArgs a;
+  memset(&a, 0, sizeof a);
+  a.jobs = djobs;
+  a.njobs = (int) jobs.size();
+  FkBins &fb = a.bins;
+  fb.bintype = b->bintype; fb.geometry = geom;
+  fb.ns = b->ns; fb.nmu = b->nmu; fb.np = b->np; fb.ntot = b->ntot;
+  fb.tabtype = b->tabtype; fb.soff = b->soff; fb.poff = b->poff;
+  fb.stab_len = b->stab_len; fb.ptab_len = b->ptab_len;
+  fb.mutab_lds = mut;
+  fb.s2min = b->s2bin[0]; fb.s2max = b->s2bin[b->ns];
+  fb.pmin = (b->bintype == FCFC_BIN_SPI) ? b->pbin[0] : 0;
+  fb.pmax = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
+  fb.nmu2 = (double) b->nmu * b->nmu;
+  fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
+  a.tiles = tiles; a.ntiles = ntiles;
+  a.queue = dqueue;
+  a.gcnt_u = (unsigned long long *) dhist; a.gcnt_d = (double *) dhist;
+  a.stats = dstats;
+  a.nh = L.nh;
+  a.off_hist = L.off_hist; a.off_stab = L.off_stab; a.off_ptab = L.off_ptab;
+  a.off_mutab = L.off_mutab; a.off_edges = L.off_edges; a.off_strip = L.off_strip;
+  a.off_chunk = L.off_chunk;
+  a.census = c->census;
+  a.G = G;
+  int occ = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *) fn, FK_NT, L.total));
+  if (occ < 1) occ = 1;
+  int nblock = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
+  if (nblock > ntiles) nblock = std::max(ntiles, 1);
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  if (ntiles > 0)
+    hipLaunchKernelGGL(fn, dim3(nblock), dim3(FK_NT), L.total, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipEventElapsedTime(kms, c->ev0, c->ev1));
+  c->stats.tile = jobs.empty() ? 0 : jobs[0].P.T;
+  c->stats.cell_div = jobs.empty() ? 0 : (int) lround(sqrt(jobs[0].P.g.Rxy2) / jobs[0].P.g.wx);
+  return 0;
+}
+
+static int stats_from(fcfc_gpu_ctx *c, unsigned long long *dstats) {
+  unsigned long long st[3] = {0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(st, dstats, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stats.evals = (double) st[0];
+  c->stats.rechecked = (double) st[1];
+  c->stats.census = (double) st[2];
+  return 0;
+}
+
+/* counts of njob jobs; job k's histogram lands in cnt + k * ntot */
+static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
+    const fcfc_cat *const *bs, const double *const *bsz, const uint32_t *fl,
+    const fcfc_bins *bins, uint32_t flags, void *cnt) {
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(c, hipSetDevice(c->device));
+  memset(&c->stats, 0, sizeof c->stats);
+  c->census = (flags & FCFC_GPU_CENSUS) ? 1 : 0;
+  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 : MODE_C64;
+  Alloc al(c, njob > 1);
+  std::vector<CopyKey> copies;
+  std::vector<PJob> jobs(njob);
+  for (int k = 0; k < njob; k++) {
+    const int autoc = (fl[k] & FCFC_GPU_AUTO) ? 1 : 0;
+    const int wt = (fl[k] & FCFC_GPU_WEIGHTED) ? 1 : 0;
+    if (!cat_ok(as[k]) || (!autoc && !cat_ok(bs[k]))) { set_err(c, "invalid catalogue in job %d", k); return FCFC_ERR_ARG; }
+    if (bins->geometry == FCFC_GEOM_BOX && !bsz[k]) { set_err(c, "box size required"); return FCFC_ERR_ARG; }
+    int e = setup_job(c, al, copies, as[k], bs[k], bins, bsz[k], autoc, wt, mode, &jobs[k]);
+    if (e) return e;
+    jobs[k].hist_off = (long long) k * bins->ntot;
+  }
+  int err = 0;
+  const size_t hbytes = 8 * (size_t) bins->ntot * njob;
+  void *dh = ws(c, WS_HIST, hbytes, &err);
+  unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
+  if (err) return err;
+  hipLaunchKernelGGL(k_zero_u64, dim3(nblk((int64_t) bins->ntot * njob, 256)), dim3(256), 0, c->stream,
+      (unsigned long long *) dh, bins->ntot * njob);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
+  float ms = 0;
+  int e = run_jobs(c, al, bins, jobs, mode, nullptr, dh, misc + 8, (unsigned int *) misc, &ms);
+  if (e) return e;
+  const int n = bins->ntot * njob;
+  if (flags & FCFC_GPU_CNT_DEVICE) {
+    for (int k = 0; k < njob; k++) {
+      char *dst = (char *) cnt + (size_t) k * bins->ntot * 8;
+      const char *src = (const char *) dh + (size_t) k * bins->ntot * 8;
+      if (jobs[k].wt) hipLaunchKernelGGL(k_add_f64, dim3(nblk(bins->ntot, 256)), dim3(256), 0, c->stream, (double *) dst, (const double *) src, bins->ntot);
+      else hipLaunchKernelGGL(k_add_u64, dim3(nblk(bins->ntot, 256)), dim3(256), 0, c->stream, (long long *) dst, (const unsigned long long *) src, bins->ntot);
+    }
+    HIPCHK(c, hipGetLastError());
+  }
+  else {
+    std::vector<unsigned long long> h(n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), dh, 8 * (size_t) n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < njob; k++) {
+      const size_t o = (size_t) k * bins->ntot;
+      if (jobs[k].wt) { double *dst = (double *) cnt + o; const double *hd = (const double *) h.data() + o; for (int i = 0; i < bins->ntot; i++) dst[i] += hd[i]; }
+      else { int64_t *dst = (int64_t *) cnt + o; for (int i = 0; i < bins->ntot; i++) dst[i] += (int64_t) h[o + i]; }
+    }
+  }
+  if ((e = stats_from(c, misc + 8))) return e;
+  c->stats.kernel_ms = ms;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
 extern "C" int fcfc_gpu_count(fcfc_gpu_ctx *c, const fcfc_cat *a,
     const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
     uint32_t flags, void *cnt) {
   if (!c) return FCFC_ERR_ARG;
-  auto t0 = std::chrono::steady_clock::now();
-  const int autoc = (flags & FCFC_GPU_AUTO) ? 1 : 0;
-  const int wt = (flags & FCFC_GPU_WEIGHTED) ? 1 : 0;
-  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 : MODE_C64;
-  if (!cat_ok(a) || (!autoc && !cat_ok(b)) || !bins_ok(bins) || !cnt) {
-    set_err(c, "invalid argument"); return FCFC_ERR_ARG;
+  if (!bins_ok(bins) || !cnt) { set_err(c, "invalid argument"); return FCFC_ERR_ARG; }
+  const fcfc_cat *as[1] = {a}, *bs[1] = {b};
+  const double *bz[1] = {bsize};
+  uint32_t fl[1] = {flags & (FCFC_GPU_AUTO | FCFC_GPU_WEIGHTED)};
+  return count_jobs(c, 1, as, bs, bz, fl, bins, flags, cnt);
+}
+
+extern "C" int fcfc_gpu_count_batch(fcfc_gpu_ctx *c, int njob,
+    const fcfc_batch_job *jobs, const fcfc_bins *bins, uint32_t flags,
+    void *cnt) {
+  if (!c || njob < 0 || (njob && !jobs) || !cnt) return FCFC_ERR_ARG;
+  if (!bins_ok(bins)) { set_err(c, "invalid bins"); return FCFC_ERR_ARG; }
+  if (njob == 0) return 0;
+  std::vector<const fcfc_cat *> as(njob), bs(njob);
+  std::vector<const double *> bz(njob);
+  std::vector<uint32_t> fl(njob);
+  for (int k = 0; k < njob; k++) {
+    as[k] = &jobs[k].a; bs[k] = &jobs[k].b; bz[k] = jobs[k].bsize;
+    fl[k] = jobs[k].flags & (FCFC_GPU_AUTO | FCFC_GPU_WEIGHTED);
   }
-  if (bins->geometry == FCFC_GEOM_BOX && !bsize) { set_err(c, "box size required"); return FCFC_ERR_ARG; }
-  HIPCHK(c, hipSetDevice(c->device));
-  memset(&c->stats, 0, sizeof c->stats);
-  Prepared pp;
-  int e = prepare(c, a, b, bins, bsize, autoc, wt, mode, &pp);
-  if (e) return e;
-  int err = 0;
-  void *dh = ws(c, WS_HIST, 8 * (size_t) bins->ntot, &err);
-  unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
-  if (err) return err;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, c->stream, (unsigned long long *) dh, bins->ntot);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
-  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  FkArgs dummy; (void) dummy;
-  e = run_count(c, bins, pp.P, pp.A, pp.B, autoc, wt, mode, pp.tiles, pp.ntiles, nullptr, nullptr,
-      dh, misc + 8, (unsigned int *) (misc));
-  if (e) return e;
-  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-  if (flags & FCFC_GPU_CNT_DEVICE) {
-    if (wt) hipLaunchKernelGGL(k_add_f64, dim3(nblk(bins->ntot, 256)), dim3(256), 0, c->stream, (double *) cnt, (const double *) dh, bins->ntot);
-    else hipLaunchKernelGGL(k_add_u64, dim3(nblk(bins->ntot, 256)), dim3(256), 0, c->stream, (long long *) cnt, (const unsigned long long *) dh, bins->ntot);
-    HIPCHK(c, hipGetLastError());
-  }
-  else {
-    std::vector<unsigned long long> h(bins->ntot);
-    HIPCHK(c, hipMemcpyAsync(h.data(), dh, 8 * (size_t) bins->ntot, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (wt) { double *o = (double *) cnt; const double *hd = (const double *) h.data(); for (int i = 0; i < bins->ntot; i++) o[i] += hd[i]; }
-    else { int64_t *o = (int64_t *) cnt; for (int i = 0; i < bins->ntot; i++) o[i] += (int64_t) h[i]; }
-  }
-  unsigned long long st[3] = {0, 0, 0};
-  HIPCHK(c, hipMemcpyAsync(st, misc + 8, sizeof st, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  c->stats.evals = (double) st[0];
-  c->stats.rechecked = (double) st[1];
-  c->stats.census = (double) st[2];
-  c->stats.kernel_ms = ms;
-  c->stats.cell_div = (int) lround(sqrt(pp.P.g.Rxy2) / pp.P.g.wx);
-  c->stats.tile = pp.P.T;
-  c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  return 0;
+  return count_jobs(c, njob, as.data(), bs.data(), bz.data(), fl.data(), bins,
+      flags & ~(FCFC_GPU_AUTO | FCFC_GPU_WEIGHTED), cnt);
 }
 
 extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
@@ -1459,74 +1589,47 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
   if (bins->geometry == FCFC_GEOM_BOX && !bsize) { set_err(c, "box size required"); return FCFC_ERR_ARG; }
   HIPCHK(c, hipSetDevice(c->device));
   memset(&c->stats, 0, sizeof c->stats);
+  c->census = 0;
   const int dev = (flags & FCFC_GPU_CNT_DEVICE) ? 1 : 0;
+  const int npass = (autoc || !gb) ? 1 : 2;
+  Alloc al(c, npass > 1);
+  std::vector<CopyKey> copies;
+  std::vector<PJob> jobs(npass);
   int err = 0;
+  for (int p = 0; p < npass; p++) {
+    const fcfc_cat *pa = p ? b : a, *pb = p ? a : b;
+    int e = setup_job(c, al, copies, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &jobs[p]);
+    if (e) return e;
+    jobs[p].gA = (double *) al.get(p ? WS_GB : WS_GA, sizeof(double) * (size_t) (pa->n > 0 ? pa->n : 1));
+    if (al.err) return al.err;
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, jobs[p].gA, (int64_t) pa->n, 0.0);
+  }
   double *dG = (double *) ws(c, WS_G, sizeof(double) * bins->ntot, &err);
   unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
   if (err) return err;
   HIPCHK(c, hipMemcpyAsync(dG, grad_cnt, sizeof(double) * bins->ntot,
       dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
-  float ktot = 0;
-  double evals = 0;
-  for (int pass = 0; pass < (autoc ? 1 : 2); pass++) {
-    if (pass == 1 && !gb) break;
-    const fcfc_cat *pa = pass ? b : a, *pb = pass ? a : b;
-    double *gout = pass ? gb : ga;
-    Prepared pp;
-    int e = prepare(c, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &pp);
-    if (e) return e;
-    double *dgA = (double *) ws(c, WS_GA, sizeof(double) * (size_t) (pa->n > 0 ? pa->n : 1), &err);
-    if (err) return err;
-    hipLaunchKernelGGL(k_zero_u64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, (unsigned long long *) dgA, (int) pa->n);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    e = run_count(c, bins, pp.P, pp.A, pp.B, autoc, 1, MODE_GRAD, pp.tiles, pp.ntiles, dG, dgA,
-        nullptr, misc + 8, (unsigned int *) misc);
-    if (e) return e;
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
+  float ms = 0;
+  int e = run_jobs(c, al, bins, jobs, MODE_GRAD, dG, nullptr, misc + 8, (unsigned int *) misc, &ms);
+  if (e) return e;
+  for (int p = 0; p < npass; p++) {
+    const fcfc_cat *pa = p ? b : a;
+    double *gout = p ? gb : ga;
     if (dev) {
-      hipLaunchKernelGGL(k_add_f64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, gout, dgA, (int) pa->n);
+      hipLaunchKernelGGL(k_add_f64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, gout, jobs[p].gA, (int) pa->n);
       HIPCHK(c, hipGetLastError());
     }
     else {
       std::vector<double> h((size_t) pa->n);
-      HIPCHK(c, hipMemcpyAsync(h.data(), dgA, sizeof(double) * (size_t) pa->n, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(h.data(), jobs[p].gA, sizeof(double) * (size_t) pa->n, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
       for (int64_t i = 0; i < pa->n; i++) gout[i] += h[(size_t) i];
     }
-    unsigned long long st[3];
-    HIPCHK(c, hipMemcpyAsync(st, misc + 8, sizeof st, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    ktot += ms;
-    evals += (double) st[0];
   }
-  c->stats.evals = evals;
-  c->stats.kernel_ms = ktot;
-  c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  return 0;
-}
-
-extern "C" int fcfc_gpu_count_batch(fcfc_gpu_ctx *c, int njob,
-    const fcfc_batch_job *jobs, const fcfc_bins *bins, uint32_t flags,
-    void *cnt) {
-  /* first version: sequential jobs (the persistent multi-job launch is
-   * added on top of the same kernels) */
-  if (!c || njob < 0 || (njob && !jobs) || !cnt) return FCFC_ERR_ARG;
-  double evals = 0, kms = 0, rc = 0;
-  auto t0 = std::chrono::steady_clock::now();
-  for (int k = 0; k < njob; k++) {
-    const fcfc_batch_job *j = jobs + k;
-    uint32_t f = (flags & ~(FCFC_GPU_AUTO | FCFC_GPU_WEIGHTED)) | j->flags;
-    const int wt = (f & FCFC_GPU_WEIGHTED) ? 1 : 0;
-    char *dst = (char *) cnt + (size_t) k * bins->ntot * 8;
-    (void) wt;
-    int e = fcfc_gpu_count(c, &j->a, &j->b, bins, j->bsize, f, dst);
-    if (e) return e;
-    evals += c->stats.evals; kms += c->stats.kernel_ms; rc += c->stats.rechecked;
-  }
-  c->stats.evals = evals; c->stats.kernel_ms = kms; c->stats.rechecked = rc;
+  if ((e = stats_from(c, misc + 8))) return e;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stats.kernel_ms = ms;
   c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }

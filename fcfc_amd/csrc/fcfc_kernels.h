@@ -72,7 +72,7 @@ struct FkCat {
   const int *cell_start;
 };
 
-struct FkTile { int row, p0, p1, iz0, iz1; };
+struct FkTile { int row, p0, p1, iz0, iz1, job; };
 
 /* Floor of the exact quotient a / b for a >= 0, b > 0 (b not tiny):
  * approximate quotient from the hardware reciprocal, then one-step
