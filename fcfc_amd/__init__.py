@@ -241,6 +241,22 @@ class Engine:
         self._check(lib().fcfc_gpu_set_tuning(self._h, cell_div, tile, zdiv, nblock),
                     "fcfc_gpu_set_tuning")
 
+    def autotune(self, a, b, bins, isauto=False, withwt=False, precision="fp64"):
+        """Pick grid/tile shapes for counts like this one (cached in the
+        engine per density bucket).  Returns (cell_div, zdiv, tile, est_ms)."""
+        flags = (L.FCFC_GPU_AUTO if isauto else 0) | (L.FCFC_GPU_WEIGHTED if withwt else 0)
+        if precision == "fp32":
+            flags |= L.FCFC_GPU_PREC_FP32
+        best = (C.c_int * 4)()
+        bs = bins.bsize_rescaled
+        ca = a._c()
+        cb = (b if b is not None else a)._c()
+        self._sync_torch(a, b)
+        e = lib().fcfc_gpu_autotune(self._h, C.byref(ca), C.byref(cb), C.byref(bins._b),
+                                    _dptr(bs) if bs is not None else None, flags, best)
+        self._check(e, "fcfc_gpu_autotune")
+        return best[0], best[1], best[2], best[3] / 1000.0
+
     def count_pairs(self, a, b, bins, isauto=False, withwt=False,
                     precision="fp64", census=False, out=None):
         """Histogram of unordered (auto) or all (cross) pairs; the result is

@@ -90,6 +90,8 @@ SIGNATURES = {
                                           C.POINTER(fcfc_cat), C.POINTER(fcfc_bins), _dp,
                                           C.c_uint32, _dp, _dp, _dp]),
     "fcfc_gpu_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "fcfc_gpu_autotune": (C.c_int, [C.c_void_p, C.POINTER(fcfc_cat), C.POINTER(fcfc_cat),
+                                    C.POINTER(fcfc_bins), _dp, C.c_uint32, C.POINTER(C.c_int)]),
     "fcfc_save_pair_count": (C.c_int, [C.c_char_p, C.POINTER(fcfc_bins), C.c_int, _dp,
                                        C.c_int, C.c_int, C.c_uint64, C.c_uint64,
                                        C.c_double, C.c_double, C.c_void_p]),

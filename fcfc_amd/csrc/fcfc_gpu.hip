@@ -809,6 +809,10 @@ struct fcfc_gpu_ctx {
   fcfc_gpu_stats stats{};
   int t_cell_div = 0, t_tile = 0, t_zdiv = 0, t_nblock = 0;
   int census = 0;
+  int tile_cap = 0;            /* autotuner: count only the first tiles */
+  int last_ntiles = 0;
+  struct Tuned { int geom, bintype, mode, wt, bucket, cd, zd, T; };
+  std::vector<Tuned> tuned;    /* autotuner results per density bucket */
   std::vector<DevBuf> bufs;
 };
 
@@ -906,6 +910,7 @@ extern "C" int fcfc_gpu_set_tuning(fcfc_gpu_ctx *c, int cell_div, int tile,
 /* ---- per-job plan ------------------------------------------------------ */
 
 struct JobPlan {
+  int bucket;       /* density bucket for the autotuner cache */
   FkGrid g;
   int T;            /* objects per tile */
   int zmaxc;        /* max z-cells per tile */
@@ -960,6 +965,20 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
   }
   vol = ext[0] * ext[1] * ext[2];
   const double nden = (double) std::max<int64_t>(nB, 1) / vol;
+  /* shapes picked by fcfc_gpu_autotune for this kind of job */
+  const int bucket = (int) floor(2 * log2(nden * sqrt(rxy2) * sqrt(rxy2) * sqrt(r2) + 1e-300));
+  int tT = c->t_tile;
+  if (cd <= 0 || zd <= 0 || tT <= 0) {
+    for (const auto &t : c->tuned)
+      if (t.geom == b->geometry && t.bintype == b->bintype && t.mode == mode &&
+          t.bucket == bucket) {
+        if (cd <= 0) cd = t.cd;
+        if (zd <= 0) zd = t.zd;
+        if (tT <= 0) tT = t.T;
+        break;
+      }
+  }
+  P->bucket = bucket;
   if (cd <= 0) {
     /* about 24..64 B objects per xy cell per unit of R along z is a good
      * balance between pruning and segment overhead */
@@ -970,7 +989,7 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
   }
   if (zd <= 0) zd = 4;
   P->R = (mode == MODE_GRAD) ? 1 : 2;
-  P->T = c->t_tile > 0 ? c->t_tile : FK_NT * P->R;
+  P->T = tT > 0 ? tT : FK_NT * P->R;
   if (P->T > FK_NT * P->R) P->T = FK_NT * P->R;
   double w = Rxy / cd;
   if (b->geometry == FCFC_GEOM_BOX) {
@@ -1466,10 +1485,12 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   int occ = 0;
   HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *) fn, FK_NT, L.total));
   if (occ < 1) occ = 1;
+  if (c->tile_cap > 0 && c->tile_cap < ntiles) a.ntiles = c->tile_cap;
   int nblock = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
-  if (nblock > ntiles) nblock = std::max(ntiles, 1);
+  if (nblock > a.ntiles) nblock = std::max(a.ntiles, 1);
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  if (ntiles > 0)
+  c->last_ntiles = ntiles;
+  if (a.ntiles > 0)
     hipLaunchKernelGGL(fn, dim3(nblock), dim3(FK_NT), L.total, c->stream, a);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -1631,5 +1652,80 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->stats.kernel_ms = ms;
   c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+/* Project-written autotuner: for jobs like (a, b, bins, flags) try grid and
+ * tile shapes (cells per maximum separation, z-cells per cell, A objects
+ * per tile), timing each on the first eighth of the tile list (a slab of the
+ * box at the true density) and extrapolating to all tiles.  Coordinate
+ * descent over cell_div, then zdiv, then tile.  The winner is cached in the
+ * context for this geometry / binning / precision and density bucket and
+ * used by later counts whose shape is left automatic. */
+extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
+    const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
+    uint32_t flags, int *best) {
+  if (!c || !bins_ok(bins)) return FCFC_ERR_ARG;
+  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 : MODE_C64;
+  const int save_cd = c->t_cell_div, save_zd = c->t_zdiv, save_t = c->t_tile;
+  std::vector<unsigned char> scratch(8 * (size_t) bins->ntot);
+  double bestms = 1e300;
+  int bcd = 4, bzd = 4, bT = 512;
+  auto measure = [&](int cd, int zd, int T, double *est) -> int {
+    c->t_cell_div = cd; c->t_zdiv = zd; c->t_tile = T;
+    /* first pass: learn the number of tiles of this shape with a tiny cap */
+    c->tile_cap = 1;
+    memset(scratch.data(), 0, scratch.size());
+    int e = fcfc_gpu_count(c, a, b, bins, bsize, flags & ~FCFC_GPU_CNT_DEVICE, scratch.data());
+    if (e) return e;
+    const int nt = c->last_ntiles;
+    c->tile_cap = std::max(1, nt / 8);
+    e = fcfc_gpu_count(c, a, b, bins, bsize, flags & ~FCFC_GPU_CNT_DEVICE, scratch.data());
+    if (e) return e;
+    *est = c->stats.kernel_ms * (double) nt / (double) c->tile_cap;
+    return 0;
+  };
+  int e = 0;
+  const int cds[] = {3, 4, 6, 8};
+  for (int cd : cds) {
+    double est;
+    if ((e = measure(cd, 4, 512, &est))) break;
+    if (est < bestms) { bestms = est; bcd = cd; }
+  }
+  const int zds[] = {2, 4, 8};
+  for (int zd : zds) {
+    double est;
+    if (e || (e = measure(bcd, zd, 512, &est))) break;
+    if (est < bestms) { bestms = est; bzd = zd; }
+  }
+  const int Ts[] = {256, 512};
+  for (int T : Ts) {
+    double est;
+    if (e || (e = measure(bcd, bzd, T, &est))) break;
+    if (est < bestms) { bestms = est; bT = T; }
+  }
+  c->tile_cap = 0;
+  c->t_cell_div = save_cd; c->t_zdiv = save_zd; c->t_tile = save_t;
+  if (e) return e;
+  /* cache for the density bucket of this job */
+  {
+    JobPlan P;
+    double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    const int autoc = (flags & FCFC_GPU_AUTO) ? 1 : 0;
+    if (bins->geometry == FCFC_GEOM_SURVEY) {
+      for (int k = 0; k < 3; k++) { lo[k] = 1e300; hi[k] = -1e300; }
+      bbox(c, a, lo, hi);
+      if (!autoc) bbox(c, b, lo, hi);
+    }
+    make_plan(c, bins, bsize, lo, hi, autoc, mode, a->n, autoc ? a->n : b->n, &P);
+    bool found = false;
+    for (auto &t : c->tuned)
+      if (t.geom == bins->geometry && t.bintype == bins->bintype && t.mode == mode && t.bucket == P.bucket) {
+        t.cd = bcd; t.zd = bzd; t.T = bT; found = true;
+      }
+    if (!found) c->tuned.push_back({bins->geometry, bins->bintype, mode, (flags & FCFC_GPU_WEIGHTED) ? 1 : 0,
+                                    P.bucket, bcd, bzd, bT});
+  }
+  if (best) { best[0] = bcd; best[1] = bzd; best[2] = bT; best[3] = (int) lround(bestms * 1000); }
   return 0;
 }

@@ -195,6 +195,15 @@ int fcfc_gpu_count_backward(fcfc_gpu_ctx *ctx, const fcfc_cat *a,
 int fcfc_gpu_set_tuning(fcfc_gpu_ctx *ctx, int cell_div, int tile,
     int zcell_div, int nblock);
 
+/* Autotuner (no counterpart in the reference): times candidate grid / tile
+ * shapes for counts like (a, b, bins, flags) on the first eighth of their tile
+ * list (a slab of the box at the true density), and keeps the fastest in the
+ * context for this geometry, binning, precision and density bucket; later
+ * counts whose shape is automatic use it.  best (may be NULL) receives
+ * {cell_div, zcell_div, tile, estimated full-count microseconds}. */
+int fcfc_gpu_autotune(fcfc_gpu_ctx *ctx, const fcfc_cat *a, const fcfc_cat *b,
+    const fcfc_bins *bins, const double *bsize, uint32_t flags, int *best);
+
 /* ---- pair-count files (src/fcfc/2pt_box/save_res.c:45-296) -------------- */
 /* Write a pair-count file byte-compatible with FCFC's save_res:
  * ofmt 0 = binary, 1 = ASCII.  `cnt` are the FINAL counts (auto already
