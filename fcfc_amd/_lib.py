@@ -116,6 +116,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise OSError(f"{LIB_PATH} is not built; run `python -m fcfc_amd.build`")
+    # Share one HIP runtime with PyTorch: torch bundles its own libamdhip64
+    # (same SONAME, different file name in its DT_NEEDED).  Loaded first, it
+    # satisfies this library's dependency; loaded second, a process would hold
+    # two HIP runtimes and torch would find no GPU.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)

@@ -166,7 +166,7 @@ struct FkJob {
   int autoc;                  /* COUNT: A == B, half shell + self tile;
                                  GRAD: A == B, exclude j == i          */
   int wt;                     /* weighted histogram (double)          */
-  int pad;
+  int swap_all;               /* GRAD: every pair oriented (a - D) - b  */
   long long hist_off;         /* histogram of this job: gcnt + hist_off */
   double *gA;                 /* GRAD: output, original object order  */
   FkWin win;                  /* FP32 error model (MODE_C32)          */
@@ -200,6 +200,7 @@ struct FkStripLds {
 /* Compute the (up to three) B segments of one strip for tile t. */
 __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
     int dy, int diag, int *b0, int *len, int *code) {
+  const int diag_in = diag;
   const FkGrid &g = J.g;
   b0[0] = b0[1] = b0[2] = 0;
   len[0] = len[1] = len[2] = 0;
@@ -220,7 +221,9 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
   const double zc1 = (double) (t.iz1 + 1) + hz / g.wz + 1e-6;
   int izlo = (int) floor(zc0), izhi = (int) floor(zc1);
   const int *cs = J.B.cell_start + (size_t) (bx * g.ncy + by) * g.ncz;
-  const int cxy = (cx + 1) + 3 * (cy + 1);
+  const int swapf = (diag & 2) ? 32 : 0;   /* gradient: count evaluates from the other tile */
+  diag &= 1;
+  const int cxy = (cx + 1) + 3 * (cy + 1) + swapf;
   int ns = 0;
   if (g.periodic) {
     if (izhi - izlo + 1 > g.ncz) {   /* cannot happen with the tile limits */
@@ -241,7 +244,10 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
     }
     if (izhi >= g.ncz && !diag) {    /* image above: B shifted by +Lz */
       int c0 = 0, c1 = izhi - g.ncz;
-      b0[ns] = cs[c0]; len[ns] = cs[c1 + 1] - cs[c0]; code[ns] = cxy + 9 * 2; ns++;
+      /* gradient of auto counts, same row: the count takes this image from
+       * the lower tile (the (0,0) strip keeps the lower wrap only) */
+      const int sw = (dx == 0 && dy == 0 && (diag_in & 4)) ? 32 : 0;
+      b0[ns] = cs[c0]; len[ns] = cs[c1 + 1] - cs[c0]; code[ns] = (cxy | sw) + 9 * 2; ns++;
     }
   }
   else {
@@ -616,6 +622,9 @@ k_count(const FkArgs a) {
       since_flush = 0;
     }
     const int ix = t.row / J.g.ncy, iy = t.row - ix * J.g.ncy;
+    /* weights exist for this job (a weighted-capable kernel may run
+     * unweighted jobs of a batch, whose catalogues carry no weights) */
+    const bool jw = (MODE == MODE_GRAD) || (WT && J.wt);
 
     /* A objects of this lane */
     double ax[R], ay[R], az[R], as[R], aw[R], gacc[R];
@@ -634,7 +643,7 @@ k_count(const FkArgs a) {
       ai[r] = av[r] ? p : 0x7fffffff;
       ax[r] = J.A.x[q]; ay[r] = J.A.y[q]; az[r] = J.A.z[q];
       as[r] = NEED_S ? J.A.s[q] : 0.0;
-      aw[r] = WT ? J.A.w[q] : 1.0;
+      aw[r] = jw ? J.A.w[q] : 1.0;
       gacc[r] = 0.0;
       fax[r] = (float) (ax[r] - tcx); fay[r] = (float) (ay[r] - tcy); faz[r] = (float) (az[r] - tcz);
     }
@@ -644,8 +653,17 @@ k_count(const FkArgs a) {
     auto pair = [&](int r, int j, bool self) -> int {
       int idx;
       if (!F32) {
-        if (GEOM == FCFC_GEOM_BOX)
-          idx = fk_pair_box64<BIN, HYB, MUT>(cx[j] - ax[r], cy[j] - ay[r], cz[j] - az[r], bn, stab, ptab, mutab, s2e, pe);
+        if (GEOM == FCFC_GEOM_BOX) {
+          if (MODE == MODE_GRAD && (cc[j] & 32)) {
+            const int cd = cc[j] & 31;
+            const double ux = ax[r] - (double) (cd % 3 - 1) * J.g.Lx;
+            const double uy = ay[r] - (double) ((cd / 3) % 3 - 1) * J.g.Ly;
+            const double uz = az[r] - (double) (cd / 9 - 1) * J.g.Lz;
+            idx = fk_pair_box64<BIN, HYB, MUT>(cx[j] - ux, cy[j] - uy, cz[j] - uz, bn, stab, ptab, mutab, s2e, pe);
+          }
+          else
+            idx = fk_pair_box64<BIN, HYB, MUT>(cx[j] - ax[r], cy[j] - ay[r], cz[j] - az[r], bn, stab, ptab, mutab, s2e, pe);
+        }
         else
           idx = fk_pair_sv64<BIN, HYB, MUT>(ax[r], ay[r], az[r], as[r], cx[j], cy[j], cz[j],
               NEED_S ? cs_[j] : 0.0, bn, stab, ptab, mutab, s2e, pe);
@@ -704,7 +722,7 @@ k_count(const FkArgs a) {
           if (F32) { fx[q] = (float) (x - tcx); fy[q] = (float) (y - tcy); fz[q] = (float) (z - tcz); cc[q] = 13; }
           else { cx[q] = x; cy[q] = y; cz[q] = z; }
           if (NEED_S) cs_[q] = J.A.s[jg];
-          if (NEED_W) cw[q] = J.A.w[jg];
+          if (NEED_W) cw[q] = jw ? J.A.w[jg] : 1.0;
           cj[q] = jg;
         }
         __syncthreads();
@@ -751,15 +769,20 @@ k_count(const FkArgs a) {
           const int jg = st->b0[sgi] + rem;
           const int code = st->code[sgi];
           double x = J.B.x[jg], y = J.B.y[jg], z = J.B.z[jg];
-          if (GEOM == FCFC_GEOM_BOX && code != 13) {
-            x = x + (double) (code % 3 - 1) * J.g.Lx;
-            y = y + (double) ((code / 3) % 3 - 1) * J.g.Ly;
-            z = z + (double) (code / 9 - 1) * J.g.Lz;
+          /* gradient: pairs that the count evaluates from the other object's
+           * tile keep that orientation, (a - D) - b (see fk_strip_segments) */
+          const bool swp = (MODE == MODE_GRAD) && (J.swap_all || (code & 32));
+          if (GEOM == FCFC_GEOM_BOX && (code & 31) != 13 && !swp) {
+            const int cd = code & 31;
+            x = x + (double) (cd % 3 - 1) * J.g.Lx;
+            y = y + (double) ((cd / 3) % 3 - 1) * J.g.Ly;
+            z = z + (double) (cd / 9 - 1) * J.g.Lz;
           }
           if (F32) { fx[q] = (float) (x - tcx); fy[q] = (float) (y - tcy); fz[q] = (float) (z - tcz); cc[q] = code; }
           else { cx[q] = x; cy[q] = y; cz[q] = z; }
+          if (MODE == MODE_GRAD) cc[q] = (code & 31) | (swp ? 32 : 0);
           if (NEED_S) cs_[q] = J.B.s[jg];
-          if (NEED_W) cw[q] = J.B.w[jg];
+          if (NEED_W) cw[q] = jw ? J.B.w[jg] : 1.0;
           cj[q] = jg;
         }
         __syncthreads();
@@ -843,7 +866,7 @@ static void *ws(fcfc_gpu_ctx *c, int slot, size_t bytes, int *err) {
   DevBuf &b = c->bufs[slot];
   if (bytes == 0) bytes = 8;
   if (b.cap < bytes) {
-    if (b.p) { (void) hipFree(b.p); b.p = nullptr; b.cap = 0; }
+    if (b.p) { (void) hipStreamSynchronize(c->stream); (void) hipFree(b.p); b.p = nullptr; b.cap = 0; }
     size_t nb = bytes + bytes / 8 + 256;
     if (hipMalloc(&b.p, nb) != hipSuccess) {
       set_err(c, "hipMalloc of %zu bytes failed", nb);
@@ -1047,6 +1070,13 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
       if (autoc && mode != MODE_GRAD) {
         if (dx < 0 || (dx == 0 && dy < 0)) continue;
         if (dx == 0 && dy == 0) diag = 1;
+      }
+      else if (autoc) {
+        /* gradient of auto counts: full shell; flag the pairs that the count
+         * evaluates from the other object's tile (negative half plane: 2;
+         * the own-row strip, whose upper z image is taken from below: 4) */
+        if (dx < 0 || (dx == 0 && dy < 0)) diag = 2;
+        if (dx == 0 && dy == 0) diag = 4;
       }
       P->strips.push_back(dx); P->strips.push_back(dy); P->strips.push_back(diag);
     }
@@ -1310,7 +1340,7 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
 struct PJob {
   JobPlan P;
   DevCat A, B;
-  int autoc = 0, wt = 0;
+  int autoc = 0, wt = 0, swap_all = 0;
   double *gA = nullptr;
   long long hist_off = 0;
 };
@@ -1441,6 +1471,7 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     f.nstrips = J.P.nstrips;
     f.autoc = J.autoc;
     f.wt = J.wt;
+    f.swap_all = J.swap_all;
     f.hist_off = J.hist_off;
     f.gA = J.gA;
     if (m == MODE_C32) fp32_windows(b, J.P.g, J.P.zmaxc, &f.win);
@@ -1621,6 +1652,9 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
     const fcfc_cat *pa = p ? b : a, *pb = p ? a : b;
     int e = setup_job(c, al, copies, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &jobs[p]);
     if (e) return e;
+    /* second pass of a cross gradient: the count evaluated every pair with
+     * the other catalogue as A */
+    jobs[p].swap_all = p;
     jobs[p].gA = (double *) al.get(p ? WS_GB : WS_GA, sizeof(double) * (size_t) (pa->n > 0 ? pa->n : 1));
     if (al.err) return al.err;
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, jobs[p].gA, (int64_t) pa->n, 0.0);

@@ -372,7 +372,12 @@ static int run_pairs(int geom, const double *bsize, const orc_cat *A,
           }
         }
       }
-      if (G && ga) ga[i] += gacc;
+      if (G && ga) {
+        /* atomic: for auto gradients ga and gb are the same array, which
+         * other threads update at index i as their pair partner */
+#pragma omp atomic
+        ga[i] += gacc;
+      }
     }
 #pragma omp critical
     {

@@ -1,0 +1,53 @@
+"""Batched counting (many catalogue pairs in one persistent launch) and the
+autotuner: every job's histogram must equal the single-job count exactly."""
+import numpy as np
+import pytest
+
+import fcfc_amd as F
+
+pytestmark = pytest.mark.gpu
+L = 1000.0
+
+
+@pytest.fixture(scope="module")
+def eng():
+    with F.Engine() as e:
+        yield e
+
+
+def _mocks(b, n):
+    sizes = F.gen_sizes(7, n, 300, 3000)
+    out = []
+    for k, nk in enumerate(sizes):
+        nh = max(1, int(nk) // 100)
+        x, y, z, w = F.gen_clustered(100 + k, int(nk) // 2, nh, 50, 2.0, L, 0.3)
+        xr, yr, zr = F.gen_uniform(500 + k, 3 * int(nk), L)
+        out.append((F.Catalog(b, x, y, z, w), F.Catalog(b, xr, yr, zr)))
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_equals_single(eng, prec):
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=20, bsize=L)
+    mocks = _mocks(b, 12)
+    jobs = []
+    for D, R in mocks:
+        jobs += [(D, None, True, True), (D, R, False, True), (R, None, True, False)]
+    print(f"case batch prec={prec} jobs={len(jobs)}", flush=True)
+    got = eng.count_batch(jobs, b, precision=prec)
+    for k, (a, c, au, wt) in enumerate(jobs):
+        ref = eng.count_pairs(a, c, b, isauto=au, withwt=wt, precision=prec)
+        if wt:
+            np.testing.assert_allclose(got[k], ref, rtol=1e-12, atol=0)
+        else:
+            np.testing.assert_array_equal(got[k], ref)
+
+
+def test_autotune_keeps_counts(eng):
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_ISO, 0, 200, 5, bsize=L)
+    x, y, z = F.gen_uniform(9, 20000, L)
+    D = F.Catalog(b, x, y, z)
+    ref = eng.count_pairs(D, None, b, isauto=True)
+    cd, zd, tile, est = eng.autotune(D, None, b, isauto=True)
+    assert cd in (3, 4, 6, 8) and zd in (2, 4, 8) and tile in (256, 512) and est > 0
+    np.testing.assert_array_equal(eng.count_pairs(D, None, b, isauto=True), ref)
