@@ -147,7 +147,9 @@ def xi_multipoles_torch(wd, wr, eng, D, R, bins, rr_cnt=None, poles=(0, 2)):
     ddn = dd / (Wd * (Wd - 1))
     drn = dr / (Wd * Wr)
     rrn = rr / (Wr * (Wr - 1))
-    xi = ((ddn - 2 * drn + rrn) / rrn).reshape(nmu, ns)
+    # bins without random pairs carry no information (FCFC would print inf/nan)
+    safe = torch.where(rrn > 0, rrn, torch.ones_like(rrn))
+    xi = torch.where(rrn > 0, (ddn - 2 * drn + rrn) / safe, torch.zeros_like(rrn)).reshape(nmu, ns)
     mu = (torch.arange(nmu, dtype=torch.float64, device=xi.device) + 0.5) / nmu
     out = []
     for ell in poles:

@@ -67,7 +67,9 @@ def test_box_cli_matches_reference(tmp_path):
         # multipoles from xi(s,mu) with the midpoint rule (eval_cf.c:326-354)
         from fcfc_amd.xi import multipoles
         ml = multipoles(xi[:, 4].reshape(100, 40), 100, (0, 2, 4))
-        np.testing.assert_allclose(xil[:, 3:].T, ml, rtol=1e-8, atol=1e-12)
+        # xi.txt holds 10 significant digits (%.10lg)
+        np.testing.assert_allclose(xil[:, 3:].T, ml, rtol=1e-7,
+                                   atol=1e-9 * np.abs(xi[:, 4]).max())
 
 
 def test_survey_cli_matches_reference(tmp_path):
