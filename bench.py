@@ -7,13 +7,18 @@ DD (weighted), DR (weighted, unit random weights) and RR (unweighted) in
 (s, mu) bins: 40 linear s bins in [0, 200) x 100 mu bins.  Synthetic data
 from the project's seeded generator, resident in HBM before timing.
 
-One step = the three counts (grid build + pair counting) with the FP32
-distance kernel whose pairs near a bin edge are re-evaluated in FP64.  The
-same step is then run once with the plain FP64 kernel; counts of the two are
-compared (integer counts must be identical, weighted ones within 1e-12).
+One step = the three counts (grid build + pair counting).  Three forms of
+the count are measured on the same data:
+  fp64   FP64 distances; the mu bin from the hardware reciprocal square root,
+         settled exactly near its edges (the headline path, `value`);
+  plain  FP64 with the reference's per-pair arithmetic transcribed directly
+         (an IEEE division per (s,mu) pair) -- the "plain FP64 GPU kernel";
+  fp32   FP32 distances, pairs near a bin edge re-evaluated in FP64.
+Integer counts of all forms must be identical, weighted ones within 1e-12.
 
 Prints ONE JSON line.  `value` = candidate pair-distance evaluations per
-second of the FP32 + recheck step.
+second of the timed steps (whole job: grid build, tiling and counting, data
+resident in HBM).
 """
 import argparse
 import json
@@ -103,6 +108,20 @@ def cpu_baseline(target_s=15.0):
                       f"count phase {t:.2f} s, {pairs:.4g} unordered in-range pairs"}
 
 
+def traffic_from_profile():
+    """HBM bytes per launch of the RR count kernel, from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    kernel calibration recorded there); None when absent."""
+    fn = os.path.join(ROOT, "profiles", "k_count_smu_box_RR_counters.txt")
+    try:
+        for l in open(fn):
+            if l.startswith("hbm_bytes_per_launch"):
+                return float(l.split()[1])
+    except OSError:
+        pass
+    return None
+
+
 def mocks_workload(args):
     """256 periodic-box mocks (L = 1000): sizes log-uniform in [5e4, 5e5] from
     the seed, each half uniform / half in halos (sigma = 2) with lognormal
@@ -183,10 +202,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0,
                     help="fraction of the workload sizes (1 = the headline configuration)")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "plain"], default="fp64",
+                    help="form of the timed steps (the others run one step each)")
+    ap.add_argument("--skip", nargs="*", default=[], choices=["fp64", "fp32", "plain"],
+                    help="forms not to run for comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["c2", "mocks"], default="c2",
                     help="c2: the headline clustered box DD/DR/RR; mocks: 256 batched mocks")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="start with a 1 GiB device-to-device copy (calibrates HBM byte "
+                         "counters of a profiler run)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick grid/tile shapes with the autotuner before timing")
     args = ap.parse_args()
@@ -201,6 +226,13 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calib:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del src, dst
     if args.workload == "mocks":
         mocks_workload(args)
         return
@@ -217,9 +249,10 @@ def main():
     if args.autotune:
         for name, a, c, au, wt in (("DD", D, None, True, True), ("DR", D, R, False, True),
                                    ("RR", R, None, True, False)):
-            log(f"autotune {name}: {eng.autotune(a, c, b, isauto=au, withwt=wt, precision='fp32')}")
+            log(f"autotune {name}: {eng.autotune(a, c, b, isauto=au, withwt=wt, precision=args.precision)}")
+    prec = args.precision
     for i in range(args.warmup):
-        run_step(eng, b, D, R, "fp32")
+        run_step(eng, b, D, R, prec)
         log(f"warmup step {i} done")
     torch.cuda.synchronize()
     t1 = time.time()
@@ -228,7 +261,7 @@ def main():
     rechk = 0.0
     res = None
     for i in range(args.steps):
-        res, ev, km, rc = run_step(eng, b, D, R, "fp32")
+        res, ev, km, rc = run_step(eng, b, D, R, prec)
         evals += ev
         rechk += rc
         for k in kms:
@@ -241,43 +274,51 @@ def main():
         "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
+        "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": "box L=1000 Mpc/h; D=2e6 clustered (1e6 uniform + 2000 halos x 500, "
                                "sigma=2) lognormal(0,0.3) weights; R=2e7 uniform; DD+DR+RR "
                                "(s,mu) 40x100 in [0,200)", "scale": args.scale,
-                   "n_data": D.n, "n_random": R.n, "precision": "fp32 distances, fp64 recheck "
-                   "near bin edges, fp64 weighted sums"},
+                   "n_data": D.n, "n_random": R.n, "precision": prec},
     }
     step_evals = evals / args.steps
     line["evals_per_step"] = step_evals
-    line["rechecked_fraction"] = rechk / max(evals, 1)
-    # dominant kernel: RR (the largest count), FP32 path, unweighted
+    if prec == "fp32":
+        line["rechecked_fraction"] = rechk / max(evals, 1)
+    # dominant kernel: the RR count (unweighted)
     rr_ms = kms["RR"] / args.steps
     rr_evals = res["RR_evals"]
+    peak = PEAK_FP32_VALU if prec == "fp32" else PEAK_FP64_VALU
     ach = rr_evals * FLOPS_PAIR["smu"] / (rr_ms * 1e-3)
-    line["roofline"] = {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32_VALU / 1e12,
-                        "unit": "TFLOP/s", "frac": ach / PEAK_FP32_VALU, "traffic": None,
-                        "kernel": "k_count (s,mu) FP32+recheck, RR", "kernel_ms": rr_ms,
-                        "flops_per_pair": FLOPS_PAIR["smu"], "evals": rr_evals}
+    line["roofline"] = {"bound": "valu", "achieved": ach / 1e12, "peak": peak / 1e12,
+                        "unit": "TFLOP/s", "frac": ach / peak, "traffic": traffic_from_profile(),
+                        "kernel": f"k_count (s,mu) box {prec}, RR auto count", "kernel_ms": rr_ms,
+                        "flops_per_pair": FLOPS_PAIR["smu"], "evals": rr_evals,
+                        "evals_per_s": rr_evals / (rr_ms * 1e-3)}
     line["kernel_ms"] = {k: v / args.steps for k, v in kms.items()}
+    # in-range pairs of RR (unordered), the algorithm-independent rate
+    line["inrange_pairs_per_s_RR"] = float(res["RR"].sum()) / (rr_ms * 1e-3)
 
-    if not args.no_fp64:
+    # the other forms, one step each, on the same data
+    forms = {}
+    for other in [f for f in ("plain", "fp64", "fp32") if f != prec and f not in args.skip]:
         t2 = time.time()
-        res64, ev64, km64, _ = run_step(eng, b, D, R, "fp64")
+        ro, evo, kmo, rco = run_step(eng, b, D, R, other)
         torch.cuda.synchronize()
-        dt64 = time.time() - t2
-        line["fp64"] = {"evals_per_s": ev64 / dt64, "ms_per_step": dt64 * 1e3,
-                        "kernel_ms": km64}
-        line["speedup_vs_fp64"] = (ev64 / dt64) and value / (ev64 / dt64)
-        rr_equal = bool(np.array_equal(res["RR"], res64["RR"]))
-        wrel = max(float(np.max(np.abs(res[k] - res64[k]) / np.maximum(np.abs(res64[k]), 1e-300)))
+        dto = time.time() - t2
+        eq = {k: bool(np.array_equal(res[k], ro[k])) for k in ("RR",)}
+        wrel = max(float(np.max(np.abs(res[k] - ro[k]) / np.maximum(np.abs(ro[k]), 1e-300)))
                    for k in ("DD", "DR"))
-        line["counts_bit_exact"] = {"RR_fp32_eq_fp64": rr_equal, "weighted_max_rel": wrel}
-        # in-range pairs: exact unweighted totals of RR; DD and DR pair
-        # numbers from an unweighted FP64 count would cost another step, so
-        # the in-range rate is quoted for RR alone
-        rr_pairs = float(res64["RR"].sum())
-        line["inrange_pairs_per_s_RR"] = rr_pairs / (km64["RR"] * 1e-3)
+        forms[other] = {"evals_per_s": evo / dto, "ms_per_step": dto * 1e3, "kernel_ms": kmo,
+                        "RR_evals_per_s_kernel": ro["RR_evals"] / (kmo["RR"] * 1e-3),
+                        "RR_identical": eq["RR"], "weighted_max_rel": wrel}
+        if other == "fp32":
+            forms[other]["rechecked_fraction"] = rco / max(evo, 1)
+        log(f"{other} step done: {evo / dto:.4g} evals/s")
+    line["forms"] = forms
+    if "plain" in forms:
+        line["speedup_vs_plain_fp64"] = value / forms["plain"]["evals_per_s"]
+    line["counts_bit_exact"] = all(f["RR_identical"] and f["weighted_max_rel"] <= 1e-12
+                                   for f in forms.values())
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     eng.close()
     print(json.dumps(line), flush=True)

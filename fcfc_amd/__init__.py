@@ -41,6 +41,17 @@ class FcfcError(RuntimeError):
         self.code = code
 
 
+def _prec_flag(precision):
+    """'fp64': FP64 distances with the fast exact mu bin; 'fp32': FP32
+    distances with FP64 re-evaluation near bin edges; 'plain': the
+    reference's per-pair FP64 arithmetic transcribed directly."""
+    try:
+        return {"fp64": L.FCFC_GPU_PREC_FP64, "fp32": L.FCFC_GPU_PREC_FP32,
+                "plain": L.FCFC_GPU_PREC_PLAIN}[precision]
+    except KeyError:
+        raise ValueError("precision must be 'fp64', 'fp32' or 'plain'") from None
+
+
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
 
@@ -200,6 +211,19 @@ class Catalog:
         return c
 
 
+def _job_array(jobs, bins):
+    arr = (L.fcfc_batch_job * len(jobs))()
+    bs = bins.bsize_rescaled
+    for k, (a, b, isauto, withwt) in enumerate(jobs):
+        arr[k].a = a._c()
+        arr[k].b = (b if b is not None else a)._c()
+        if bs is not None:
+            for i in range(3):
+                arr[k].bsize[i] = bs[i]
+        arr[k].flags = (L.FCFC_GPU_AUTO if isauto else 0) | (L.FCFC_GPU_WEIGHTED if withwt else 0)
+    return arr
+
+
 class Engine:
     """One HIP device context (fcfc_gpu_init)."""
 
@@ -245,8 +269,7 @@ class Engine:
         """Pick grid/tile shapes for counts like this one (cached in the
         engine per density bucket).  Returns (cell_div, zdiv, tile, est_ms)."""
         flags = (L.FCFC_GPU_AUTO if isauto else 0) | (L.FCFC_GPU_WEIGHTED if withwt else 0)
-        if precision == "fp32":
-            flags |= L.FCFC_GPU_PREC_FP32
+        flags |= _prec_flag(precision)
         best = (C.c_int * 4)()
         bs = bins.bsize_rescaled
         ca = a._c()
@@ -262,10 +285,7 @@ class Engine:
         """Histogram of unordered (auto) or all (cross) pairs; the result is
         added to `out` when given (FCFC accumulates into cnt)."""
         flags = (L.FCFC_GPU_AUTO if isauto else 0) | (L.FCFC_GPU_WEIGHTED if withwt else 0)
-        if precision == "fp32":
-            flags |= L.FCFC_GPU_PREC_FP32
-        elif precision != "fp64":
-            raise ValueError("precision must be 'fp64' or 'fp32'")
+        flags |= _prec_flag(precision)
         if census:
             flags |= L.FCFC_GPU_CENSUS
         if out is None:
@@ -296,22 +316,20 @@ class Engine:
         self._check(e, "fcfc_gpu_count_backward")
         return (ga, None) if isauto else (ga, gb)
 
+    def plan(self, jobs, bins, precision="fp64"):
+        """A prepared batch (see fcfc_gpu_plan_create): jobs as for
+        count_batch, catalogues on the device.  plan.run() recounts them,
+        re-reading the weights from the catalogues' device arrays."""
+        return Plan(self, jobs, bins, precision)
+
     def count_batch(self, jobs, bins, precision="fp64"):
-        """jobs: list of (a, b_or_None, isauto, withwt).  Returns an array
-        (njob, ntot) of float64 (weighted jobs) / int64 bit patterns viewed
-        per job: use the returned list."""
+        """jobs: list of (a, b_or_None, isauto, withwt).  Returns one
+        histogram per job (float64 for weighted jobs, int64 otherwise)."""
         n = len(jobs)
-        arr = (L.fcfc_batch_job * n)()
-        for k, (a, b, isauto, withwt) in enumerate(jobs):
-            arr[k].a = a._c()
-            arr[k].b = (b if b is not None else a)._c()
-            bs = bins.bsize_rescaled
-            if bs is not None:
-                for i in range(3):
-                    arr[k].bsize[i] = bs[i]
-            arr[k].flags = (L.FCFC_GPU_AUTO if isauto else 0) | (L.FCFC_GPU_WEIGHTED if withwt else 0)
+        arr = _job_array(jobs, bins)
         raw = np.zeros((max(n, 1), bins.ntot), dtype=np.int64)
-        flags = L.FCFC_GPU_PREC_FP32 if precision == "fp32" else 0
+        flags = _prec_flag(precision)
+        self._sync_torch(*[c for j in jobs for c in j[:2] if c is not None])
         e = lib().fcfc_gpu_count_batch(self._h, n, arr, C.byref(bins._b), flags, raw.ctypes.data)
         self._check(e, "fcfc_gpu_count_batch")
         out = []
@@ -324,6 +342,45 @@ class Engine:
         self._check(lib().fcfc_gpu_last_stats(self._h, C.byref(s)), "fcfc_gpu_last_stats")
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+
+
+class Plan:
+    """Prepared batch replayed as a HIP graph (fcfc_gpu_plan_*).  The jobs'
+    catalogues must stay alive (their device arrays are re-read by run())."""
+
+    def __init__(self, eng, jobs, bins, precision="fp64"):
+        self._eng = eng
+        self._jobs = list(jobs)
+        self._keep = [c for j in self._jobs for c in j[:2] if c is not None]
+        self.bins = bins
+        arr = _job_array(self._jobs, bins)
+        eng._sync_torch(*self._keep)
+        h = C.c_void_p()
+        e = lib().fcfc_gpu_plan_create(eng._h, len(self._jobs), arr, C.byref(bins._b),
+                                       _prec_flag(precision), C.byref(h))
+        eng._check(e, "fcfc_gpu_plan_create")
+        self._h = h
+
+    def run(self):
+        """Count every job again; returns one histogram per job."""
+        n = len(self._jobs)
+        raw = np.zeros((n, self.bins.ntot), dtype=np.int64)
+        self._eng._sync_torch(*self._keep)
+        e = lib().fcfc_gpu_plan_run(self._h, raw.ctypes.data)
+        self._eng._check(e, "fcfc_gpu_plan_run")
+        return [raw[k].view(np.float64).copy() if j[3] else raw[k].copy()
+                for k, j in enumerate(self._jobs)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fcfc_gpu_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 def pair_counts(cnt, a, b, isauto, withwt):
     """The caller conventions of FCFC's eval_pairs (eval_cf.c:117-167):

@@ -26,6 +26,7 @@ FCFC_GPU_PREC_FP64 = 0x00
 FCFC_GPU_PREC_FP32 = 0x10
 FCFC_GPU_CNT_DEVICE = 0x20
 FCFC_GPU_CENSUS = 0x40
+FCFC_GPU_PREC_PLAIN = 0x80
 
 _dp = C.POINTER(C.c_double)
 
@@ -86,6 +87,11 @@ SIGNATURES = {
     "fcfc_gpu_last_stats": (C.c_int, [C.c_void_p, C.POINTER(fcfc_gpu_stats)]),
     "fcfc_gpu_count_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(fcfc_batch_job),
                                        C.POINTER(fcfc_bins), C.c_uint32, C.c_void_p]),
+    "fcfc_gpu_plan_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(fcfc_batch_job),
+                                       C.POINTER(fcfc_bins), C.c_uint32,
+                                       C.POINTER(C.c_void_p)]),
+    "fcfc_gpu_plan_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fcfc_gpu_plan_destroy": (C.c_int, [C.c_void_p]),
     "fcfc_gpu_count_backward": (C.c_int, [C.c_void_p, C.POINTER(fcfc_cat),
                                           C.POINTER(fcfc_cat), C.POINTER(fcfc_bins), _dp,
                                           C.c_uint32, _dp, _dp, _dp]),

@@ -16,6 +16,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "fcfc_gpu.h"
@@ -24,6 +26,8 @@
 #define MODE_C64  0   /* FP64 counts                                    */
 #define MODE_C32  1   /* FP32 distances + FP64 recheck near bin edges   */
 #define MODE_GRAD 2   /* weight gradient (FP64 bins)                    */
+#define MODE_C32C 3   /* MODE_C32 with the rounding census of the rechecked pairs */
+#define MODE_P64  4   /* FP64, the reference's per-pair arithmetic as written */
 
 /* ======================================================================== */
 /*                          grid build kernels                              */
@@ -151,10 +155,17 @@ __global__ void k_tile_emit(int nrow, int ncz, const int *__restrict__ cstart,
 struct FkWin {
   float e_s2;                 /* absolute window on s^2 (or s_perp^2)       */
   float e_pi;                 /* absolute window on pi                      */
-  float e_rrel, e_rabs, e_rz; /* mu window: dt = t e_rrel + inv_s (t e_rabs + e_rz) */
-  float s2_tiny;              /* below this s^2 always re-evaluate          */
+  float mu_w;                 /* mu bin certain if |fract(t) - 1/2| <= mu_w */
+  float s2_floor;             /* pairs below this s^2 are re-evaluated      */
   float nmu_f;
 };
+
+/* FP32 tables: entry = bin (low 16 bits) | flags; index (int) s2 - soff + 1
+ * (entry 0: the integer cell below the table, entries len + 1 and len + 2:
+ * above it, the last one without flags). */
+#define FK_F_LO   0x10000u    /* the lower end of the integer cell is an edge */
+#define FK_F_HI   0x20000u    /* the upper end is an edge                     */
+#define FK_F_ALW  0x40000u    /* hybrid cell: always re-evaluate              */
 
 /* One counting job: a catalogue pair and its histogram.  A launch counts the
  * tiles of one or many jobs (batched mocks); tiles carry their job index. */
@@ -184,7 +195,7 @@ struct FkArgs {
   unsigned long long *stats;  /* evals, rechecked, census */
   int nh;                     /* LDS histogram copies */
   /* LDS byte offsets */
-  int off_hist, off_stab, off_ptab, off_mutab, off_edges, off_strip, off_chunk;
+  int off_hist, off_stab, off_ptab, off_edges, off_strip, off_chunk, off_queue;
   int census;
   const double *G;            /* GRAD: dL/dcnt */
 };
@@ -224,30 +235,29 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
   const int swapf = (diag & 2) ? 32 : 0;   /* gradient: count evaluates from the other tile */
   diag &= 1;
   const int cxy = (cx + 1) + 3 * (cy + 1) + swapf;
-  int ns = 0;
   if (g.periodic) {
     if (izhi - izlo + 1 > g.ncz) {   /* cannot happen with the tile limits */
       izlo = 0; izhi = g.ncz - 1;
     }
     if (izlo < 0) {                  /* image below z = 0: B shifted by -Lz */
       int c0 = izlo + g.ncz, c1 = g.ncz - 1;
-      b0[ns] = cs[c0]; len[ns] = cs[c1 + 1] - cs[c0]; code[ns] = cxy + 9 * 0; ns++;
+      b0[0] = cs[c0]; len[0] = cs[c1 + 1] - cs[c0]; code[0] = cxy + 9 * 0;
     }
     {
       int c0 = max(izlo, 0), c1 = min(izhi, g.ncz - 1);
       if (diag) {
         /* auto counts, same row: only objects after the tile (j > i) */
         int e = cs[c1 + 1];
-        b0[ns] = t.p1; len[ns] = max(e - t.p1, 0); code[ns] = cxy + 9 * 1; ns++;
+        b0[1] = t.p1; len[1] = max(e - t.p1, 0); code[1] = cxy + 9 * 1;
       }
-      else if (c1 >= c0) { b0[ns] = cs[c0]; len[ns] = cs[c1 + 1] - cs[c0]; code[ns] = cxy + 9 * 1; ns++; }
+      else if (c1 >= c0) { b0[1] = cs[c0]; len[1] = cs[c1 + 1] - cs[c0]; code[1] = cxy + 9 * 1; }
     }
     if (izhi >= g.ncz && !diag) {    /* image above: B shifted by +Lz */
       int c0 = 0, c1 = izhi - g.ncz;
       /* gradient of auto counts, same row: the count takes this image from
        * the lower tile (the (0,0) strip keeps the lower wrap only) */
       const int sw = (dx == 0 && dy == 0 && (diag_in & 4)) ? 32 : 0;
-      b0[ns] = cs[c0]; len[ns] = cs[c1 + 1] - cs[c0]; code[ns] = (cxy | sw) + 9 * 2; ns++;
+      b0[2] = cs[c0]; len[2] = cs[c1 + 1] - cs[c0]; code[2] = (cxy | sw) + 9 * 2;
     }
   }
   else {
@@ -262,132 +272,165 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
 
 /* ---------------- FP64 pair evaluation (the arithmetic contract) -------- */
 
+/* Pair functions return a histogram index; every value >= ntot means "not
+ * counted".  Table sentinels are FK_OUT = 0xFFFF >= ntot (the histogram
+ * lives in LDS, so ntot < 40960), hence sb + ns * mb needs no range test of
+ * its own: an out-of-range s (or pi) gives sb >= ntot, a dropped mu gives
+ * mb = nmu and sb + ns * nmu >= ntot. */
+#define FK_NONE 0xFFFFFFFFu
+#define FK_AMB  0xFFFFFFFEu   /* FP32 path: within a rounding bound of an edge */
+#define FK_DBL_EPS 2.220446049250313e-16
+#define FK_FAR 1.0e30          /* coordinates of idle lanes: every pair out */
+
 /* s^2 (or s_perp^2) bin from the LDS table, FK_OUT outside [s2min, s2max).
  * Integer tables (setup_cf.c:385-410): index (int) s2 - (int) s2min, exactly
  * the reference's cvtt lookup (metric_common.c:500-510); the two sentinel
  * entries perform the range check.  Hybrid tables refine downwards against
  * the edges (FCFC_LOOKUP_HYBRID, metric_common.c:56-64). */
-template <bool HYB>
+template <bool HYB, bool MK = false>
 __device__ __forceinline__ unsigned fk_sbin(double s2, const FkBins &bn,
-    const uint16_t *stab, const double *s2e) {
+    const unsigned *stab, const double *s2e) {
   if (HYB) {
     if (!(s2 < bn.s2max) || (bn.s2min != 0 && s2 < bn.s2min)) return FK_OUT;
-    int k = (int) s2 - bn.soff + 1;
-    k = k < 1 ? 1 : (k > bn.stab_len ? bn.stab_len : k);
-    unsigned sb = stab[k];
+    int k = (int) s2 - bn.soff;
+    k = k < 0 ? 0 : (k >= bn.stab_len ? bn.stab_len - 1 : k);
+    unsigned sb = MK ? (stab[k] & 0xFFFFu) : stab[k];
     if (sb >= (unsigned) bn.ns) {
       sb -= bn.ns;
       while (sb != 0 && s2 < s2e[sb]) sb--;
     }
     return sb;
   }
-  unsigned k = (unsigned) ((int) s2 - bn.soff + 1);
-  k = min(k, (unsigned) (bn.stab_len + 1));
-  return stab[k];
+  /* (int) s2 - soff below zero wraps to a large unsigned value: clamped to
+   * the upper sentinel like s2 >= s2max */
+  const unsigned k = min((unsigned) ((int) s2 - bn.soff), (unsigned) bn.stab_len);
+  return MK ? (stab[k] & 0xFFFFu) : stab[k];
 }
 
-template <bool HYB>
+template <bool HYB, bool MK = false>
 __device__ __forceinline__ unsigned fk_pbin(double p, const FkBins &bn,
-    const uint16_t *ptab, const double *pe) {
+    const unsigned *ptab, const double *pe) {
   if (HYB) {
     if (!(p < bn.pmax) || p < bn.pmin) return FK_OUT;
-    int k = (int) p - bn.poff + 1;
-    k = k < 1 ? 1 : (k > bn.ptab_len ? bn.ptab_len : k);
-    unsigned pb = ptab[k];
+    int k = (int) p - bn.poff;
+    k = k < 0 ? 0 : (k >= bn.ptab_len ? bn.ptab_len - 1 : k);
+    unsigned pb = MK ? (ptab[k] & 0xFFFFu) : ptab[k];
     if (pb >= (unsigned) bn.np) {
       pb -= bn.np;
       while (pb != 0 && p < pe[pb]) pb--;
     }
     return pb;
   }
-  unsigned k = (unsigned) ((int) p - bn.poff + 1);
-  k = min(k, (unsigned) (bn.ptab_len + 1));
-  return ptab[k];
+  const unsigned k = min((unsigned) ((int) p - bn.poff), (unsigned) bn.ptab_len);
+  return MK ? (ptab[k] & 0xFFFFu) : ptab[k];
 }
 
-/* mu bin of the reference's AVX-512 path: m = floor of the exact quotient
- * fl(num * nmu^2) / s2 (trunc(RZ(.)), metric_common.c:484-486), m = 0 when
- * s2 < DBL_EPSILON, pair dropped (-1) when m >= nmu^2 (mu = 1), bin =
- * mutab[m] = isqrt(m).  The quotient comes from the hardware reciprocal
- * plus one residual correction; only when it lies within 1e-6 of an integer
- * is the floor settled with exact-sign fma residuals. */
-template <bool MUT>
-__device__ __forceinline__ int fk_mubin(double num, double s2, const FkBins &bn,
-    const uint8_t *mutab) {
+/* mu bin of (s, mu) counts.  The reference takes m = floor of the exact
+ * quotient a / s2 with a = fl(num * nmu^2) (trunc of the RZ division,
+ * metric_common.c:484-486), sets m = 0 when s2 < DBL_EPSILON, drops the pair
+ * when m >= nmu^2 and bins it in mutab[m] = isqrt(m) (setup_cf.c:513-531).
+ * Since isqrt(floor(q)) = floor(sqrt(q)) for q >= 0, the bin is floor(t),
+ * t = sqrt(a / s2), and the drop rule is floor(t) >= nmu.
+ * The caller passes t_apx, t from hardware reciprocal square roots (relative
+ * error below 2^-21 including the products).  When t_apx is farther than
+ * nmu 2^-20 from every integer its floor is the bin; otherwise, and for all
+ * s2 below s2_exact = s2max 2^-12 (which covers s2 < DBL_EPSILON and bounds
+ * t_apx by nmu (1 + 2^-21) on the fast path), the bin is settled exactly:
+ * k = rint(t_apx) is the bin if a - k^2 s2 >= 0 (one fma, exact sign; k^2
+ * is exact), else k - 1. */
+__device__ __forceinline__ unsigned fk_mu_exact(double a, double s2, double t_apx,
+    const FkBins &bn) {
+  if (!(s2 >= FK_DBL_EPS)) return 0u;
+  double k = __builtin_rint(t_apx);
+  if (__builtin_fma(-(k * k), s2, a) < 0.0) k -= 1.0;
+  return (unsigned) (int) __builtin_fmin(k, bn.nmu_d);
+}
+
+/* The same mu bin transcribed directly from the reference's arithmetic:
+ * the correctly rounded quotient q = fl(a / s2), floored, with the one
+ * correction that turns round-to-nearest into truncation of the exact
+ * quotient (q can only round up onto an integer k, and then a - k s2 < 0),
+ * and the isqrt table mapping m to the bin. */
+__device__ __forceinline__ unsigned fk_mu_plain(double num, double s2, const FkBins &bn) {
+  if (!(s2 >= FK_DBL_EPS)) return 0u;
   const double a = num * bn.nmu2;
-  const double r = __builtin_amdgcn_rcp(s2);
-  const double q0 = a * r;
-  const double q1 = __builtin_fma(__builtin_fma(-q0, s2, a), r, q0);
-  int m = (int) q1;
-  const double f = q1 - (double) m;
-  if (f < 1e-6 || f > 1.0 - 1e-6) {
-    double mm = (double) m;
-    if (__builtin_fma(-mm, s2, a) < 0.0) mm -= 1.0;
-    else if (__builtin_fma(-(mm + 1.0), s2, a) >= 0.0) mm += 1.0;
-    m = (int) mm;
-  }
-  if (!(s2 >= 2.220446049250313e-16)) m = 0;
-  if (m >= bn.nmu * bn.nmu) return -1;
-  return MUT ? (int) mutab[m] : fk_isqrt(m);
+  const double q = a / s2;
+  double m = __builtin_floor(q);
+  if (m == q && __builtin_fma(-m, s2, a) < 0.0) m -= 1.0;
+  if (m >= bn.nmu2) return (unsigned) bn.nmu;
+  return (unsigned) fk_isqrt((int) m);
 }
 
-/* Box pair, B already at its periodic image: histogram index or -1. */
-template <int BIN, bool HYB, bool MUT>
-__device__ __forceinline__ int fk_pair_box64(double dx, double dy, double dz,
-    const FkBins &bn, const uint16_t *stab, const uint16_t *ptab,
-    const uint8_t *mutab, const double *s2e, const double *pe) {
+__device__ __forceinline__ bool fk_mu_near(double t_apx, double s2, const FkBins &bn) {
+  const double fr = __builtin_amdgcn_fract(t_apx);
+  return !(__builtin_fabs(fr - 0.5) <= bn.mu_w) || s2 < bn.s2_exact;
+}
+
+/* Box pair, B already at its periodic image (b' = fl(b + D)). */
+template <int BIN, bool HYB, bool MK = false, bool PLAIN = false>
+__device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
+    const FkBins &bn, const unsigned *stab, const unsigned *ptab,
+    const double *s2e, const double *pe) {
   if (BIN == FCFC_BIN_SPI) {
-    const double pi = fabs(dz);
+    const double pi = __builtin_fabs(dz);
     const double s2 = __builtin_fma(dy, dy, dx * dx);          /* :416-420 */
-    const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
-    const unsigned pb = fk_pbin<HYB>(pi, bn, ptab, pe);
-    return (sb == FK_OUT || pb == FK_OUT) ? -1 : (int) (sb + bn.ns * pb);
+    const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
+    const unsigned pb = fk_pbin<HYB, MK>(pi, bn, ptab, pe);
+    return sb + __umul24((unsigned) bn.ns, pb);
   }
   const double dz2 = dz * dz;
   const double s2 = __builtin_fma(dy, dy, __builtin_fma(dx, dx, dz2));  /* :426-430 */
-  const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
-  if (BIN == FCFC_BIN_ISO) return sb == FK_OUT ? -1 : (int) sb;
-  const int mb = fk_mubin<MUT>(dz2, s2, bn, mutab);
-  return (sb == FK_OUT || mb < 0) ? -1 : (int) sb + bn.ns * mb;
+  const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
+  if (BIN == FCFC_BIN_ISO) return sb;
+  if (PLAIN) return sb + __umul24((unsigned) bn.ns, fk_mu_plain(dz2, s2, bn));
+  const double t = (__builtin_fabs(dz) * bn.nmu_d) * __builtin_amdgcn_rsq(s2);
+  unsigned mb = (unsigned) (int) t;
+  if (fk_mu_near(t, s2, bn)) mb = fk_mu_exact(dz2 * bn.nmu2, s2, t, bn);
+  return sb + __umul24((unsigned) bn.ns, mb);
 }
 
-/* Survey pair (src/fcfc/2pt/metric_common.c:283-460, FMA vector formulas). */
-template <int BIN, bool HYB, bool MUT>
-__device__ __forceinline__ int fk_pair_sv64(double ax, double ay, double az,
+/* Survey pair (src/fcfc/2pt/metric_common.c:283-460, FMA vector formulas):
+ * t = 2 a.b, s = |a|^2 + |b|^2, d = |a|^2 - |b|^2, pi^2 = d^2 / (t + s),
+ * s^2 = s - t, s_perp^2 = (s - t) - pi^2.  (s, mu) bins need pi^2 only in
+ * the exact branch of the mu bin: sqrt(pi^2) = |d| / sqrt(t + s). */
+template <int BIN, bool HYB, bool PLAIN = false>
+__device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
     double as, double bx, double by, double bz, double bs, const FkBins &bn,
-    const uint16_t *stab, const uint16_t *ptab, const uint8_t *mutab,
-    const double *s2e, const double *pe) {
+    const unsigned *stab, const unsigned *ptab, const double *s2e, const double *pe) {
   if (BIN == FCFC_BIN_ISO) {
     const double dx = ax - bx, dy = ay - by, dz = az - bz;
     const double s2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-    const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
-    return sb == FK_OUT ? -1 : (int) sb;
+    return fk_sbin<HYB>(s2, bn, stab, s2e);
   }
   const double t = __builtin_fma(az, bz, __builtin_fma(ay, by, ax * bx)) * 2.0;
   const double s = as + bs;
   const double d = as - bs;
-  const double p2 = (d * d) / (t + s);
   if (BIN == FCFC_BIN_SMU) {
     const double s2 = s - t;
     const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
-    const int mb = fk_mubin<MUT>(p2, s2, bn, mutab);
-    return (sb == FK_OUT || mb < 0) ? -1 : (int) sb + bn.ns * mb;
+    if (PLAIN) return sb + __umul24((unsigned) bn.ns, fk_mu_plain((d * d) / (t + s), s2, bn));
+    const double tm = (__builtin_fabs(d) * bn.nmu_d) *
+        (__builtin_amdgcn_rsq(t + s) * __builtin_amdgcn_rsq(s2));
+    unsigned mb = (unsigned) (int) tm;
+    if (fk_mu_near(tm, s2, bn)) mb = fk_mu_exact(((d * d) / (t + s)) * bn.nmu2, s2, tm, bn);
+    return sb + __umul24((unsigned) bn.ns, mb);
   }
   /* (s_perp, pi): pbin holds pi^2 edges; with s2min == 0 the reference has
    * no lower bound on s_perp^2 (a rounding-negative value lands in bin 0) */
+  const double p2 = (d * d) / (t + s);
   const double sp2 = (s - t) - p2;
   const unsigned sb = fk_sbin<HYB>(sp2, bn, stab, s2e);
   const unsigned pb = fk_pbin<HYB>(p2, bn, ptab, pe);
-  return (sb == FK_OUT || pb == FK_OUT) ? -1 : (int) (sb + bn.ns * pb);
+  return sb + __umul24((unsigned) bn.ns, pb);
 }
 
 /* The other arithmetic paths of the reference, for the rounding census
  * (box only): vectorised a-side ((a+S)-b, var 0), per-pair periodic wrap
  * (var 2) and the unfused scalar remainder path (var 3). */
 template <int BIN, bool HYB>
-__device__ int fk_pair_box_variant(int var, const double a[3], const double bo[3],
-    const int dcode[3], const FkGrid &g, const FkBins &bn, const uint16_t *stab,
-    const uint16_t *ptab, const double *s2e, const double *pe) {
+__device__ unsigned fk_box_variant(int var, const double a[3], const double bo[3],
+    const int dcode[3], const FkGrid &g, const FkBins &bn, const unsigned *stab,
+    const unsigned *ptab, const double *s2e, const double *pe) {
   const double L[3] = {g.Lx, g.Ly, g.Lz};
   double d[3];
   for (int k = 0; k < 3; k++) {
@@ -404,166 +447,181 @@ __device__ int fk_pair_box_variant(int var, const double a[3], const double bo[3
   if (BIN == FCFC_BIN_SPI) {
     const double pi = fabs(d[2]);
     const double s2 = scalar ? (d[0] * d[0] + d[1] * d[1]) : __builtin_fma(d[1], d[1], d[0] * d[0]);
-    const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
-    const unsigned pb = fk_pbin<HYB>(pi, bn, ptab, pe);
-    return (sb == FK_OUT || pb == FK_OUT) ? -1 : (int) (sb + bn.ns * pb);
+    const unsigned sb = fk_sbin<HYB, true>(s2, bn, stab, s2e);
+    const unsigned pb = fk_pbin<HYB, true>(pi, bn, ptab, pe);
+    return sb + __umul24((unsigned) bn.ns, pb);
   }
   const double dz2 = d[2] * d[2];
   double s2;
   if (scalar) { double t = d[0] * d[0]; t = t + d[1] * d[1]; s2 = t + dz2; }
   else s2 = __builtin_fma(d[1], d[1], __builtin_fma(d[0], d[0], dz2));
-  const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
-  if (sb == FK_OUT) return -1;
-  if (BIN == FCFC_BIN_ISO) return (int) sb;
+  const unsigned sb = fk_sbin<HYB, true>(s2, bn, stab, s2e);
+  if (BIN == FCFC_BIN_ISO || sb == FK_OUT) return sb;
   int m = 0;
-  if (s2 >= 2.220446049250313e-16) {
+  if (s2 >= FK_DBL_EPS) {
     if (scalar) {
       const double q = (dz2 / s2) * bn.nmu2;           /* :184 */
-      if (q >= bn.nmu2) return -1;
+      if (q >= bn.nmu2) return FK_NONE;
       m = (int) q;
     }
     else {
       const double mq = fk_floor_div(dz2 * bn.nmu2, s2);
-      if (mq >= bn.nmu2) return -1;
+      if (mq >= bn.nmu2) return FK_NONE;
       m = (int) mq;
     }
   }
-  return (int) sb + bn.ns * fk_isqrt(m);
+  return sb + __umul24((unsigned) bn.ns, (unsigned) fk_isqrt(m));
 }
 
 /* ---------------- FP32 pair evaluation with exactness windows ----------- */
 /* Coordinates relative to the A tile centre in FP32.  Returns the histogram
- * index, -1 (certainly outside) or -2 (within the rounding bound of a bin
- * edge: the caller re-evaluates the pair in FP64). */
+ * index (>= ntot: not counted) or FK_AMB when the pair lies within the
+ * rounding bound of a bin edge that matters (it is then re-evaluated in
+ * FP64).  The s^2 cell flags say which ends of the integer cell are edges,
+ * so the window test is branch-free; a pair whose s (or pi) is certainly
+ * out of range is never AMB. */
 template <int BIN>
-__device__ __forceinline__ int fk_pair_box32(float dx, float dy, float dz,
-    const FkBins &bn, const FkWin &wn, const uint16_t *stab,
-    const uint16_t *ptab) {
+__device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
+    const FkBins &bn, const FkWin &wn, const unsigned *stab,
+    const unsigned *ptab) {
   float s2;
   if (BIN == FCFC_BIN_SPI) s2 = __builtin_fmaf(dy, dy, dx * dx);
   else s2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, dz * dz));
-  /* s bin: integer part indexes the table, the fraction decides whether an
-   * edge is within the window */
   const float fl = __builtin_floorf(s2);
-  const int kraw = (int) fl - bn.soff + 1;
-  const unsigned k = min((unsigned) kraw, (unsigned) (bn.stab_len + 1));
-  const unsigned sb = stab[k];
+  const unsigned e = stab[min((unsigned) ((int) fl - bn.soff_m1), (unsigned) (bn.stab_len + 2))];
   const float f = s2 - fl;
-  /* beyond the table (kraw outside [0, len+1]) the pair is out whatever the
-   * rounding: the window is far below one unit of s^2 */
-  bool near = (kraw >= 0 && kraw <= bn.stab_len + 1) && fabsf(f - 0.5f) > 0.5f - wn.e_s2;
-  if (near) {
-    const unsigned kn = (f < 0.5f) ? (k > 0 ? k - 1 : 0) : min(k + 1, (unsigned) (bn.stab_len + 1));
-    near = (stab[kn] != sb);
-  }
-  int res;
-  bool amb = near || (sb >= (unsigned) bn.ns && sb != FK_OUT);   /* hybrid entry */
-  bool out = (sb == FK_OUT);
-  if (BIN == FCFC_BIN_ISO) {
-    res = (int) sb;
-  }
-  else if (BIN == FCFC_BIN_SPI) {
-    const float pi = fabsf(dz);
+  const unsigned msk = FK_F_ALW | (f < wn.e_s2 ? FK_F_LO : 0u) | (f > 1.0f - wn.e_s2 ? FK_F_HI : 0u);
+  const bool amb = (e & msk) != 0;
+  const unsigned sb = e & 0xFFFFu;
+  if (BIN == FCFC_BIN_ISO) return amb ? FK_AMB : sb;
+  if (BIN == FCFC_BIN_SPI) {
+    const float pi = __builtin_fabsf(dz);
     const float fp = __builtin_floorf(pi);
-    const int kpr = (int) fp - bn.poff + 1;
-    const unsigned kp = min((unsigned) kpr, (unsigned) (bn.ptab_len + 1));
-    const unsigned pb = ptab[kp];
+    const unsigned ep = ptab[min((unsigned) ((int) fp - bn.poff_m1), (unsigned) (bn.ptab_len + 2))];
     const float g = pi - fp;
-    bool nearp = (kpr >= 0 && kpr <= bn.ptab_len + 1) && fabsf(g - 0.5f) > 0.5f - wn.e_pi;
-    if (nearp) {
-      const unsigned kn = (g < 0.5f) ? (kp > 0 ? kp - 1 : 0) : min(kp + 1, (unsigned) (bn.ptab_len + 1));
-      nearp = (ptab[kn] != pb);
-    }
-    amb = amb || nearp || (pb >= (unsigned) bn.np && pb != FK_OUT);
-    out = out || (pb == FK_OUT);
-    res = (int) (sb + bn.ns * pb);
+    const unsigned mp = FK_F_ALW | (g < wn.e_pi ? FK_F_LO : 0u) | (g > 1.0f - wn.e_pi ? FK_F_HI : 0u);
+    const bool ambp = (ep & mp) != 0;
+    const unsigned pb = ep & 0xFFFFu;
+    const bool out = (sb == FK_OUT && !amb) || (pb == FK_OUT && !ambp);
+    return ((amb || ambp) && !out) ? FK_AMB : sb + __umul24((unsigned) bn.ns, pb);
   }
-  else {
-    /* mu bin = floor(nmu |dz| / s) = isqrt(floor(nmu^2 dz^2 / s^2)) */
-    const float inv_s = __builtin_amdgcn_rsqf(s2);
-    const float t = wn.nmu_f * fabsf(dz) * inv_s;
-    const float ft = __builtin_floorf(t);
-    const float dt = __builtin_fmaf(t, wn.e_rrel, inv_s * __builtin_fmaf(t, wn.e_rabs, wn.e_rz));
-    const bool nearm = !(fabsf(t - ft - 0.5f) <= 0.5f - dt) || s2 < wn.s2_tiny;
-    amb = amb || nearm;
-    out = out || (ft >= wn.nmu_f);
-    res = (int) sb + bn.ns * (int) ft;
-  }
-  return amb ? -2 : (out ? -1 : res);
+  /* mu bin = floor(t), t = nmu |dz| / s; above s2_floor the FP32 error of t
+   * is below 1/2 - mu_w (fp32_windows) and t <= nmu (1 + 2^-20) */
+  const float t = (wn.nmu_f * __builtin_fabsf(dz)) * __builtin_amdgcn_rsqf(s2);
+  const bool nearm = !(__builtin_fabsf(__builtin_amdgcn_fractf(t) - 0.5f) <= wn.mu_w) ||
+                     s2 < wn.s2_floor;
+  const bool ambt = amb || (nearm && sb != FK_OUT);
+  return ambt ? FK_AMB : sb + __umul24((unsigned) bn.ns, (unsigned) (int) t);
 }
 
-/* Exact FP64 re-evaluation of a pair flagged by the FP32 windows (rare, kept
- * out of line so that the FP32 loop stays small).  With census enabled the
- * other rounding paths of the reference are evaluated as well and pairs
- * whose bin depends on the path are counted. */
-template <int BIN, bool HYB, bool MUT>
-__device__ __noinline__ int fk_recheck(const FkBins &bn, const FkJob &J, int census,
-    double ax, double ay, double az, int jg, int code, const uint16_t *stab,
-    const uint16_t *ptab, const uint8_t *mutab, const double *s2e, const double *pe,
-    unsigned int *ncensus) {
+/* Exact FP64 re-evaluation of a pair flagged by the FP32 windows (rare: the
+ * pairs are queued and re-evaluated together after each chunk).  The tables
+ * are the flagged FP32 ones, read through their bin bits.  With census
+ * enabled the other rounding paths of the reference are evaluated as well
+ * and pairs whose bin depends on the path are counted. */
+template <int BIN, bool HYB>
+__device__ __forceinline__ bool fk_census(const FkBins &bn, const FkJob &J, double ax,
+    double ay, double az, const double bo[3], const int c3[3], unsigned idx,
+    const unsigned *stab, const unsigned *ptab, const double *s2e, const double *pe) {
+  const unsigned nt = (unsigned) bn.ntot;
+  const double av3[3] = {ax, ay, az};
+  bool differ = false;
+  for (int v = 0; v < 4; v++) {
+    if (v == 1) continue;
+    unsigned iv = fk_box_variant<BIN, HYB>(v, av3, bo, c3, J.g, bn, stab, ptab, s2e, pe);
+    if (iv >= nt) iv = FK_NONE;
+    differ = differ || (iv != idx);
+  }
+  return differ;
+}
+
+template <int BIN, bool HYB, bool CEN>
+__device__ __forceinline__ unsigned fk_recheck(const FkBins &bn, const FkJob &J,
+    double ax, double ay, double az, int jg, int code, const unsigned *stab,
+    const unsigned *ptab, const double *s2e, const double *pe, unsigned int *ncensus) {
   const int c3[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
   const double bo[3] = {J.B.x[jg], J.B.y[jg], J.B.z[jg]};
   const double ex = bo[0] + c3[0] * J.g.Lx;
   const double ey = bo[1] + c3[1] * J.g.Ly;
   const double ez = bo[2] + c3[2] * J.g.Lz;
-  const int idx = fk_pair_box64<BIN, HYB, MUT>(ex - ax, ey - ay, ez - az, bn, stab, ptab, mutab, s2e, pe);
-  if (census) {
-    const double av3[3] = {ax, ay, az};
-    const int iv0 = fk_pair_box_variant<BIN, HYB>(0, av3, bo, c3, J.g, bn, stab, ptab, s2e, pe);
-    const int iv2 = fk_pair_box_variant<BIN, HYB>(2, av3, bo, c3, J.g, bn, stab, ptab, s2e, pe);
-    const int iv3 = fk_pair_box_variant<BIN, HYB>(3, av3, bo, c3, J.g, bn, stab, ptab, s2e, pe);
-    if (iv0 != idx || iv2 != idx || iv3 != idx) (*ncensus)++;
-  }
+  unsigned idx = fk_box64<BIN, HYB, true>(ex - ax, ey - ay, ez - az, bn, stab, ptab, s2e, pe);
+  if (idx >= (unsigned) bn.ntot) idx = FK_NONE;
+  if (CEN && fk_census<BIN, HYB>(bn, J, ax, ay, az, bo, c3, idx, stab, ptab, s2e, pe))
+    (*ncensus)++;
   return idx;
 }
 
 /* ---------------------------- the kernel --------------------------------- */
 
-template <int BIN, int GEOM, bool WT, int MODE, bool HYB, bool MUT, int R>
+/* One persistent kernel for counts (FP64 or FP32 + recheck) and for the
+ * weight gradient.  Template parameters: binning, geometry, weighted
+ * histogram support, mode, hybrid tables, A objects per lane. */
+template <int BIN, int GEOM, bool WT, int MODE, bool HYB, int R>
 __global__ void __launch_bounds__(FK_NT)
 k_count(const FkArgs a) {
   extern __shared__ __attribute__((aligned(16))) char lds[];
-  constexpr bool F32 = (MODE == MODE_C32);
+  constexpr bool F32 = (MODE == MODE_C32 || MODE == MODE_C32C);
+  constexpr bool PLAIN = (MODE == MODE_P64);
+  constexpr bool GRAD = (MODE == MODE_GRAD);
   constexpr bool NEED_S = (GEOM == FCFC_GEOM_SURVEY && BIN != FCFC_BIN_ISO);
-  constexpr bool NEED_W = WT || (MODE == MODE_GRAD);
+  constexpr bool NEED_W = WT || GRAD;
+  constexpr bool NEED_C = F32 || GRAD;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const FkBins &bn = a.bins;
+  const unsigned ntot = (unsigned) bn.ntot;
 
-  /* ---- LDS carve-up ---- */
+  /* ---- LDS carve-up (lds_plan on the host computes the same offsets) ---- */
   unsigned int *hist_u = (unsigned int *) (lds + a.off_hist);
   double *hist_d = (double *) (lds + a.off_hist);
-  uint16_t *stab = (uint16_t *) (lds + a.off_stab);
-  uint16_t *ptab = (uint16_t *) (lds + a.off_ptab);
-  uint8_t *mutab = (uint8_t *) (lds + a.off_mutab);
+  /* tables: raw[0] and raw[len + 1], raw[len + 2] are FK_OUT sentinels,
+   * raw[1 + c] is integer cell c; the FP64 lookups index from raw + 1, the
+   * FP32 ones (which carry edge flags) from raw */
+  unsigned *stab_raw = (unsigned *) (lds + a.off_stab);
+  unsigned *ptab_raw = (unsigned *) (lds + a.off_ptab);
+  unsigned *stab = stab_raw + 1;
+  unsigned *ptab = ptab_raw + 1;
+  unsigned *wq = (unsigned *) (lds + a.off_queue) + wave * FK_QCAP;   /* FP32 re-evaluation queue */
   double *s2e = (double *) (lds + a.off_edges);
   double *pe = s2e + (bn.ns + 1);
   FkStripLds *st = (FkStripLds *) (lds + a.off_strip);
-  char *chunk = lds + a.off_chunk;
-  double *cx = (double *) chunk, *cy = cx + FK_CH, *cz = cy + FK_CH;
-  float *fx = (float *) chunk, *fy = fx + FK_CH, *fz = fy + FK_CH;
-  char *after = F32 ? (char *) (fz + FK_CH) : (char *) (cz + FK_CH);
-  double *cs_ = (double *) after;                     /* survey |r|^2 */
-  double *cw = NEED_S ? cs_ + FK_CH : cs_;            /* weights */
-  int *cj = (int *) (NEED_W ? (char *) (cw + FK_CH) : (char *) cw);  /* sorted index */
-  int *cc = cj + FK_CH;                               /* shift code */
+  char *cp = lds + a.off_chunk;
+  double *cx = (double *) cp, *cy = cx + FK_CH, *cz = cy + FK_CH;
+  float *fx = (float *) cp, *fy = fx + FK_CH, *fz = fy + FK_CH;
+  cp += (F32 ? 12 : 24) * FK_CH;
+  double *cs_ = (double *) cp;                 /* survey |r|^2 */
+  if (NEED_S) cp += 8 * FK_CH;
+  double *cw = (double *) cp;                  /* weights */
+  if (NEED_W) cp += 8 * FK_CH;
+  int *cj = (int *) cp;                        /* sorted index */
+  cp += 4 * FK_CH;
+  int *cc = (int *) cp;                        /* periodic image code */
   __shared__ int s_tile;
 
   /* ---- stage tables ---- */
   const int hist_elems = a.nh * bn.ntot;
-  if (MODE == MODE_GRAD) {
+  if (GRAD) {
     for (int i = tid; i < bn.ntot; i += FK_NT) hist_d[i] = a.G[i];
   }
   else if (WT) { for (int i = tid; i < hist_elems; i += FK_NT) hist_d[i] = 0.0; }   /* covers the u32 view */
   else { for (int i = tid; i < hist_elems; i += FK_NT) hist_u[i] = 0u; }
-  for (int i = tid; i < bn.stab_len + 2; i += FK_NT)
-    stab[i] = (i == 0 || i == bn.stab_len + 1) ? (uint16_t) FK_OUT : bn.stab[i - 1];
-  if (BIN == FCFC_BIN_SPI)
-    for (int i = tid; i < bn.ptab_len + 2; i += FK_NT)
-      ptab[i] = (i == 0 || i == bn.ptab_len + 1) ? (uint16_t) FK_OUT : bn.ptab[i - 1];
-  if (BIN == FCFC_BIN_SMU && MUT) {
-    const int nm2 = bn.nmu * bn.nmu;
-    for (int i = tid; i < nm2; i += FK_NT) mutab[i] = (uint8_t) fk_isqrt(i);
+  {
+    /* cell c of a table of len cells; FP32 entries carry the edge flags */
+    auto cell = [](const uint16_t *t, int len, int c) -> unsigned {
+      return (c >= 0 && c < len) ? (unsigned) t[c] : FK_OUT;
+    };
+    auto entry = [&](const uint16_t *t, int len, int nb, int c) -> unsigned {
+      const unsigned v = cell(t, len, c);
+      if (!F32) return v;
+      unsigned f = 0;
+      if (cell(t, len, c - 1) != v) f |= FK_F_LO;
+      if (cell(t, len, c + 1) != v) f |= FK_F_HI;
+      if (v >= (unsigned) nb && v != FK_OUT) f |= FK_F_ALW;
+      return v | f;
+    };
+    for (int i = tid; i <= bn.stab_len + 2; i += FK_NT) stab_raw[i] = entry(bn.stab, bn.stab_len, bn.ns, i - 1);
+    if (BIN == FCFC_BIN_SPI)
+      for (int i = tid; i <= bn.ptab_len + 2; i += FK_NT) ptab_raw[i] = entry(bn.ptab, bn.ptab_len, bn.np, i - 1);
   }
   if (HYB) {
     for (int i = tid; i <= bn.ns; i += FK_NT) s2e[i] = bn.s2bin[i];
@@ -610,7 +668,7 @@ k_count(const FkArgs a) {
     if (t.job != cur) {
       /* switch job: flush the histogram of the previous one, stage the new
        * job descriptor in LDS */
-      if (MODE != MODE_GRAD) flush();
+      if (!GRAD) flush();
       __syncthreads();
       {
         const int *src = (const int *) (a.jobs + t.job);
@@ -624,95 +682,127 @@ k_count(const FkArgs a) {
     const int ix = t.row / J.g.ncy, iy = t.row - ix * J.g.ncy;
     /* weights exist for this job (a weighted-capable kernel may run
      * unweighted jobs of a batch, whose catalogues carry no weights) */
-    const bool jw = (MODE == MODE_GRAD) || (WT && J.wt);
+    const bool jw = GRAD || (WT && J.wt);
 
-    /* A objects of this lane */
+    /* A objects of this lane; idle lanes sit far away, so that every pair
+     * they form is out of range */
     double ax[R], ay[R], az[R], as[R], aw[R], gacc[R];
     float fax[R], fay[R], faz[R];
     int ai[R];
-    bool av[R];
-    /* tile centre for the FP32 relative coordinates */
     const double tcx = J.g.ox + (ix + 0.5) * J.g.wx;
     const double tcy = J.g.oy + (iy + 0.5) * J.g.wy;
     const double tcz = J.g.oz + 0.5 * (t.iz0 + t.iz1 + 1) * J.g.wz;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int p = t.p0 + tid + r * FK_NT;
-      av[r] = p < t.p1;
-      const int q = av[r] ? p : t.p0;
-      ai[r] = av[r] ? p : 0x7fffffff;
-      ax[r] = J.A.x[q]; ay[r] = J.A.y[q]; az[r] = J.A.z[q];
-      as[r] = NEED_S ? J.A.s[q] : 0.0;
-      aw[r] = jw ? J.A.w[q] : 1.0;
+      const bool v = p < t.p1;
+      ai[r] = v ? p : 0x7fffffff;
+      ax[r] = v ? J.A.x[p] : FK_FAR;
+      ay[r] = v ? J.A.y[p] : FK_FAR;
+      az[r] = v ? J.A.z[p] : FK_FAR;
+      as[r] = NEED_S ? (v ? J.A.s[p] : 3.0 * FK_FAR * FK_FAR) : 0.0;
+      aw[r] = (jw && v) ? J.A.w[p] : 1.0;
       gacc[r] = 0.0;
-      fax[r] = (float) (ax[r] - tcx); fay[r] = (float) (ay[r] - tcy); faz[r] = (float) (az[r] - tcz);
+      fax[r] = v ? (float) (ax[r] - tcx) : 1.0e18f;
+      fay[r] = v ? (float) (ay[r] - tcy) : 1.0e18f;
+      faz[r] = v ? (float) (az[r] - tcz) : 1.0e18f;
     }
     const int nva = min(t.p1 - t.p0, R * FK_NT);
+    const bool jwt = WT && J.wt;
+    const bool grad_self = GRAD && J.autoc;
 
-    /* one pair: lane's A object r against staged B object j */
-    auto pair = [&](int r, int j, bool self) -> int {
-      int idx;
-      if (!F32) {
-        if (GEOM == FCFC_GEOM_BOX) {
-          if (MODE == MODE_GRAD && (cc[j] & 32)) {
-            const int cd = cc[j] & 31;
-            const double ux = ax[r] - (double) (cd % 3 - 1) * J.g.Lx;
-            const double uy = ay[r] - (double) ((cd / 3) % 3 - 1) * J.g.Ly;
-            const double uz = az[r] - (double) (cd / 9 - 1) * J.g.Lz;
-            idx = fk_pair_box64<BIN, HYB, MUT>(cx[j] - ux, cy[j] - uy, cz[j] - uz, bn, stab, ptab, mutab, s2e, pe);
+    /* FP32: FP64 re-evaluation of the n queued pairs of this wavefront
+     * (entries: A slot << 8 | chunk index), while the chunk is staged */
+    const int lane = tid & 63;
+    const unsigned long long lanemask_lt = (1ull << lane) - 1;
+    auto drain = [&](int n) {
+      for (int e0 = 0; e0 < n; e0 += 64) {
+        if (e0 + lane < n) {
+          const unsigned v = wq[e0 + lane];
+          const int p = t.p0 + (int) (v >> 8);
+          const int j = (int) (v & 255u);
+          const double qx = J.A.x[p], qy = J.A.y[p], qz = J.A.z[p];
+          const unsigned idx = fk_recheck<BIN, HYB, MODE == MODE_C32C>(bn, J, qx, qy, qz, cj[j], cc[j],
+                                                                       stab, ptab, s2e, pe, &ncensus);
+          nrecheck++;
+          if (idx < ntot) {
+            if (WT && jwt) atomicAdd(myhist_d + idx, J.A.w[p] * cw[j]);
+            else atomicAdd(myhist_u + idx, 1u);
+          }
+        }
+      }
+    };
+
+    /* one staged chunk of cnt B objects against the lane's A objects */
+    auto run_chunk = [&](auto selfc, int cnt) {
+      constexpr bool SELF = decltype(selfc)::value;
+      int qn = 0;                 /* queued pairs of this wavefront */
+#pragma unroll 2
+      for (int j = 0; j < cnt; j++) {
+        double bx = 0, by = 0, bz = 0, bs = 0;
+        float gx = 0, gy = 0, gz = 0;
+        if (F32) { gx = fx[j]; gy = fy[j]; gz = fz[j]; }
+        else { bx = cx[j]; by = cy[j]; bz = cz[j]; }
+        if (NEED_S) bs = cs_[j];
+        const double wj = NEED_W ? cw[j] : 1.0;
+        const int jj = (SELF || grad_self || F32) ? cj[j] : 0;
+        const int code = NEED_C ? cc[j] : 13;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          unsigned idx;
+          if (F32) {
+            idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw, ptab_raw);
+            if (SELF && !(jj > ai[r])) idx = FK_NONE;
+            /* pairs near an edge: queue (A slot, chunk entry) for FP64 */
+            const bool amb = (idx == FK_AMB);
+            const unsigned long long m = __ballot(amb);
+            if (m) {
+              if (amb) {
+                const int pos = qn + __popcll(m & lanemask_lt);
+                wq[pos] = ((unsigned) (tid + r * FK_NT) << 8) | (unsigned) j;
+                idx = FK_NONE;
+              }
+              qn += __popcll(m);
+              if (qn > FK_QCAP - 64) { drain(qn); qn = 0; }
+            }
+          }
+          else if (GEOM == FCFC_GEOM_BOX) {
+            if (GRAD && (code & 32)) {
+              /* pair that the count evaluates from the other side:
+               * dx = b - (a - D) with D the image of this B segment */
+              const int cd = code & 31;
+              const double ux = ax[r] - (double) (cd % 3 - 1) * J.g.Lx;
+              const double uy = ay[r] - (double) ((cd / 3) % 3 - 1) * J.g.Ly;
+              const double uz = az[r] - (double) (cd / 9 - 1) * J.g.Lz;
+              idx = fk_box64<BIN, HYB>(bx - ux, by - uy, bz - uz, bn, stab, ptab, s2e, pe);
+            }
+            else
+              idx = fk_box64<BIN, HYB, false, PLAIN>(bx - ax[r], by - ay[r], bz - az[r], bn, stab,
+                                                      ptab, s2e, pe);
           }
           else
-            idx = fk_pair_box64<BIN, HYB, MUT>(cx[j] - ax[r], cy[j] - ay[r], cz[j] - az[r], bn, stab, ptab, mutab, s2e, pe);
-        }
-        else
-          idx = fk_pair_sv64<BIN, HYB, MUT>(ax[r], ay[r], az[r], as[r], cx[j], cy[j], cz[j],
-              NEED_S ? cs_[j] : 0.0, bn, stab, ptab, mutab, s2e, pe);
-      }
-      else {
-        idx = fk_pair_box32<BIN>(fx[j] - fax[r], fy[j] - fay[r], fz[j] - faz[r], bn, J.win, stab, ptab);
-        if (idx == -2) {
-          nrecheck++;
-          idx = fk_recheck<BIN, HYB, MUT>(bn, J, a.census, ax[r], ay[r], az[r], cj[j], cc[j],
-              stab, ptab, mutab, s2e, pe, &ncensus);
+            idx = fk_sv64<BIN, HYB, PLAIN>(ax[r], ay[r], az[r], as[r], bx, by, bz, bs, bn,
+                                           stab, ptab, s2e, pe);
+          if (!F32 && SELF && !(jj > ai[r])) idx = FK_NONE;
+          if (GRAD && grad_self && jj == ai[r]) idx = FK_NONE;
+          if (idx < ntot) {
+            if (GRAD) gacc[r] += hist_d[idx] * wj;
+            else if (WT && jwt) atomicAdd(myhist_d + idx, aw[r] * wj);
+            else atomicAdd(myhist_u + idx, 1u);
+          }
         }
       }
-      if (!av[r]) idx = -1;
-      if (self && !(cj[j] > ai[r])) idx = -1;
-      if (MODE == MODE_GRAD && J.autoc && cj[j] == ai[r]) idx = -1;
-      return idx;
-    };
-
-    auto accum = [&](int r, int j, int idx) {
-      if (idx < 0) return;
-      if (MODE == MODE_GRAD) gacc[r] += hist_d[idx] * cw[j];
-      else if (WT && J.wt) atomicAdd(myhist_d + idx, aw[r] * cw[j]);
-      else atomicAdd(myhist_u + idx, 1u);
-    };
-
-    /* process one staged chunk of `cnt` B objects, two at a time */
-    auto run_chunk = [&](int cnt, bool self) {
-      int j = 0;
-      for (; j + 1 < cnt; j += 2) {
-        int id[2][R];
-#pragma unroll
-        for (int r = 0; r < R; r++) { id[0][r] = pair(r, j, self); id[1][r] = pair(r, j + 1, self); }
-#pragma unroll
-        for (int r = 0; r < R; r++) { accum(r, j, id[0][r]); accum(r, j + 1, id[1][r]); }
-      }
-      if (j < cnt) {
-#pragma unroll
-        for (int r = 0; r < R; r++) accum(r, j, pair(r, j, self));
-      }
+      if (F32 && qn) drain(qn);
     };
 
     auto guard = [&](int cnt) {
-      if (MODE == MODE_GRAD || (WT && J.wt)) return;
+      if (GRAD || jwt) return;
       since_flush += (double) cnt * nva;
       if (since_flush > 2.0e9) { flush(); since_flush = 0; }
     };
 
     /* ---- auto counts: pairs inside the tile (j > i) ---- */
-    if (MODE != MODE_GRAD && J.autoc) {
+    if (!GRAD && J.autoc) {
       for (int c0 = t.p0; c0 < t.p1; c0 += FK_CH) {
         const int cnt = min(FK_CH, t.p1 - c0);
         __syncthreads();
@@ -726,7 +816,7 @@ k_count(const FkArgs a) {
           cj[q] = jg;
         }
         __syncthreads();
-        run_chunk(cnt, true);
+        run_chunk(std::integral_constant<bool, true>(), cnt);
         evals += (double) cnt * nva * 0.5;
         guard(cnt);
       }
@@ -771,7 +861,7 @@ k_count(const FkArgs a) {
           double x = J.B.x[jg], y = J.B.y[jg], z = J.B.z[jg];
           /* gradient: pairs that the count evaluates from the other object's
            * tile keep that orientation, (a - D) - b (see fk_strip_segments) */
-          const bool swp = (MODE == MODE_GRAD) && (J.swap_all || (code & 32));
+          const bool swp = GRAD && (J.swap_all || (code & 32));
           if (GEOM == FCFC_GEOM_BOX && (code & 31) != 13 && !swp) {
             const int cd = code & 31;
             x = x + (double) (cd % 3 - 1) * J.g.Lx;
@@ -780,26 +870,26 @@ k_count(const FkArgs a) {
           }
           if (F32) { fx[q] = (float) (x - tcx); fy[q] = (float) (y - tcy); fz[q] = (float) (z - tcz); cc[q] = code; }
           else { cx[q] = x; cy[q] = y; cz[q] = z; }
-          if (MODE == MODE_GRAD) cc[q] = (code & 31) | (swp ? 32 : 0);
+          if (GRAD) cc[q] = (code & 31) | (swp ? 32 : 0);
           if (NEED_S) cs_[q] = J.B.s[jg];
           if (NEED_W) cw[q] = jw ? J.B.w[jg] : 1.0;
           cj[q] = jg;
         }
         __syncthreads();
-        run_chunk(cnt, false);
+        run_chunk(std::integral_constant<bool, false>(), cnt);
         evals += (double) cnt * nva;
         guard(cnt);
       }
     }
 
-    if (MODE == MODE_GRAD) {
+    if (GRAD) {
 #pragma unroll
       for (int r = 0; r < R; r++)
-        if (av[r]) J.gA[J.A.idx[ai[r]]] += gacc[r];
+        if (ai[r] != 0x7fffffff) J.gA[J.A.idx[ai[r]]] += gacc[r];
     }
   }
 
-  if (MODE != MODE_GRAD) flush();
+  if (!GRAD) flush();
   if (tid == 0) atomicAdd(a.stats + 0, (unsigned long long) evals);
   if (F32) {
     if (nrecheck) atomicAdd(a.stats + 1, (unsigned long long) nrecheck);
@@ -831,6 +921,7 @@ struct fcfc_gpu_ctx {
   char err[512] = {0};
   fcfc_gpu_stats stats{};
   int t_cell_div = 0, t_tile = 0, t_zdiv = 0, t_nblock = 0;
+  int t_nh = 0;                /* LDS histogram copies (0: by occupancy) */
   int census = 0;
   int tile_cap = 0;            /* autotuner: count only the first tiles */
   int last_ntiles = 0;
@@ -938,6 +1029,7 @@ struct JobPlan {
   int T;            /* objects per tile */
   int zmaxc;        /* max z-cells per tile */
   int R;            /* objects per lane */
+  int T_auto;       /* T chosen by the plan (not by the caller's tuning) */
   std::vector<int> strips;
   int nstrips;
 };
@@ -1011,9 +1103,6 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
     while (cd < 8 && objs_per_col / (cd * cd) > 256.0) cd++;
   }
   if (zd <= 0) zd = 4;
-  P->R = (mode == MODE_GRAD) ? 1 : 2;
-  P->T = tT > 0 ? tT : FK_NT * P->R;
-  if (P->T > FK_NT * P->R) P->T = FK_NT * P->R;
   double w = Rxy / cd;
   if (b->geometry == FCFC_GEOM_BOX) {
     g.periodic = 1;
@@ -1059,6 +1148,16 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
       return FCFC_ERR_CFG;
     }
   }
+  /* objects per tile: two per lane when the rows are long enough to fill
+   * such tiles, one per lane otherwise (and always for the gradient) */
+  P->T_auto = (mode != MODE_GRAD && tT <= 0);
+  if (mode == MODE_GRAD) P->T = FK_NT;
+  else if (tT > 0) P->T = std::min(tT, 2 * FK_NT);
+  else {
+    const double per_row = (double) nA / ((double) g.ncx * g.ncy);
+    P->T = (per_row >= 3.0 * FK_NT) ? 2 * FK_NT : FK_NT;
+  }
+  P->R = (P->T + FK_NT - 1) / FK_NT;
   /* strips */
   P->strips.clear();
   const int kx = (int) ceil(Rxy / g.wx), ky = (int) ceil(Rxy / g.wy);
@@ -1122,6 +1221,11 @@ struct Alloc {
     return p;
   }
 };
+
+__global__ void k_gather(int n, const int *__restrict__ idx, const double *__restrict__ src,
+    double *__restrict__ dst) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[idx[i]];
+}
 
 __global__ void k_fill_f64(double *p, int64_t n, double v) {
   for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) p[i] = v;
@@ -1241,7 +1345,7 @@ static int stage_cat(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, int need_s,
   return 0;
 }
 
-struct LdsPlan { int off_hist, off_stab, off_ptab, off_mutab, off_edges, off_strip, off_chunk, total, nh; };
+struct LdsPlan { int off_hist, off_stab, off_ptab, off_edges, off_strip, off_chunk, off_queue, total, nh; };
 
 static int align16(int v) { return (v + 15) & ~15; }
 
@@ -1253,55 +1357,66 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   if (mode == MODE_GRAD) o += b->ntot * 8;
   else o += nh * b->ntot * (wt ? 8 : 4);
   o = align16(o);
-  L.off_stab = o; o = align16(o + (b->stab_len + 2) * 2);
-  L.off_ptab = o; if (b->bintype == FCFC_BIN_SPI) o = align16(o + (b->ptab_len + 2) * 2);
-  L.off_mutab = o;
-  if (b->bintype == FCFC_BIN_SMU && b->nmu * b->nmu <= 16384) o = align16(o + b->nmu * b->nmu);
+  L.off_stab = o; o = align16(o + (b->stab_len + 3) * 4);
+  L.off_ptab = o; if (b->bintype == FCFC_BIN_SPI) o = align16(o + (b->ptab_len + 3) * 4);
   L.off_edges = o;
   if (b->tabtype) o = align16(o + 8 * (b->ns + 1 + (b->bintype == FCFC_BIN_SPI ? b->np + 1 : 0)));
   L.off_strip = o; o = align16(o + (int) sizeof(FkStripLds));
   L.off_chunk = o;
+  /* per staged B object: coordinates, |r|^2 (survey), weight, index, image code */
   int per = (mode == MODE_C32) ? 12 : 24;
   if (need_s) per += 8;
   if (wt || mode == MODE_GRAD) per += 8;
-  per += 8;   /* index + code */
+  per += 4;
+  if (mode == MODE_C32 || mode == MODE_GRAD) per += 4;
   o = align16(o + FK_CH * per);
+  L.off_queue = o;
+  if (mode == MODE_C32) o += (FK_NT / 64) * FK_QCAP * 4;
   L.total = o;
   return L;
 }
 
 typedef void (*kfn_t)(const FkArgs);
 
-/* Kernel variants: binning x geometry x weights x mode x table type x mu
- * table in LDS; two A objects per lane for counts, one for the gradient. */
+/* Kernel variants: binning x geometry x weights x mode x table type x A
+ * objects per lane (R = 1: tiles of FK_NT objects, R = 2: 2 FK_NT; hybrid
+ * tables and the gradient use one R each). */
 template <int BIN, int GEOM, bool WT, int MODE>
-static kfn_t pick5(int hyb, int mut) {
-  constexpr int R = (MODE == MODE_GRAD) ? 1 : 2;
-  if (BIN == FCFC_BIN_SMU) {
-    if (hyb) return mut ? k_count<BIN, GEOM, WT, MODE, true, true, R> : k_count<BIN, GEOM, WT, MODE, true, false, R>;
-    return mut ? k_count<BIN, GEOM, WT, MODE, false, true, R> : k_count<BIN, GEOM, WT, MODE, false, false, R>;
+static kfn_t pick5(int hyb, int R) {
+  if constexpr (MODE == MODE_GRAD) {
+    return hyb ? k_count<BIN, GEOM, WT, MODE, true, 1> : k_count<BIN, GEOM, WT, MODE, false, 1>;
   }
-  return hyb ? k_count<BIN, GEOM, WT, MODE, true, false, R> : k_count<BIN, GEOM, WT, MODE, false, false, R>;
+  else {
+    if (hyb) return k_count<BIN, GEOM, WT, MODE, true, 2>;
+    return R == 1 ? k_count<BIN, GEOM, WT, MODE, false, 1> : k_count<BIN, GEOM, WT, MODE, false, 2>;
+  }
 }
 
 template <int BIN, int GEOM>
-static kfn_t pick3(int wt, int mode, int hyb, int mut) {
-  if (mode == MODE_GRAD) return pick5<BIN, GEOM, true, MODE_GRAD>(hyb, mut);
-  if (GEOM == FCFC_GEOM_BOX && mode == MODE_C32)
-    return wt ? pick5<BIN, GEOM, true, (GEOM == FCFC_GEOM_BOX ? MODE_C32 : MODE_C64)>(hyb, mut)
-              : pick5<BIN, GEOM, false, (GEOM == FCFC_GEOM_BOX ? MODE_C32 : MODE_C64)>(hyb, mut);
-  return wt ? pick5<BIN, GEOM, true, MODE_C64>(hyb, mut) : pick5<BIN, GEOM, false, MODE_C64>(hyb, mut);
+static kfn_t pick3(int wt, int mode, int hyb, int R) {
+  if (mode == MODE_GRAD) return pick5<BIN, GEOM, true, MODE_GRAD>(hyb, 1);
+  if constexpr (GEOM == FCFC_GEOM_BOX) {
+    if (mode == MODE_C32)
+      return wt ? pick5<BIN, GEOM, true, MODE_C32>(hyb, R) : pick5<BIN, GEOM, false, MODE_C32>(hyb, R);
+    if (mode == MODE_C32C)
+      return wt ? pick5<BIN, GEOM, true, MODE_C32C>(hyb, R) : pick5<BIN, GEOM, false, MODE_C32C>(hyb, R);
+  }
+  if constexpr (BIN == FCFC_BIN_SMU) {
+    if (mode == MODE_P64)
+      return wt ? pick5<BIN, GEOM, true, MODE_P64>(hyb, R) : pick5<BIN, GEOM, false, MODE_P64>(hyb, R);
+  }
+  return wt ? pick5<BIN, GEOM, true, MODE_C64>(hyb, R) : pick5<BIN, GEOM, false, MODE_C64>(hyb, R);
 }
 
-static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int mut) {
+static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int R) {
   if (geom == FCFC_GEOM_BOX) {
-    if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_BOX>(wt, mode, hyb, mut);
-    if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_BOX>(wt, mode, hyb, mut);
-    return pick3<FCFC_BIN_SPI, FCFC_GEOM_BOX>(wt, mode, hyb, mut);
+    if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_BOX>(wt, mode, hyb, R);
+    if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_BOX>(wt, mode, hyb, R);
+    return pick3<FCFC_BIN_SPI, FCFC_GEOM_BOX>(wt, mode, hyb, R);
   }
-  if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_SURVEY>(wt, mode, hyb, mut);
-  if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_SURVEY>(wt, mode, hyb, mut);
-  return pick3<FCFC_BIN_SPI, FCFC_GEOM_SURVEY>(wt, mode, hyb, mut);
+  if (bintype == FCFC_BIN_ISO) return pick3<FCFC_BIN_ISO, FCFC_GEOM_SURVEY>(wt, mode, hyb, R);
+  if (bintype == FCFC_BIN_SMU) return pick3<FCFC_BIN_SMU, FCFC_GEOM_SURVEY>(wt, mode, hyb, R);
+  return pick3<FCFC_BIN_SPI, FCFC_GEOM_SURVEY>(wt, mode, hyb, R);
 }
 
 /* FP32 exactness windows.  With coordinates relative to the A tile centre
@@ -1311,8 +1426,10 @@ static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int m
  * evaluated at the largest in-range s^2 (plus the first integer above it:
  * beyond that the table answers "outside" without a window).  The mu bin
  * floor(t), t = nmu |dz| / s, has |t32 - t| <= t 10u + (sqrt(3) u E t +
- * nmu u E) / s.  All bounds are inflated by 25%; pairs with s^2 below
- * s2_tiny (where second-order terms matter) are always re-evaluated. */
+ * nmu u E) / s; with s^2 >= s2_floor = s2max / 64 and t <= nmu this is a
+ * constant D, and the bin is certain when |fract(t32) - 1/2| <= 1/2 - D.
+ * All bounds are inflated by 25%; pairs below s2_floor (about 1/512 of the
+ * in-range volume) are always re-evaluated. */
 static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
     FkWin *win) {
   const double u = ldexp(1.0, -24);
@@ -1327,10 +1444,12 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
   FkWin &w = *win;
   w.e_s2 = (float) (es2 * 1.25 + 1e-7);
   w.e_pi = (float) ((u * (2 * pmx + E) + tiny) * 1.25 + 1e-7);
-  w.e_rrel = (float) (10 * u * 1.25);
-  w.e_rabs = (float) (sqrt(3.0) * u * E * 1.25 + 1e-9);
-  w.e_rz = (float) (b->nmu * (u * E + tiny) * 1.25 + 1e-9);
-  w.s2_tiny = (float) std::max(1e-6, pow(1e4 * u * E, 2));
+  const double s2f = std::max(s2max / 64.0, pow(1e4 * u * E, 2));
+  const double nmu = b->nmu;
+  const double D = 1.25 * (nmu * 10 * u + (sqrt(3.0) * u * E * nmu + nmu * (u * E + tiny)) / sqrt(s2f))
+                   + 4 * nmu * u;
+  w.mu_w = (float) std::max(0.0, 0.5 - D);
+  w.s2_floor = (float) s2f;
   w.nmu_f = (float) b->nmu;
 }
 
@@ -1364,9 +1483,12 @@ static bool same_grid(const FkGrid &a, const FkGrid &b) {
       a.oz == b.oz && a.wx == b.wx && a.wy == b.wy && a.wz == b.wz;
 }
 
+/* weights of a sorted catalogue re-read from the caller's device array */
+struct Gather { const double *src; const int *idx; double *dst; int n; };
+
 static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
     const fcfc_cat *a, const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
-    int autoc, int wt, int mode, PJob *J) {
+    int autoc, int wt, int mode, PJob *J, std::vector<Gather> *gathers = nullptr) {
   const int need_s = (bins->geometry == FCFC_GEOM_SURVEY && bins->bintype != FCFC_BIN_ISO);
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   int e;
@@ -1390,15 +1512,25 @@ static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
     const int upslot = (q == 0) ? WS_IN_X : WS_IN2_X;
     if ((e = stage_cat(c, al, k, need_s, wt, J->P.g, slot, upslot, dst[q]))) return e;
     if (al.own) copies.push_back(CopyKey{k->x, k->n, wt, J->P.g, *dst[q]});
+    if (gathers && wt && k->w && k->on_device && k->n > 0)
+      gathers->push_back(Gather{k->w, dst[q]->idx, dst[q]->w, (int) k->n});
   }
   if (autoc) J->B = J->A;
   return 0;
 }
 
-/* Tile all jobs, then count (or differentiate) them in one persistent launch. */
-static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
+/* A recorded count launch: kernel, grid, LDS bytes and arguments. */
+struct Launch {
+  kfn_t fn = nullptr;
+  int nblock = 0, lds = 0, ntiles = 0;
+  FkArgs a;
+};
+
+/* Tile all jobs and prepare the persistent launch that counts (or
+ * differentiates) them; every buffer comes from al. */
+static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     std::vector<PJob> &jobs, int mode, const double *G, void *dhist,
-    unsigned long long *dstats, unsigned int *dqueue, float *kms) {
+    unsigned long long *dstats, unsigned int *dqueue, Launch *out) {
   const int geom = b->geometry;
   int m = mode;
   if (m == MODE_C32 && geom != FCFC_GEOM_BOX) m = MODE_C64;    /* survey FP32: FP64 kernel */
@@ -1406,14 +1538,20 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   int anywt = 0;
   for (auto &j : jobs) anywt |= j.wt;
   if (m == MODE_GRAD) anywt = 1;
+  /* one kernel serves every job of the launch: tiles of the longest kind */
+  {
+    int R = 1;
+    for (auto &j : jobs) R = std::max(R, (j.P.T + FK_NT - 1) / FK_NT);
+    if (m == MODE_GRAD) R = 1;
+    for (auto &j : jobs) { j.P.R = R; if (j.P.T_auto) j.P.T = R * FK_NT; }
+  }
   /* ---- tiles of all jobs ---- */
   std::vector<int> rowoff(jobs.size() + 1, 0);
   for (size_t k = 0; k < jobs.size(); k++) rowoff[k + 1] = rowoff[k] + jobs[k].P.g.ncx * jobs[k].P.g.ncy;
   const int nrow = rowoff.back();
-  int err = 0;
-  int *rown = (int *) ws(c, WS_ROWN, sizeof(int) * (size_t) nrow, &err);
-  int *rowscan = (int *) ws(c, WS_ROWOFF, sizeof(int) * ((size_t) nrow + 1), &err);
-  if (err) return err;
+  int *rown = (int *) al.get(WS_ROWN, sizeof(int) * (size_t) nrow);
+  int *rowscan = (int *) al.get(WS_ROWOFF, sizeof(int) * ((size_t) nrow + 1));
+  if (al.err) return al.err;
   for (size_t k = 0; k < jobs.size(); k++) {
     const PJob &J = jobs[k];
     const int nr = J.P.g.ncx * J.P.g.ncy;
@@ -1424,39 +1562,54 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   int ntiles = 0;
   HIPCHK(c, hipMemcpyAsync(&ntiles, rowscan + nrow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  FkTile *tiles = (FkTile *) ws(c, WS_TILES, sizeof(FkTile) * (size_t) (ntiles > 0 ? ntiles : 1), &err);
-  if (err) return err;
+  FkTile *tiles = (FkTile *) al.get(WS_TILES, sizeof(FkTile) * (size_t) (ntiles > 0 ? ntiles : 1));
+  if (al.err) return al.err;
   for (size_t k = 0; k < jobs.size(); k++) {
     const PJob &J = jobs[k];
     const int nr = J.P.g.ncx * J.P.g.ncy;
     hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
         J.P.T, J.P.zmaxc, rowscan + rowoff[k], (int) k, tiles);
   }
-  /* ---- LDS plan and kernel ---- */
-  int nh = 1;
-  if (m != MODE_GRAD) {
-    int per = b->ntot * (anywt ? 8 : 4);
-    nh = (per * 4 <= 16384) ? 4 : ((per * 2 <= 24576) ? 2 : 1);
-  }
-  LdsPlan L = lds_plan(b, m, anywt, need_s, nh);
-  if (L.total > 160 * 1024) {
-    set_err(c, "binning tables need %d bytes of LDS (limit 160 KiB)", L.total);
+  /* ---- kernel and LDS plan ---- */
+  const int hyb = b->tabtype ? 1 : 0;
+  const int R = jobs.empty() ? 1 : jobs[0].P.R;
+  kfn_t fn = pick_kernel(b->bintype, geom, anywt, (m == MODE_C32 && c->census) ? MODE_C32C : m, hyb, R);
+  /* histogram copies: as many as keep the occupancy of a single copy */
+  const int lds_max = 160 * 1024 - 2048;       /* room for the static LDS */
+  LdsPlan L = lds_plan(b, m, anywt, need_s, 1);
+  if (L.total > lds_max) {
+    set_err(c, "binning tables need %d bytes of LDS (limit %d)", L.total, lds_max);
     return FCFC_ERR_CFG;
   }
-  const int hyb = b->tabtype ? 1 : 0;
-  const int mut = (b->bintype == FCFC_BIN_SMU && b->nmu * b->nmu <= 16384) ? 1 : 0;
-  kfn_t fn = pick_kernel(b->bintype, geom, anywt, m, hyb, mut);
+  if (m != MODE_GRAD) {
+    auto occ_of = [&](const LdsPlan &P, int *occ) -> int {
+      HIPCHK(c, hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize, P.total));
+      HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, (const void *) fn, FK_NT, P.total));
+      return 0;
+    };
+    int occ1 = 0;
+    if (int e = occ_of(L, &occ1)) return e;
+    const int nhmax = c->t_nh > 0 ? c->t_nh : 8;
+    for (int nh = 2; nh <= nhmax; nh *= 2) {
+      LdsPlan L2 = lds_plan(b, m, anywt, need_s, nh);
+      int occ = 0;
+      if (L2.total > lds_max) break;
+      if (int e = occ_of(L2, &occ)) return e;
+      if (occ < occ1 && c->t_nh <= 0) break;
+      L = L2;
+    }
+  }
   HIPCHK(c, hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
   /* ---- tables, strips, job descriptors ---- */
   size_t nstr = 0;
   for (auto &j : jobs) nstr += j.P.strips.size();
-  uint16_t *dstab = (uint16_t *) ws(c, WS_STAB, sizeof(uint16_t) * (b->stab_len + 1), &err);
-  uint16_t *dptab = (uint16_t *) ws(c, WS_PTAB, sizeof(uint16_t) * (b->ptab_len + 1), &err);
-  double *ds2e = (double *) ws(c, WS_S2E, sizeof(double) * (b->ns + 1), &err);
-  double *dpe = (double *) ws(c, WS_PE, sizeof(double) * (b->np + 1), &err);
-  int *dstrips = (int *) ws(c, WS_STRIPS, sizeof(int) * (nstr ? nstr : 1), &err);
+  uint16_t *dstab = (uint16_t *) al.get(WS_STAB, sizeof(uint16_t) * (b->stab_len + 1));
+  uint16_t *dptab = (uint16_t *) al.get(WS_PTAB, sizeof(uint16_t) * (b->ptab_len + 1));
+  double *ds2e = (double *) al.get(WS_S2E, sizeof(double) * (b->ns + 1));
+  double *dpe = (double *) al.get(WS_PE, sizeof(double) * (b->np + 1));
+  int *dstrips = (int *) al.get(WS_STRIPS, sizeof(int) * (nstr ? nstr : 1));
   FkJob *djobs = nullptr;
-  if (err) return err;
+  if (al.err) return al.err;
   std::vector<int> hstr;
   hstr.reserve(nstr);
   std::vector<FkJob> hj(jobs.size());
@@ -1496,12 +1649,15 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   fb.bintype = b->bintype; fb.geometry = geom;
   fb.ns = b->ns; fb.nmu = b->nmu; fb.np = b->np; fb.ntot = b->ntot;
   fb.tabtype = b->tabtype; fb.soff = b->soff; fb.poff = b->poff;
+  fb.soff_m1 = b->soff - 1; fb.poff_m1 = b->poff - 1;
   fb.stab_len = b->stab_len; fb.ptab_len = b->ptab_len;
-  fb.mutab_lds = mut;
   fb.s2min = b->s2bin[0]; fb.s2max = b->s2bin[b->ns];
   fb.pmin = (b->bintype == FCFC_BIN_SPI) ? b->pbin[0] : 0;
   fb.pmax = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
   fb.nmu2 = (double) b->nmu * b->nmu;
+  fb.nmu_d = (double) b->nmu;
+  fb.s2_exact = std::max(b->s2bin[b->ns] * ldexp(1.0, -12), 4 * FK_DBL_EPS);
+  fb.mu_w = 0.5 - ldexp((double) b->nmu, -20);
   fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
   a.tiles = tiles; a.ntiles = ntiles;
   a.queue = dqueue;
@@ -1509,8 +1665,8 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   a.stats = dstats;
   a.nh = L.nh;
   a.off_hist = L.off_hist; a.off_stab = L.off_stab; a.off_ptab = L.off_ptab;
-  a.off_mutab = L.off_mutab; a.off_edges = L.off_edges; a.off_strip = L.off_strip;
-  a.off_chunk = L.off_chunk;
+  a.off_edges = L.off_edges; a.off_strip = L.off_strip;
+  a.off_chunk = L.off_chunk; a.off_queue = L.off_queue;
   a.census = c->census;
   a.G = G;
   int occ = 0;
@@ -1519,16 +1675,32 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   if (c->tile_cap > 0 && c->tile_cap < ntiles) a.ntiles = c->tile_cap;
   int nblock = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
   if (nblock > a.ntiles) nblock = std::max(a.ntiles, 1);
-  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   c->last_ntiles = ntiles;
-  if (a.ntiles > 0)
-    hipLaunchKernelGGL(fn, dim3(nblock), dim3(FK_NT), L.total, c->stream, a);
+  c->stats.tile = jobs.empty() ? 0 : jobs[0].P.T;
+  c->stats.cell_div = jobs.empty() ? 0 : (int) lround(sqrt(jobs[0].P.g.Rxy2) / jobs[0].P.g.wx);
+  out->fn = fn; out->nblock = nblock; out->lds = L.total; out->ntiles = a.ntiles; out->a = a;
+  return 0;
+}
+
+static int launch(fcfc_gpu_ctx *c, const Launch &Lc) {
+  if (Lc.ntiles > 0)
+    hipLaunchKernelGGL(Lc.fn, dim3(Lc.nblock), dim3(FK_NT), Lc.lds, c->stream, Lc.a);
   HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+/* Tile all jobs, then count (or differentiate) them in one persistent launch. */
+static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
+    std::vector<PJob> &jobs, int mode, const double *G, void *dhist,
+    unsigned long long *dstats, unsigned int *dqueue, float *kms) {
+  Launch Lc;
+  int e = build_launch(c, al, b, jobs, mode, G, dhist, dstats, dqueue, &Lc);
+  if (e) return e;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  if ((e = launch(c, Lc))) return e;
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipEventElapsedTime(kms, c->ev0, c->ev1));
-  c->stats.tile = jobs.empty() ? 0 : jobs[0].P.T;
-  c->stats.cell_div = jobs.empty() ? 0 : (int) lround(sqrt(jobs[0].P.g.Rxy2) / jobs[0].P.g.wx);
   return 0;
 }
 
@@ -1542,6 +1714,34 @@ static int stats_from(fcfc_gpu_ctx *c, unsigned long long *dstats) {
   return 0;
 }
 
+/* add the device histograms of the jobs (8 bytes per bin: int64 or
+ * double) into the caller's cnt (device or host) */
+static int deliver(fcfc_gpu_ctx *c, const std::vector<PJob> &jobs, int ntot,
+    const void *dh, uint32_t flags, void *cnt) {
+  const int njob = (int) jobs.size();
+  const int n = ntot * njob;
+  if (flags & FCFC_GPU_CNT_DEVICE) {
+    for (int k = 0; k < njob; k++) {
+      char *dst = (char *) cnt + (size_t) k * ntot * 8;
+      const char *src = (const char *) dh + (size_t) k * ntot * 8;
+      if (jobs[k].wt) hipLaunchKernelGGL(k_add_f64, dim3(nblk(ntot, 256)), dim3(256), 0, c->stream, (double *) dst, (const double *) src, ntot);
+      else hipLaunchKernelGGL(k_add_u64, dim3(nblk(ntot, 256)), dim3(256), 0, c->stream, (long long *) dst, (const unsigned long long *) src, ntot);
+    }
+    HIPCHK(c, hipGetLastError());
+  }
+  else {
+    std::vector<unsigned long long> h(n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), dh, 8 * (size_t) n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < njob; k++) {
+      const size_t o = (size_t) k * ntot;
+      if (jobs[k].wt) { double *dst = (double *) cnt + o; const double *hd = (const double *) h.data() + o; for (int i = 0; i < ntot; i++) dst[i] += hd[i]; }
+      else { int64_t *dst = (int64_t *) cnt + o; for (int i = 0; i < ntot; i++) dst[i] += (int64_t) h[o + i]; }
+    }
+  }
+  return 0;
+}
+
 /* counts of njob jobs; job k's histogram lands in cnt + k * ntot */
 static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
     const fcfc_cat *const *bs, const double *const *bsz, const uint32_t *fl,
@@ -1550,7 +1750,8 @@ static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
   HIPCHK(c, hipSetDevice(c->device));
   memset(&c->stats, 0, sizeof c->stats);
   c->census = (flags & FCFC_GPU_CENSUS) ? 1 : 0;
-  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 : MODE_C64;
+  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 :
+                   ((flags & FCFC_GPU_PREC_PLAIN) ? MODE_P64 : MODE_C64);
   Alloc al(c, njob > 1);
   std::vector<CopyKey> copies;
   std::vector<PJob> jobs(njob);
@@ -1574,26 +1775,7 @@ static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
   float ms = 0;
   int e = run_jobs(c, al, bins, jobs, mode, nullptr, dh, misc + 8, (unsigned int *) misc, &ms);
   if (e) return e;
-  const int n = bins->ntot * njob;
-  if (flags & FCFC_GPU_CNT_DEVICE) {
-    for (int k = 0; k < njob; k++) {
-      char *dst = (char *) cnt + (size_t) k * bins->ntot * 8;
-      const char *src = (const char *) dh + (size_t) k * bins->ntot * 8;
-      if (jobs[k].wt) hipLaunchKernelGGL(k_add_f64, dim3(nblk(bins->ntot, 256)), dim3(256), 0, c->stream, (double *) dst, (const double *) src, bins->ntot);
-      else hipLaunchKernelGGL(k_add_u64, dim3(nblk(bins->ntot, 256)), dim3(256), 0, c->stream, (long long *) dst, (const unsigned long long *) src, bins->ntot);
-    }
-    HIPCHK(c, hipGetLastError());
-  }
-  else {
-    std::vector<unsigned long long> h(n);
-    HIPCHK(c, hipMemcpyAsync(h.data(), dh, 8 * (size_t) n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int k = 0; k < njob; k++) {
-      const size_t o = (size_t) k * bins->ntot;
-      if (jobs[k].wt) { double *dst = (double *) cnt + o; const double *hd = (const double *) h.data() + o; for (int i = 0; i < bins->ntot; i++) dst[i] += hd[i]; }
-      else { int64_t *dst = (int64_t *) cnt + o; for (int i = 0; i < bins->ntot; i++) dst[i] += (int64_t) h[o + i]; }
-    }
-  }
+  if ((e = deliver(c, jobs, bins->ntot, dh, flags, cnt))) return e;
   if ((e = stats_from(c, misc + 8))) return e;
   c->stats.kernel_ms = ms;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1689,6 +1871,105 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
   return 0;
 }
 
+/* ---- prepared batches replayed as a HIP graph -------------------------- */
+
+struct fcfc_gpu_plan {
+  fcfc_gpu_ctx *c;
+  Alloc al;                          /* every device buffer of the plan */
+  std::vector<PJob> jobs;
+  std::vector<Gather> gathers;
+  int ntot = 0;
+  uint32_t flags = 0;
+  Launch L;
+  void *dh = nullptr;                /* njob x ntot histograms (8 bytes per bin) */
+  unsigned long long *misc = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  explicit fcfc_gpu_plan(fcfc_gpu_ctx *c_) : c(c_), al(c_, true) {}
+  ~fcfc_gpu_plan() {
+    if (c && c->stream) (void) hipStreamSynchronize(c->stream);
+    if (exec) (void) hipGraphExecDestroy(exec);
+    if (graph) (void) hipGraphDestroy(graph);
+  }
+};
+
+/* Sort and tile the catalogues of njob jobs once; record the launch
+ * sequence of one count -- weight gathers from the callers' device arrays,
+ * histogram and queue clears (fill kernels), the persistent count kernel --
+ * as a HIP graph. */
+extern "C" int fcfc_gpu_plan_create(fcfc_gpu_ctx *c, int njob, const fcfc_batch_job *jobs,
+    const fcfc_bins *bins, uint32_t flags, fcfc_gpu_plan **out) {
+  if (!c || !out || njob < 1 || !jobs || !bins_ok(bins)) return FCFC_ERR_ARG;
+  *out = nullptr;
+  HIPCHK(c, hipSetDevice(c->device));
+  memset(&c->stats, 0, sizeof c->stats);
+  c->census = 0;
+  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 :
+                   ((flags & FCFC_GPU_PREC_PLAIN) ? MODE_P64 : MODE_C64);
+  std::unique_ptr<fcfc_gpu_plan> P(new fcfc_gpu_plan(c));
+  P->ntot = bins->ntot;
+  P->flags = flags;
+  P->jobs.resize(njob);
+  std::vector<CopyKey> copies;
+  for (int k = 0; k < njob; k++) {
+    const fcfc_batch_job &j = jobs[k];
+    const int autoc = (j.flags & FCFC_GPU_AUTO) ? 1 : 0;
+    const int wt = (j.flags & FCFC_GPU_WEIGHTED) ? 1 : 0;
+    if (!cat_ok(&j.a) || (!autoc && !cat_ok(&j.b))) { set_err(c, "invalid catalogue in job %d", k); return FCFC_ERR_ARG; }
+    int e = setup_job(c, P->al, copies, &j.a, &j.b, bins, j.bsize, autoc, wt, mode, &P->jobs[k], &P->gathers);
+    if (e) return e;
+    P->jobs[k].hist_off = (long long) k * bins->ntot;
+  }
+  P->dh = P->al.get(WS_HIST, 8 * (size_t) bins->ntot * njob);
+  P->misc = (unsigned long long *) P->al.get(WS_MISC, 64 * 8);
+  if (P->al.err) return P->al.err;
+  int e = build_launch(c, P->al, bins, P->jobs, mode, nullptr, P->dh, P->misc + 8,
+                       (unsigned int *) P->misc, &P->L);
+  if (e) return e;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (const Gather &g : P->gathers)
+    hipLaunchKernelGGL(k_gather, dim3(nblk(g.n, 256)), dim3(256), 0, c->stream, g.n, g.idx, g.src, g.dst);
+  hipLaunchKernelGGL(k_zero_u64, dim3(nblk((int64_t) bins->ntot * njob, 256)), dim3(256), 0, c->stream,
+      (unsigned long long *) P->dh, bins->ntot * njob);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, P->misc, 64);
+  e = launch(c, P->L);
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  if (e) { if (graph) (void) hipGraphDestroy(graph); return e; }
+  if (ce != hipSuccess) { set_err(c, "graph capture failed: %s", hipGetErrorString(ce)); return FCFC_ERR_GPU; }
+  P->graph = graph;
+  HIPCHK(c, hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+  *out = P.release();
+  return 0;
+}
+
+/* One count of every job of the plan, histograms added into cnt (njob x
+ * ntot; FCFC_GPU_CNT_DEVICE in the plan's flags: a device pointer). */
+extern "C" int fcfc_gpu_plan_run(fcfc_gpu_plan *P, void *cnt) {
+  if (!P || !cnt) return FCFC_ERR_ARG;
+  fcfc_gpu_ctx *c = P->c;
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  HIPCHK(c, hipGraphLaunch(P->exec, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  int e = deliver(c, P->jobs, P->ntot, P->dh, P->flags, cnt);
+  if (e) return e;
+  if ((e = stats_from(c, P->misc + 8))) return e;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->stats.kernel_ms = ms;
+  c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+extern "C" int fcfc_gpu_plan_destroy(fcfc_gpu_plan *P) {
+  delete P;
+  return 0;
+}
+
 /* Project-written autotuner: for jobs like (a, b, bins, flags) try grid and
  * tile shapes (cells per maximum separation, z-cells per cell, A objects
  * per tile), timing each on the first eighth of the tile list (a slab of the
@@ -1700,11 +1981,12 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
     uint32_t flags, int *best) {
   if (!c || !bins_ok(bins)) return FCFC_ERR_ARG;
-  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 : MODE_C64;
+  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 :
+                   ((flags & FCFC_GPU_PREC_PLAIN) ? MODE_P64 : MODE_C64);
   const int save_cd = c->t_cell_div, save_zd = c->t_zdiv, save_t = c->t_tile;
   std::vector<unsigned char> scratch(8 * (size_t) bins->ntot);
   double bestms = 1e300;
-  int bcd = 4, bzd = 4, bT = 512;
+  int bcd = 4, bzd = 4, bT = 0;
   auto measure = [&](int cd, int zd, int T, double *est) -> int {
     c->t_cell_div = cd; c->t_zdiv = zd; c->t_tile = T;
     /* first pass: learn the number of tiles of this shape with a tiny cap */
@@ -1723,16 +2005,16 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
   const int cds[] = {3, 4, 6, 8};
   for (int cd : cds) {
     double est;
-    if ((e = measure(cd, 4, 512, &est))) break;
+    if ((e = measure(cd, 4, 0, &est))) break;
     if (est < bestms) { bestms = est; bcd = cd; }
   }
   const int zds[] = {2, 4, 8};
   for (int zd : zds) {
     double est;
-    if (e || (e = measure(bcd, zd, 512, &est))) break;
+    if (e || (e = measure(bcd, zd, 0, &est))) break;
     if (est < bestms) { bestms = est; bzd = zd; }
   }
-  const int Ts[] = {256, 512};
+  const int Ts[] = {FK_NT, 2 * FK_NT};
   for (int T : Ts) {
     double est;
     if (e || (e = measure(bcd, bzd, T, &est))) break;

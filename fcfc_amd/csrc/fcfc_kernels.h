@@ -30,11 +30,12 @@
 
 #include <stdint.h>
 
-#define FK_NT        256       /* threads per workgroup (4 wave64)          */
+#define FK_NT        512       /* threads per workgroup (8 wave64)          */
 #define FK_NSB       64        /* strips per batch = one wavefront          */
 #define FK_MAXSEG    3         /* z images per strip                        */
 #define FK_CH        256       /* B objects per LDS chunk                   */
 #define FK_OUT       0xFFFFu   /* table sentinel: outside [s2min, s2max)    */
+#define FK_QCAP      128       /* FP32 re-evaluation queue per wavefront    */
 
 /* Binning constants passed by value to the kernels. */
 struct FkBins {
@@ -42,11 +43,15 @@ struct FkBins {
   int ns, nmu, np, ntot;
   int tabtype;               /* 0 integer table, 1 hybrid table             */
   int soff, poff;            /* (int) s2min, (int) pmin (or pi^2 min)       */
+  int soff_m1, poff_m1;      /* soff - 1, poff - 1                          */
   int stab_len, ptab_len;    /* table lengths without sentinels             */
-  int mutab_lds;             /* mu table (isqrt) staged in LDS              */
   double s2min, s2max;       /* squared s (or s_perp) range                 */
   double pmin, pmax;         /* pi range (box) or pi^2 range (survey)       */
   double nmu2;               /* nmu * nmu as double                          */
+  double nmu_d;              /* nmu as double                                */
+  double mu_w;               /* mu bin settled exactly when the fraction of  */
+                             /* t is outside [0.5 - mu_w, 0.5 + mu_w]        */
+  double s2_exact;           /* ... or s^2 is below this value               */
   const uint16_t *stab;      /* device tables (global), staged to LDS       */
   const uint16_t *ptab;
   const double *s2bin;       /* edges for the hybrid refinement              */

@@ -134,8 +134,15 @@ const char *fcfc_gpu_last_error(const fcfc_gpu_ctx *ctx);
 /* flags for fcfc_gpu_count* */
 #define FCFC_GPU_AUTO         0x01u  /* b is ignored: auto pairs of a, once */
 #define FCFC_GPU_WEIGHTED     0x02u  /* FP64 weighted counts (double cnt[]) */
-#define FCFC_GPU_PREC_FP64    0x00u  /* plain FP64 kernel                   */
+#define FCFC_GPU_PREC_FP64    0x00u  /* FP64 distances; mu bin from the
+                                        reciprocal square root, settled
+                                        exactly near its edges             */
 #define FCFC_GPU_PREC_FP32    0x10u  /* FP32 distances + FP64 edge recheck  */
+#define FCFC_GPU_PREC_PLAIN   0x80u  /* FP64, the reference's per-pair
+                                        arithmetic transcribed directly (an
+                                        IEEE division per (s,mu) pair); the
+                                        baseline the other forms are timed
+                                        against                            */
 #define FCFC_GPU_CNT_DEVICE   0x20u  /* cnt is a device pointer             */
 #define FCFC_GPU_CENSUS       0x40u  /* count pairs whose bin is rounding-path
                                         dependent in the reference (FP32 mode) */
@@ -175,6 +182,25 @@ typedef struct {
 int fcfc_gpu_count_batch(fcfc_gpu_ctx *ctx, int njob,
     const fcfc_batch_job *jobs, const fcfc_bins *bins, uint32_t flags,
     void *cnt);
+
+/*
+ * Prepared batches (no counterpart in the reference).  fcfc_gpu_plan_create
+ * sorts the catalogues of njob jobs into their grids and tiles them once,
+ * then records the device work of one count -- the weights re-read from the
+ * callers' device arrays (fcfc_cat.w of on_device catalogues), histogram and
+ * queue clears, the persistent count launch -- as a HIP graph.  Each
+ * fcfc_gpu_plan_run replays the graph and adds the njob histograms into cnt
+ * (as fcfc_gpu_count_batch; FCFC_GPU_CNT_DEVICE in flags: device pointer).
+ * For counts repeated on fixed positions with changing weights (weight
+ * fitting).  The positions are those at creation; the context must outlive
+ * the plan and runs of one context must not overlap.
+ */
+typedef struct fcfc_gpu_plan fcfc_gpu_plan;
+int fcfc_gpu_plan_create(fcfc_gpu_ctx *ctx, int njob,
+    const fcfc_batch_job *jobs, const fcfc_bins *bins, uint32_t flags,
+    fcfc_gpu_plan **plan);
+int fcfc_gpu_plan_run(fcfc_gpu_plan *plan, void *cnt);
+int fcfc_gpu_plan_destroy(fcfc_gpu_plan *plan);
 
 /*
  * Backward of the weighted counts with respect to per-object weights.

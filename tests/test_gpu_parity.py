@@ -59,7 +59,7 @@ def cats():
 
 
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "plain"])
 def test_box_cross(eng, cats, bt, prec):
     (x, y, z, w), (xr, yr, zr) = cats
     b = _bins(F.GEOM_BOX, bt)
@@ -77,7 +77,7 @@ def test_box_cross(eng, cats, bt, prec):
 
 
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "plain"])
 def test_box_auto(eng, cats, bt, prec):
     (x, y, z, w), _ = cats
     b = _bins(F.GEOM_BOX, bt)
@@ -94,21 +94,44 @@ def test_box_auto(eng, cats, bt, prec):
 
 
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
-def test_survey(eng, bt):
+@pytest.mark.parametrize("prec", ["fp64", "plain"])
+def test_survey(eng, bt, prec):
     x, y, z, w = F.gen_survey(21, 4000)
     xr, yr, zr, wr = F.gen_survey(22, 6000)
     b = _bins(F.GEOM_SURVEY, bt)
     d = F.Catalog(b, x, y, z, w)
     r = F.Catalog(b, xr, yr, zr, wr)
-    print(f"case survey bt={bt}", flush=True)
+    print(f"case survey bt={bt} prec={prec}", flush=True)
     ob = _obins(b)
     refa, cen = O.count(1, None, (d.x, d.y, d.z, d.s, None), None, True, ob, variant=0,
                         census_mask=0b111111)
-    ga = eng.count_pairs(d, None, b, isauto=True, withwt=False)
+    ga = eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec)
     _cmp(ga, refa, False, cen)
     refx, _ = O.count(1, None, _otuple(d), _otuple(r), False, ob, variant=0, weighted=True)
-    gx = eng.count_pairs(d, r, b, isauto=False, withwt=True)
+    gx = eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec)
     _cmp(gx, refx, True)
+
+
+@pytest.mark.parametrize("nmu", [100, 7, 12])
+def test_lattice_mu_edges(eng, nmu):
+    """Objects on an integer lattice (in rescaled units): many pairs sit
+    exactly on mu and s edges (e.g. (3, 0, 4): mu = 4/5), where the fast mu
+    bin must fall back to its exact test and the FP32 windows to FP64."""
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=nmu, bsize=L)
+    f = b.rescale
+    g = np.arange(14) / f + 100.0
+    x, y, z = [a.ravel().copy() for a in np.meshgrid(g, g, g, indexing="ij")]
+    d = F.Catalog(b, x, y, z)
+    assert np.all(d.x == np.round(d.x))
+    # exact coordinates: every path of the GPU is the oracle's variant 1 bit
+    # for bit (the reference's own unfused scalar path, which divides before
+    # multiplying by nmu^2, disagrees on some of these pairs: census > 0)
+    ref, _ = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None), None, True, _obins(b),
+                     variant=1)
+    for prec in ("fp64", "fp32", "plain"):
+        print(f"case lattice nmu={nmu} prec={prec}", flush=True)
+        gl = eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec)
+        np.testing.assert_array_equal(gl, ref)
 
 
 def test_hybrid_bins(eng, cats):
@@ -118,7 +141,7 @@ def test_hybrid_bins(eng, cats):
     assert b.tabtype == 1
     d = F.Catalog(b, x, y, z, w)
     r = F.Catalog(b, xr, yr, zr)
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "plain"):
         g = eng.count_pairs(d, r, b, isauto=False, withwt=False, precision=prec)
         ref, _ = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None),
                          (r.x, r.y, r.z, None, None), False, _obins(b), variant=1)
@@ -132,7 +155,7 @@ def test_launch_shapes_identical(eng, cats):
     d = F.Catalog(b, x, y, z, w)
     r = F.Catalog(b, xr, yr, zr)
     res = []
-    for cd, tile, zd, nb in [(0, 0, 0, 0), (1, 64, 1, 3), (3, 200, 7, 17), (6, 256, 2, 0)]:
+    for cd, tile, zd, nb in [(0, 0, 0, 0), (1, 64, 1, 3), (3, 200, 7, 17), (6, 700, 2, 0), (4, 1024, 4, 0)]:
         eng.set_tuning(cd, tile, zd, nb)
         res.append(eng.count_pairs(d, r, b, isauto=False, withwt=False))
         res.append(eng.count_pairs(d, None, b, isauto=True, withwt=False))
