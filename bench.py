@@ -195,6 +195,75 @@ def mocks_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def survey_workload(args):
+    """Survey-like geometry: data 1e6 and randoms 2e7 in the comoving shell
+    r in [1000, 2000] Mpc/h over a 60 x 60 deg cap, observer at the origin,
+    FKP weights; weighted DD, DR, RR in (s_perp, pi) 40 x 40 bins up to 200
+    Mpc/h, and w_p(s_perp).  One step = the three counts."""
+    import torch
+    import fcfc_amd as F
+    from fcfc_amd.xi import wp
+    nd, nr = int(1e6 * args.scale), int(2e7 * args.scale)
+    b = F.Bins.linear(F.GEOM_SURVEY, F.BIN_SPI, 0, 200, 5, pmin=0, pmax=200, dpi=5)
+    t0 = time.time()
+    x, y, z, w = F.gen_survey(31, nd)
+    xr, yr, zr, wr = F.gen_survey(32, nr)
+    D = to_device(b, (x, y, z, w), F, torch)
+    R = to_device(b, (xr, yr, zr, wr), F, torch)
+    torch.cuda.synchronize()
+    log(f"survey workload ready in {time.time() - t0:.1f} s: D={D.n} R={R.n}")
+    eng = F.Engine(0)
+    prec = "fp64" if args.precision == "fp32" else args.precision   # survey: FP64 forms
+
+    def step(p):
+        out, ev, km = {}, 0.0, {}
+        for name, a, c, au in (("DD", D, None, True), ("DR", D, R, False), ("RR", R, None, True)):
+            out[name] = eng.count_pairs(a, c, b, isauto=au, withwt=True, precision=p)
+            st = eng.stats()
+            ev += st["evals"]
+            km[name] = st["kernel_ms"]
+        return out, ev, km
+
+    for i in range(args.warmup):
+        step(prec)
+    torch.cuda.synchronize()
+    t1 = time.time()
+    ev_tot = 0.0
+    for i in range(args.steps):
+        res, ev, km = step(prec)
+        ev_tot += ev
+        log(f"survey step {i} done")
+    torch.cuda.synchronize()
+    dt = time.time() - t1
+    sw, swr = D.sumw, R.sumw
+    dd = 2 * res["DD"] / (sw * sw)
+    dr = res["DR"] / (sw * swr)
+    rr = 2 * res["RR"] / (swr * swr)
+    xi = np.where(rr > 0, (dd - 2 * dr + rr) / np.where(rr > 0, rr, 1), 0.0).reshape(40, 40)
+    line = {
+        "metric": METRIC, "value": ev_tot / dt, "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "survey shell r in [1000,2000] Mpc/h, 60x60 deg cap, FKP weights; "
+                               "D=1e6, R=2e7; weighted DD+DR+RR (s_perp,pi) 40x40 to 200 + w_p",
+                   "scale": args.scale, "n_data": D.n, "n_random": R.n, "precision": prec},
+        "kernel_ms": km, "evals_per_step": ev_tot / args.steps,
+        "wp_first_bins": [float(v) for v in wp(xi, b.pbin_raw)[:5]],
+    }
+    if "plain" not in args.skip and prec != "plain":
+        t2 = time.time()
+        rp, evp, kmp = step("plain")
+        torch.cuda.synchronize()
+        dtp = time.time() - t2
+        line["forms"] = {"plain": {"evals_per_s": evp / dtp, "kernel_ms": kmp,
+                                   "weighted_max_rel": max(float(np.max(np.abs(res[k] - rp[k]) /
+                                                           np.maximum(np.abs(rp[k]), 1e-300)))
+                                                           for k in res)}}
+    eng.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,13 +276,16 @@ def main():
     ap.add_argument("--skip", nargs="*", default=[], choices=["fp64", "fp32", "plain"],
                     help="forms not to run for comparison")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c2", "mocks"], default="c2",
-                    help="c2: the headline clustered box DD/DR/RR; mocks: 256 batched mocks")
+    ap.add_argument("--workload", choices=["c2", "mocks", "survey"], default="c2",
+                    help="c2: the headline clustered box DD/DR/RR; mocks: 256 batched mocks; "
+                         "survey: the survey-shell (s_perp, pi) counts")
     ap.add_argument("--copy-calib", action="store_true",
                     help="start with a 1 GiB device-to-device copy (calibrates HBM byte "
                          "counters of a profiler run)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="pick grid/tile shapes with the autotuner before timing")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default grid/tile shapes (by default the autotuner picks "
+                         "them for each count and form before timing; its results are cached "
+                         "in the engine like a production run's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this "
@@ -236,6 +308,9 @@ def main():
     if args.workload == "mocks":
         mocks_workload(args)
         return
+    if args.workload == "survey":
+        survey_workload(args)
+        return
 
     t0 = time.time()
     L, dcat, rcat = make_workload(args.scale)
@@ -246,11 +321,16 @@ def main():
     log(f"workload ready in {time.time() - t0:.1f} s: D={D.n} R={R.n}")
 
     eng = F.Engine(0)
-    if args.autotune:
+
+    def tune(p):
+        if args.no_autotune:
+            return
         for name, a, c, au, wt in (("DD", D, None, True, True), ("DR", D, R, False, True),
                                    ("RR", R, None, True, False)):
-            log(f"autotune {name}: {eng.autotune(a, c, b, isauto=au, withwt=wt, precision=args.precision)}")
+            log(f"autotune {p} {name}: {eng.autotune(a, c, b, isauto=au, withwt=wt, precision=p)}")
+
     prec = args.precision
+    tune(prec)
     for i in range(args.warmup):
         run_step(eng, b, D, R, prec)
         log(f"warmup step {i} done")
@@ -301,6 +381,7 @@ def main():
     # the other forms, one step each, on the same data
     forms = {}
     for other in [f for f in ("plain", "fp64", "fp32") if f != prec and f not in args.skip]:
+        tune(other)
         t2 = time.time()
         ro, evo, kmo, rco = run_step(eng, b, D, R, other)
         torch.cuda.synchronize()

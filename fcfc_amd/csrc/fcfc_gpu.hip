@@ -113,38 +113,46 @@ __global__ void k_survey_norm(int n, const double *__restrict__ x,
 /* Tiles of one row: consecutive cells, at most T objects, at most zmaxc
  * z-cells; a cell with more than T objects is split into T-sized tiles. */
 __device__ int row_tiles(const int *__restrict__ cs, int ncz, int T, int zmaxc,
-    int row, int job, FkTile *out) {
+    int row, int job, int nstrips, int S, FkTile *out) {
   int nt = 0, open = 0, tp0 = 0, tcnt = 0, tz0 = 0, tz1 = 0;
+  /* emit tile (p0, p1, iz0, iz1) as S items over the strip list */
+  auto emit = [&](int p0, int p1, int iz0, int iz1) {
+    if (out)
+      for (int q = 0; q < S; q++)
+        out[nt * S + q] = FkTile{row, p0, p1, iz0, iz1, job,
+                                 (int) ((long long) nstrips * q / S),
+                                 (int) ((long long) nstrips * (q + 1) / S)};
+    nt++;
+  };
   for (int iz = 0; iz < ncz; iz++) {
     const int c = cs[iz + 1] - cs[iz];
     if (c == 0) continue;
     if (open && (tcnt + c > T || iz - tz0 + 1 > zmaxc)) {
-      if (out) out[nt] = FkTile{row, tp0, tp0 + tcnt, tz0, tz1, job};
-      nt++; open = 0;
+      emit(tp0, tp0 + tcnt, tz0, tz1);
+      open = 0;
     }
     if (c > T) {
-      for (int k = 0; k < c; k += T) {
-        if (out) out[nt] = FkTile{row, cs[iz] + k, cs[iz] + min(k + T, c), iz, iz, job};
-        nt++;
-      }
+      for (int k = 0; k < c; k += T) emit(cs[iz] + k, cs[iz] + min(k + T, c), iz, iz);
       continue;
     }
     if (!open) { open = 1; tp0 = cs[iz]; tcnt = 0; tz0 = iz; }
     tcnt += c; tz1 = iz;
   }
-  if (open) { if (out) out[nt] = FkTile{row, tp0, tp0 + tcnt, tz0, tz1, job}; nt++; }
+  if (open) emit(tp0, tp0 + tcnt, tz0, tz1);
   return nt;
 }
 
 __global__ void k_tile_count(int nrow, int ncz, const int *__restrict__ cstart,
     int T, int zmaxc, int *__restrict__ ntile) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
-    ntile[r] = row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, 0, nullptr);
+    ntile[r] = row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, 0, 0, 1, nullptr);
 }
+/* items of row r start at S * toff[r] */
 __global__ void k_tile_emit(int nrow, int ncz, const int *__restrict__ cstart,
-    int T, int zmaxc, const int *__restrict__ toff, int job, FkTile *__restrict__ tiles) {
+    int T, int zmaxc, const int *__restrict__ toff, int job, int nstrips, int S,
+    FkTile *__restrict__ tiles) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
-    row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, job, tiles + toff[r]);
+    row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, job, nstrips, S, tiles + (size_t) S * toff[r]);
 }
 
 /* ======================================================================== */
@@ -801,8 +809,8 @@ k_count(const FkArgs a) {
       if (since_flush > 2.0e9) { flush(); since_flush = 0; }
     };
 
-    /* ---- auto counts: pairs inside the tile (j > i) ---- */
-    if (!GRAD && J.autoc) {
+    /* ---- auto counts: pairs inside the tile (j > i), first item only ---- */
+    if (!GRAD && J.autoc && t.s0 == 0) {
       for (int c0 = t.p0; c0 < t.p1; c0 += FK_CH) {
         const int cnt = min(FK_CH, t.p1 - c0);
         __syncthreads();
@@ -823,13 +831,13 @@ k_count(const FkArgs a) {
     }
 
     /* ---- strips in batches of one wavefront ---- */
-    for (int sb0 = 0; sb0 < J.nstrips; sb0 += FK_NSB) {
+    for (int sb0 = t.s0; sb0 < t.s1; sb0 += FK_NSB) {
       __syncthreads();
       if (wave == 0) {
         const int l = tid;
         int tot = 0;
         int b0[3], len[3], code[3];
-        if (sb0 + l < J.nstrips) {
+        if (sb0 + l < t.s1) {
           const int *sp = J.strips + 3 * (sb0 + l);
           fk_strip_segments(J, t, sp[0], sp[1], sp[2], b0, len, code);
           tot = len[0] + len[1] + len[2];
@@ -885,7 +893,7 @@ k_count(const FkArgs a) {
     if (GRAD) {
 #pragma unroll
       for (int r = 0; r < R; r++)
-        if (ai[r] != 0x7fffffff) J.gA[J.A.idx[ai[r]]] += gacc[r];
+        if (ai[r] != 0x7fffffff) unsafeAtomicAdd(J.gA + J.A.idx[ai[r]], gacc[r]);
     }
   }
 
@@ -925,7 +933,7 @@ struct fcfc_gpu_ctx {
   int census = 0;
   int tile_cap = 0;            /* autotuner: count only the first tiles */
   int last_ntiles = 0;
-  struct Tuned { int geom, bintype, mode, wt, bucket, cd, zd, T; };
+  struct Tuned { int geom, bintype, mode, autoc, wt, bucket, cd, zd, T; };
   std::vector<Tuned> tuned;    /* autotuner results per density bucket */
   std::vector<DevBuf> bufs;
 };
@@ -1054,7 +1062,7 @@ static double max_sep2(const fcfc_bins *b, double *rxy2, double *hzc) {
 
 /* Build the grid geometry and strip list.  lo/hi: bounding box (survey). */
 static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
-    const double lo[3], const double hi[3], int autoc, int mode, int64_t nA,
+    const double lo[3], const double hi[3], int autoc, int wt, int mode, int64_t nA,
     int64_t nB, JobPlan *P) {
   double rxy2, hzc;
   double r2 = max_sep2(b, &rxy2, &hzc);
@@ -1081,12 +1089,17 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
   vol = ext[0] * ext[1] * ext[2];
   const double nden = (double) std::max<int64_t>(nB, 1) / vol;
   /* shapes picked by fcfc_gpu_autotune for this kind of job */
-  const int bucket = (int) floor(2 * log2(nden * sqrt(rxy2) * sqrt(rxy2) * sqrt(r2) + 1e-300));
+  /* density buckets (half octaves of objects per separation volume) of
+   * B and of A: a tuned shape serves counts of the same kind only */
+  const double ndenA = (double) std::max<int64_t>(nA, 1) / vol;
+  const double rv = sqrt(rxy2) * sqrt(rxy2) * sqrt(r2);
+  const int bucket = 4096 * (int) floor(2 * log2(nden * rv + 1e-300)) +
+                     (int) floor(2 * log2(ndenA * rv + 1e-300));
   int tT = c->t_tile;
   if (cd <= 0 || zd <= 0 || tT <= 0) {
     for (const auto &t : c->tuned)
       if (t.geom == b->geometry && t.bintype == b->bintype && t.mode == mode &&
-          t.bucket == bucket) {
+          t.autoc == autoc && t.wt == wt && t.bucket == bucket) {
         if (cd <= 0) cd = t.cd;
         if (zd <= 0) zd = t.zd;
         if (tT <= 0) tT = t.T;
@@ -1095,14 +1108,13 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
   }
   P->bucket = bucket;
   if (cd <= 0) {
-    /* about 24..64 B objects per xy cell per unit of R along z is a good
-     * balance between pruning and segment overhead */
+    /* three cells per maximum separation (measured best for dense boxes);
+     * fewer for sparse catalogues, so that columns keep enough objects */
     double objs_per_col = nden * Rxy * Rxy;   /* per unit z, for cd = 1 */
-    cd = 4;
+    cd = 3;
     while (cd > 1 && objs_per_col / (cd * cd) < 4.0) cd--;
-    while (cd < 8 && objs_per_col / (cd * cd) > 256.0) cd++;
   }
-  if (zd <= 0) zd = 4;
+  if (zd <= 0) zd = 8;
   double w = Rxy / cd;
   if (b->geometry == FCFC_GEOM_BOX) {
     g.periodic = 1;
@@ -1154,8 +1166,10 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
   if (mode == MODE_GRAD) P->T = FK_NT;
   else if (tT > 0) P->T = std::min(tT, 2 * FK_NT);
   else {
-    const double per_row = (double) nA / ((double) g.ncx * g.ncy);
-    P->T = (per_row >= 3.0 * FK_NT) ? 2 * FK_NT : FK_NT;
+    /* two objects per lane only when the z extent allowed to a tile holds
+     * enough objects to fill such tiles (idle lanes cost full issue slots) */
+    const double per_window = (double) nA / ((double) g.ncx * g.ncy) * P->zmaxc / g.ncz;
+    P->T = (per_window >= 2.5 * FK_NT) ? 2 * FK_NT : FK_NT;
   }
   P->R = (P->T + FK_NT - 1) / FK_NT;
   /* strips */
@@ -1490,6 +1504,11 @@ static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
     const fcfc_cat *a, const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
     int autoc, int wt, int mode, PJob *J, std::vector<Gather> *gathers = nullptr) {
   const int need_s = (bins->geometry == FCFC_GEOM_SURVEY && bins->bintype != FCFC_BIN_ISO);
+  /* cross counts are symmetric in the two catalogues: the larger one goes to
+   * the lanes (fuller tiles, tighter windows and fewer lanes sharing one
+   * histogram bin when the other is clustered); the gradient keeps the
+   * caller's orientation */
+  if (!autoc && mode != MODE_GRAD && b->n > a->n) std::swap(a, b);
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   int e;
   if (bins->geometry == FCFC_GEOM_SURVEY) {
@@ -1497,7 +1516,7 @@ static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
     if (!autoc && (e = bbox(c, b, lo, hi))) return e;
     if (!(lo[0] <= hi[0])) { for (int k = 0; k < 3; k++) { lo[k] = 0; hi[k] = 1; } }
   }
-  if ((e = make_plan(c, bins, bsize, lo, hi, autoc, mode, a->n, autoc ? a->n : b->n, &J->P))) return e;
+  if ((e = make_plan(c, bins, bsize, lo, hi, autoc, wt, mode, a->n, autoc ? a->n : b->n, &J->P))) return e;
   J->autoc = autoc;
   J->wt = wt;
   const fcfc_cat *cs[2] = {a, autoc ? a : b};
@@ -1545,31 +1564,6 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     if (m == MODE_GRAD) R = 1;
     for (auto &j : jobs) { j.P.R = R; if (j.P.T_auto) j.P.T = R * FK_NT; }
   }
-  /* ---- tiles of all jobs ---- */
-  std::vector<int> rowoff(jobs.size() + 1, 0);
-  for (size_t k = 0; k < jobs.size(); k++) rowoff[k + 1] = rowoff[k] + jobs[k].P.g.ncx * jobs[k].P.g.ncy;
-  const int nrow = rowoff.back();
-  int *rown = (int *) al.get(WS_ROWN, sizeof(int) * (size_t) nrow);
-  int *rowscan = (int *) al.get(WS_ROWOFF, sizeof(int) * ((size_t) nrow + 1));
-  if (al.err) return al.err;
-  for (size_t k = 0; k < jobs.size(); k++) {
-    const PJob &J = jobs[k];
-    const int nr = J.P.g.ncx * J.P.g.ncy;
-    hipLaunchKernelGGL(k_tile_count, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
-        J.P.T, J.P.zmaxc, rown + rowoff[k]);
-  }
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, rown, rowscan, nrow);
-  int ntiles = 0;
-  HIPCHK(c, hipMemcpyAsync(&ntiles, rowscan + nrow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  FkTile *tiles = (FkTile *) al.get(WS_TILES, sizeof(FkTile) * (size_t) (ntiles > 0 ? ntiles : 1));
-  if (al.err) return al.err;
-  for (size_t k = 0; k < jobs.size(); k++) {
-    const PJob &J = jobs[k];
-    const int nr = J.P.g.ncx * J.P.g.ncy;
-    hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
-        J.P.T, J.P.zmaxc, rowscan + rowoff[k], (int) k, tiles);
-  }
   /* ---- kernel and LDS plan ---- */
   const int hyb = b->tabtype ? 1 : 0;
   const int R = jobs.empty() ? 1 : jobs[0].P.R;
@@ -1600,6 +1594,46 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     }
   }
   HIPCHK(c, hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
+  /* ---- tiles of all jobs ---- */
+  std::vector<int> rowoff(jobs.size() + 1, 0);
+  for (size_t k = 0; k < jobs.size(); k++) rowoff[k + 1] = rowoff[k] + jobs[k].P.g.ncx * jobs[k].P.g.ncy;
+  const int nrow = rowoff.back();
+  int *rown = (int *) al.get(WS_ROWN, sizeof(int) * (size_t) nrow);
+  int *rowscan = (int *) al.get(WS_ROWOFF, sizeof(int) * ((size_t) nrow + 1));
+  if (al.err) return al.err;
+  for (size_t k = 0; k < jobs.size(); k++) {
+    const PJob &J = jobs[k];
+    const int nr = J.P.g.ncx * J.P.g.ncy;
+    hipLaunchKernelGGL(k_tile_count, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
+        J.P.T, J.P.zmaxc, rown + rowoff[k]);
+  }
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, rown, rowscan, nrow);
+  int ntile0 = 0;
+  HIPCHK(c, hipMemcpyAsync(&ntile0, rowscan + nrow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  /* persistent workgroups, and work items per tile: enough items that the
+   * last ones are short against the whole (16 per workgroup) */
+  int occ = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *) fn, FK_NT, L.total));
+  if (occ < 1) occ = 1;
+  const int nwg = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
+  int S = 1;
+  if (ntile0 > 0) {
+    int minstr = 1 << 30;
+    for (auto &j : jobs) minstr = std::min(minstr, std::max(j.P.nstrips, 1));
+    S = (int) std::min<long long>(std::max<long long>(1, (16LL * nwg + ntile0 - 1) / ntile0),
+                                  std::min(minstr, 64));
+    if ((long long) S * ntile0 > (1LL << 30)) S = 1;
+  }
+  const int ntiles = ntile0 * S;
+  FkTile *tiles = (FkTile *) al.get(WS_TILES, sizeof(FkTile) * (size_t) (ntiles > 0 ? ntiles : 1));
+  if (al.err) return al.err;
+  for (size_t k = 0; k < jobs.size(); k++) {
+    const PJob &J = jobs[k];
+    const int nr = J.P.g.ncx * J.P.g.ncy;
+    hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
+        J.P.T, J.P.zmaxc, rowscan + rowoff[k], (int) k, J.P.nstrips, S, tiles);
+  }
   /* ---- tables, strips, job descriptors ---- */
   size_t nstr = 0;
   for (auto &j : jobs) nstr += j.P.strips.size();
@@ -1669,11 +1703,8 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   a.off_chunk = L.off_chunk; a.off_queue = L.off_queue;
   a.census = c->census;
   a.G = G;
-  int occ = 0;
-  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *) fn, FK_NT, L.total));
-  if (occ < 1) occ = 1;
   if (c->tile_cap > 0 && c->tile_cap < ntiles) a.ntiles = c->tile_cap;
-  int nblock = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
+  int nblock = nwg;
   if (nblock > a.ntiles) nblock = std::max(a.ntiles, 1);
   c->last_ntiles = ntiles;
   c->stats.tile = jobs.empty() ? 0 : jobs[0].P.T;
@@ -1986,7 +2017,8 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
   const int save_cd = c->t_cell_div, save_zd = c->t_zdiv, save_t = c->t_tile;
   std::vector<unsigned char> scratch(8 * (size_t) bins->ntot);
   double bestms = 1e300;
-  int bcd = 4, bzd = 4, bT = 0;
+  int bcd = 3, bzd = 8, bT = 0;
+  int last_T = 0;
   auto measure = [&](int cd, int zd, int T, double *est) -> int {
     c->t_cell_div = cd; c->t_zdiv = zd; c->t_tile = T;
     /* first pass: learn the number of tiles of this shape with a tiny cap */
@@ -1999,20 +2031,21 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     e = fcfc_gpu_count(c, a, b, bins, bsize, flags & ~FCFC_GPU_CNT_DEVICE, scratch.data());
     if (e) return e;
     *est = c->stats.kernel_ms * (double) nt / (double) c->tile_cap;
+    last_T = c->stats.tile;
     return 0;
   };
   int e = 0;
-  const int cds[] = {3, 4, 6, 8};
+  const int cds[] = {2, 3, 4, 6};
   for (int cd : cds) {
     double est;
-    if ((e = measure(cd, 4, 0, &est))) break;
-    if (est < bestms) { bestms = est; bcd = cd; }
+    if ((e = measure(cd, 8, 0, &est))) break;
+    if (est < bestms) { bestms = est; bcd = cd; bT = last_T; }
   }
-  const int zds[] = {2, 4, 8};
+  const int zds[] = {4, 8, 16};
   for (int zd : zds) {
     double est;
     if (e || (e = measure(bcd, zd, 0, &est))) break;
-    if (est < bestms) { bestms = est; bzd = zd; }
+    if (est < bestms) { bestms = est; bzd = zd; bT = last_T; }
   }
   const int Ts[] = {FK_NT, 2 * FK_NT};
   for (int T : Ts) {
@@ -2033,13 +2066,15 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
       bbox(c, a, lo, hi);
       if (!autoc) bbox(c, b, lo, hi);
     }
-    make_plan(c, bins, bsize, lo, hi, autoc, mode, a->n, autoc ? a->n : b->n, &P);
+    const int wt = (flags & FCFC_GPU_WEIGHTED) ? 1 : 0;
+    make_plan(c, bins, bsize, lo, hi, autoc, wt, mode, a->n, autoc ? a->n : b->n, &P);
     bool found = false;
     for (auto &t : c->tuned)
-      if (t.geom == bins->geometry && t.bintype == bins->bintype && t.mode == mode && t.bucket == P.bucket) {
+      if (t.geom == bins->geometry && t.bintype == bins->bintype && t.mode == mode &&
+          t.autoc == autoc && t.wt == wt && t.bucket == P.bucket) {
         t.cd = bcd; t.zd = bzd; t.T = bT; found = true;
       }
-    if (!found) c->tuned.push_back({bins->geometry, bins->bintype, mode, (flags & FCFC_GPU_WEIGHTED) ? 1 : 0,
+    if (!found) c->tuned.push_back({bins->geometry, bins->bintype, mode, autoc, wt,
                                     P.bucket, bcd, bzd, bT});
   }
   if (best) { best[0] = bcd; best[1] = bzd; best[2] = bT; best[3] = (int) lround(bestms * 1000); }

@@ -77,7 +77,12 @@ struct FkCat {
   const int *cell_start;
 };
 
-struct FkTile { int row, p0, p1, iz0, iz1, job; };
+/* A work item: the A objects [p0, p1) of one row (z-cells iz0..iz1) against
+ * the strips [s0, s1) of its job; a tile with many strips is split into
+ * several items so that the queue stays long (items of one tile add into
+ * the same histogram; only the item with s0 == 0 counts the pairs inside
+ * the tile). */
+struct FkTile { int row, p0, p1, iz0, iz1, job, s0, s1; };
 
 /* Floor of the exact quotient a / b for a >= 0, b > 0 (b not tiny):
  * approximate quotient from the hardware reciprocal, then one-step

@@ -131,14 +131,81 @@ def _make_fn():
     return _Fn
 
 
-def xi_multipoles_torch(wd, wr, eng, D, R, bins, rr_cnt=None, poles=(0, 2)):
+class WeightFitCounts:
+    """Differentiable DD and DR of fixed data and random positions for
+    changing data weights, for fitting loops: the counts come from a
+    prepared plan (catalogues sorted and tiled once, each forward a HIP graph
+    replay that re-reads the weights), the gradients from the GPU backward
+    kernel.
+
+        fit = WeightFitCounts(eng, D, R, bins)      # D, R: device Catalogs
+        dd, dr = fit(wd)                            # wd: (D.n,) tensor
+    R keeps the weights it was created with (none: unit weights).
+    """
+
+    def __init__(self, eng, D, R, bins, precision="fp64"):
+        torch = _torch()
+        self.eng, self.bins = eng, bins
+        self._wd = torch.ones(D.n, dtype=torch.float64, device=D.x.device)
+        wr = R.w if R.w is not None else torch.ones(R.n, dtype=torch.float64, device=R.x.device)
+        self.D = Catalog(bins, D.x, D.y, D.z, self._wd, D.s)
+        self.R = Catalog(bins, R.x, R.y, R.z, wr.contiguous(), R.s)
+        self.plan = eng.plan([(self.D, None, True, True), (self.D, self.R, False, True)],
+                             bins, precision)
+        self._fn = _make_fit_fn()
+
+    def __call__(self, wd):
+        return self._fn.apply(wd, self)
+
+    def close(self):
+        self.plan.close()
+
+
+def _make_fit_fn():
+    torch = _torch()
+
+    class _Fit(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, wd, fit):
+            fit._wd.copy_(wd.detach().to(torch.float64))
+            dd, dr = fit.plan.run()
+            ctx.fit = fit
+            ctx.wdtype = wd.dtype
+            dev = wd.device
+            return torch.from_numpy(dd).to(dev), torch.from_numpy(dr).to(dev)
+
+        @staticmethod
+        def backward(ctx, gdd, gdr):
+            fit = ctx.fit
+            g = np.zeros(fit.D.n)
+            if gdd is not None:
+                ga, _ = fit.eng.count_backward(fit.D, None, fit.bins,
+                                               gdd.detach().to(torch.float64).cpu().numpy(),
+                                               isauto=True)
+                g += ga
+            if gdr is not None:
+                ga, _ = fit.eng.count_backward(fit.D, fit.R, fit.bins,
+                                               gdr.detach().to(torch.float64).cpu().numpy(),
+                                               isauto=False)
+                g += ga
+            return torch.from_numpy(g).to(fit._wd.device).to(ctx.wdtype), None
+
+    return _Fit
+
+
+def xi_multipoles_torch(wd, wr, eng, D, R, bins, rr_cnt=None, poles=(0, 2), fit=None):
     """Differentiable Landy-Szalay xi_ell(s) from (s,mu) counts: DD and DR
-    through WeightedPairCount, RR (independent of wd) counted once and
-    passed in as rr_cnt (unordered, FP64) when available."""
+    through WeightedPairCount (or through `fit`, a WeightFitCounts of the
+    same catalogues, in fitting loops), RR (independent of wd) counted once
+    and passed in as rr_cnt (unordered, FP64) when available."""
     torch = _torch()
     ns, nmu = bins.ns, bins.nmu
-    dd = 2 * WeightedPairCount.apply(wd, None, eng, D, None, bins, True)
-    dr = WeightedPairCount.apply(wd, wr, eng, D, R, bins, False)
+    if fit is not None:
+        dd, dr = fit(wd)
+        dd = 2 * dd
+    else:
+        dd = 2 * WeightedPairCount.apply(wd, None, eng, D, None, bins, True)
+        dr = WeightedPairCount.apply(wd, wr, eng, D, R, bins, False)
     if rr_cnt is None:
         rr_cnt = WeightedPairCount.apply(wr, None, eng, R, None, bins, True).detach()
     rr = 2 * rr_cnt

@@ -93,3 +93,36 @@ def test_autograd_xi_multipoles():
         cnt = WeightedPairCount.apply(w32, None, eng, D, None, b, True)
         cnt.sum().backward()
         assert w32.grad.dtype == torch.float32 and torch.isfinite(w32.grad).all()
+
+
+def test_weight_fit_plan_matches_eager():
+    """The plan-based fitting counts (HIP graph replays) give the same loss
+    and gradient as the eager autograd path, over several weight updates."""
+    import torch
+    from fcfc_amd.xi import WeightFitCounts, WeightedPairCount, xi_multipoles_torch
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=10, bsize=L)
+    x, y, z, w = F.gen_clustered(51, 500, 8, 25, 2.0, L, 0.3)
+    xr, yr, zr = F.gen_uniform(52, 1200, L)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a * b.rescale)).cuda()
+    D = F.Catalog(b, dev(x), dev(y), dev(z))
+    R = F.Catalog(b, dev(xr), dev(yr), dev(zr))
+    wr = torch.ones(len(xr), dtype=torch.float64, device="cuda")
+    with F.Engine() as eng:
+        rr = WeightedPairCount.apply(wr, None, eng, R, None, b, True).detach()
+        fit = WeightFitCounts(eng, D, R, b)
+        try:
+            wd = torch.tensor(w, dtype=torch.float64, device="cuda")
+            for step in range(3):
+                print(f"case weight fit step={step}", flush=True)
+                w1 = wd.clone().requires_grad_(True)
+                l1 = (xi_multipoles_torch(w1, wr, eng, D, R, b, rr_cnt=rr, fit=fit) ** 2).sum()
+                l1.backward()
+                w2 = wd.clone().requires_grad_(True)
+                l2 = (xi_multipoles_torch(w2, wr, eng, D, R, b, rr_cnt=rr) ** 2).sum()
+                l2.backward()
+                assert l1.item() == pytest.approx(l2.item(), rel=1e-12)
+                np.testing.assert_allclose(w1.grad.cpu().numpy(), w2.grad.cpu().numpy(),
+                                           rtol=1e-9, atol=1e-12 * float(w2.grad.abs().max()))
+                wd = wd * (1.0 + 0.05 * torch.sin(torch.arange(len(w), device="cuda") * 0.1))
+        finally:
+            fit.close()

@@ -1,11 +1,12 @@
 """GPU parity: the HIP kernels against the CPU oracle on seeded inputs.
 
-Cross counts evaluate exactly one rounding path of the reference (the
-vectorised-B block, src/fcfc/2pt_box/metric_common.c:1154-1243), so they must
-be bit-identical to oracle variant 1.  Auto counts are compared against the
-oracle's canonical variant together with its ambiguity census (the number of
-pairs whose bin depends on the reference's rounding path): with census 0
-they must be identical as well.  Weighted counts: <= 1e-12 relative.
+Every pair is evaluated on one rounding path of the reference (the
+vectorised-B block, src/fcfc/2pt_box/metric_common.c:1154-1243, with the
+larger catalogue of a cross count as "a" -- the other orientation is the
+reference's a-side block).  Counts are compared against the oracle's
+variant 1 together with its ambiguity census (the number of pairs whose bin
+depends on the reference's rounding path): with census 0 they must be
+identical.  Weighted counts: <= 1e-12 relative.
 """
 import numpy as np
 import pytest
@@ -71,9 +72,13 @@ def test_box_cross(eng, cats, bt, prec):
                      _obins(b), variant=1, weighted=True)
     _cmp(g, ref, True)
     gu = eng.count_pairs(d, r, b, isauto=False, withwt=False, precision=prec)
-    refu, _ = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None),
-                      (r.x, r.y, r.z, None, None), False, _obins(b), variant=1)
-    _cmp(gu, refu, False)
+    refu, cen = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None),
+                        (r.x, r.y, r.z, None, None), False, _obins(b), variant=1,
+                        census_mask=0b11111)
+    _cmp(gu, refu, False, cen)
+    # cross counts do not depend on the order of the two catalogues
+    np.testing.assert_array_equal(eng.count_pairs(r, d, b, isauto=False, withwt=False,
+                                                  precision=prec), gu)
 
 
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
