@@ -124,21 +124,23 @@ def traffic_from_profile():
 
 def mocks_workload(args):
     """256 periodic-box mocks (L = 1000): sizes log-uniform in [5e4, 5e5] from
-    the seed, each half uniform / half in halos (sigma = 2) with lognormal
-    weights, 5x uniform randoms; DD/DR/RR in (s,mu) 40x100 and xi_0,2,4.
-    All 768 counts in one persistent launch vs one launch per count."""
+    the seed (times --mock-scale), each half uniform / half in halos
+    (sigma = 2) with lognormal weights, 5x uniform randoms; DD/DR/RR in
+    (s,mu) 40x100 and xi_0,2,4.  All 768 counts in one batched call (one
+    persistent launch per kernel variant) vs one call per count; shapes from
+    the autotuner for both (run on a few mocks across the size range)."""
     import torch
     import fcfc_amd as F
     from fcfc_amd.xi import multipoles
     L = 1000.0
     nm = max(1, int(round(256 * args.scale)))
-    sizes = F.gen_sizes(4040, nm, 5e4, 5e5)
+    sizes = F.gen_sizes(4040, nm, 5e4 * args.mock_scale, 5e5 * args.mock_scale)
     b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=100, bsize=L)
     mocks = []
     for k, n in enumerate(sizes):
         n = int(n)
         nh = max(1, n // 1000)
-        x, y, z, w = F.gen_clustered(9000 + k, n - nh * 500, nh, 500, 2.0, L, 0.3)
+        x, y, z, w = F.gen_clustered(9000 + k, max(n - nh * 500, 1), nh, 500, 2.0, L, 0.3)
         xr, yr, zr = F.gen_uniform(19000 + k, 5 * n, L)
         mocks.append((to_device(b, (x, y, z, w), F, torch), to_device(b, (xr, yr, zr), F, torch)))
     jobs = []
@@ -147,6 +149,13 @@ def mocks_workload(args):
     torch.cuda.synchronize()
     log(f"{nm} mocks ready, {sum(int(v) for v in sizes)} data objects")
     eng = F.Engine(0)
+    prec = args.precision
+    if not args.no_autotune:
+        order = np.argsort(sizes)
+        for k in sorted(set(int(order[int(q * (nm - 1))]) for q in (0.0, 0.33, 0.67, 1.0))):
+            for a, c, au, wt in jobs[3 * k:3 * k + 3]:
+                eng.autotune(a, c, b, isauto=au, withwt=wt, precision=prec)
+        log("autotuned on mocks of four sizes")
 
     def xi_all(res):
         out = []
@@ -160,19 +169,19 @@ def mocks_workload(args):
         return out
 
     for i in range(args.warmup):
-        eng.count_batch(jobs, b, precision="fp32")
+        eng.count_batch(jobs, b, precision=prec)
     torch.cuda.synchronize()
     t0 = time.time()
     for i in range(args.steps):
-        res = eng.count_batch(jobs, b, precision="fp32")
+        res = eng.count_batch(jobs, b, precision=prec)
         xi_b = xi_all(res)
         log(f"batched step {i} done")
     torch.cuda.synchronize()
     tb = (time.time() - t0) / args.steps
     stb = eng.stats()
-    # the same counts, one launch per catalogue pair
+    # the same counts, one call per catalogue pair
     t1 = time.time()
-    res1 = [eng.count_pairs(a, c, b, isauto=au, withwt=wt, precision="fp32") for a, c, au, wt in jobs]
+    res1 = [eng.count_pairs(a, c, b, isauto=au, withwt=wt, precision=prec) for a, c, au, wt in jobs]
     xi_s = xi_all(res1)
     torch.cuda.synchronize()
     ts = time.time() - t1
@@ -183,10 +192,11 @@ def mocks_workload(args):
         "metric": METRIC, "value": nm / tb, "unit": "mocks/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": tb * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{nm} box mocks L=1000, N log-uniform in [5e4,5e5], 5x randoms, "
-                               "DD+DR+RR (s,mu) 40x100 -> xi_0,2,4, one persistent launch",
-                   "n_data_total": int(sum(int(v) for v in sizes))},
+        "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": f"{nm} box mocks L=1000, N log-uniform in "
+                               f"[{5e4 * args.mock_scale:.3g},{5e5 * args.mock_scale:.3g}], 5x randoms, "
+                               "DD+DR+RR (s,mu) 40x100 -> xi_0,2,4, batched",
+                   "n_data_total": int(sum(int(v) for v in sizes)), "precision": prec},
         "per_catalogue_mocks_per_s": nm / ts, "speedup_vs_per_catalogue": ts / tb,
         "evals_per_s": stb["evals"] / tb, "kernel_ms": stb["kernel_ms"],
         "counts_identical_batched_vs_single": {"RR": exact, "weighted_max_rel": wrel},
@@ -276,6 +286,8 @@ def main():
     ap.add_argument("--skip", nargs="*", default=[], choices=["fp64", "fp32", "plain"],
                     help="forms not to run for comparison")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mock-scale", type=float, default=1.0,
+                    help="mocks workload: factor on the catalogue sizes")
     ap.add_argument("--workload", choices=["c2", "mocks", "survey"], default="c2",
                     help="c2: the headline clustered box DD/DR/RR; mocks: 256 batched mocks; "
                          "survey: the survey-shell (s_perp, pi) counts")

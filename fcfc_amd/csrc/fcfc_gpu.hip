@@ -1735,6 +1735,22 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   return 0;
 }
 
+/* Jobs of a batch grouped for their launches: one kernel variant per
+ * launch, so weighted and unweighted jobs, and jobs whose tiles want one or
+ * two objects per lane, go to separate persistent launches. */
+static std::vector<std::vector<PJob>> launch_groups(const std::vector<PJob> &jobs) {
+  std::vector<std::vector<PJob>> g;
+  std::vector<int> key;
+  for (const PJob &j : jobs) {
+    const int k = 2 * (j.wt ? 1 : 0) + ((j.P.T > FK_NT) ? 1 : 0);
+    size_t q = 0;
+    while (q < key.size() && key[q] != k) q++;
+    if (q == key.size()) { key.push_back(k); g.emplace_back(); }
+    g[q].push_back(j);
+  }
+  return g;
+}
+
 static int stats_from(fcfc_gpu_ctx *c, unsigned long long *dstats) {
   unsigned long long st[3] = {0, 0, 0};
   HIPCHK(c, hipMemcpyAsync(st, dstats, sizeof st, hipMemcpyDeviceToHost, c->stream));
@@ -1803,10 +1819,22 @@ static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
   hipLaunchKernelGGL(k_zero_u64, dim3(nblk((int64_t) bins->ntot * njob, 256)), dim3(256), 0, c->stream,
       (unsigned long long *) dh, bins->ntot * njob);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
+  /* one persistent launch per group (each with its own queue word) */
+  std::vector<std::vector<PJob>> groups = launch_groups(jobs);
+  std::vector<Launch> L(groups.size());
+  for (size_t q = 0; q < groups.size(); q++) {
+    int e = build_launch(c, al, bins, groups[q], mode, nullptr, dh, misc + 8,
+                         (unsigned int *) misc + q, &L[q]);
+    if (e) return e;
+  }
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (const Launch &Lq : L) { int e = launch(c, Lq); if (e) return e; }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0;
-  int e = run_jobs(c, al, bins, jobs, mode, nullptr, dh, misc + 8, (unsigned int *) misc, &ms);
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  int e = deliver(c, jobs, bins->ntot, dh, flags, cnt);
   if (e) return e;
-  if ((e = deliver(c, jobs, bins->ntot, dh, flags, cnt))) return e;
   if ((e = stats_from(c, misc + 8))) return e;
   c->stats.kernel_ms = ms;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1911,7 +1939,7 @@ struct fcfc_gpu_plan {
   std::vector<Gather> gathers;
   int ntot = 0;
   uint32_t flags = 0;
-  Launch L;
+  std::vector<Launch> L;             /* one per launch group */
   void *dh = nullptr;                /* njob x ntot histograms (8 bytes per bin) */
   unsigned long long *misc = nullptr;
   hipGraph_t graph = nullptr;
@@ -1954,9 +1982,12 @@ extern "C" int fcfc_gpu_plan_create(fcfc_gpu_ctx *c, int njob, const fcfc_batch_
   P->dh = P->al.get(WS_HIST, 8 * (size_t) bins->ntot * njob);
   P->misc = (unsigned long long *) P->al.get(WS_MISC, 64 * 8);
   if (P->al.err) return P->al.err;
-  int e = build_launch(c, P->al, bins, P->jobs, mode, nullptr, P->dh, P->misc + 8,
-                       (unsigned int *) P->misc, &P->L);
-  if (e) return e;
+  std::vector<std::vector<PJob>> groups = launch_groups(P->jobs);
+  P->L.resize(groups.size());
+  int e = 0;
+  for (size_t q = 0; q < groups.size(); q++)
+    if ((e = build_launch(c, P->al, bins, groups[q], mode, nullptr, P->dh, P->misc + 8,
+                          (unsigned int *) P->misc + q, &P->L[q]))) return e;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   for (const Gather &g : P->gathers)
@@ -1964,7 +1995,7 @@ extern "C" int fcfc_gpu_plan_create(fcfc_gpu_ctx *c, int njob, const fcfc_batch_
   hipLaunchKernelGGL(k_zero_u64, dim3(nblk((int64_t) bins->ntot * njob, 256)), dim3(256), 0, c->stream,
       (unsigned long long *) P->dh, bins->ntot * njob);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, P->misc, 64);
-  e = launch(c, P->L);
+  for (const Launch &Lq : P->L) if ((e = launch(c, Lq))) break;
   hipGraph_t graph = nullptr;
   const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
   if (e) { if (graph) (void) hipGraphDestroy(graph); return e; }
