@@ -495,9 +495,9 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
   float s2;
   if (BIN == FCFC_BIN_SPI) s2 = __builtin_fmaf(dy, dy, dx * dx);
   else s2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, dz * dz));
-  const float fl = __builtin_floorf(s2);
-  const unsigned e = stab[min((unsigned) ((int) fl - bn.soff_m1), (unsigned) (bn.stab_len + 2))];
-  const float f = s2 - fl;
+  /* s2 >= 0: truncation is the floor */
+  const unsigned e = stab[min((unsigned) ((int) s2 - bn.soff_m1), (unsigned) (bn.stab_len + 2))];
+  const float f = __builtin_amdgcn_fractf(s2);
   const unsigned msk = FK_F_ALW | (f < wn.e_s2 ? FK_F_LO : 0u) | (f > 1.0f - wn.e_s2 ? FK_F_HI : 0u);
   const bool amb = (e & msk) != 0;
   const unsigned sb = e & 0xFFFFu;
@@ -514,10 +514,10 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     return ((amb || ambp) && !out) ? FK_AMB : sb + __umul24((unsigned) bn.ns, pb);
   }
   /* mu bin = floor(t), t = nmu |dz| / s; above s2_floor the FP32 error of t
-   * is below 1/2 - mu_w (fp32_windows) and t <= nmu (1 + 2^-20) */
+   * is below 1/2 - mu_w (fp32_windows) and t <= nmu (1 + 2^-20); the cells
+   * below s2_floor carry FK_F_ALW (every pair there is re-evaluated) */
   const float t = (wn.nmu_f * __builtin_fabsf(dz)) * __builtin_amdgcn_rsqf(s2);
-  const bool nearm = !(__builtin_fabsf(__builtin_amdgcn_fractf(t) - 0.5f) <= wn.mu_w) ||
-                     s2 < wn.s2_floor;
+  const bool nearm = !(__builtin_fabsf(__builtin_amdgcn_fractf(t) - 0.5f) <= wn.mu_w);
   const bool ambt = amb || (nearm && sb != FK_OUT);
   return ambt ? FK_AMB : sb + __umul24((unsigned) bn.ns, (unsigned) (int) t);
 }
@@ -565,7 +565,7 @@ __device__ __forceinline__ unsigned fk_recheck(const FkBins &bn, const FkJob &J,
  * weight gradient.  Template parameters: binning, geometry, weighted
  * histogram support, mode, hybrid tables, A objects per lane. */
 template <int BIN, int GEOM, bool WT, int MODE, bool HYB, int R>
-__global__ void __launch_bounds__(FK_NT)
+__global__ void __launch_bounds__(FK_NT, (MODE == MODE_C32 || MODE == MODE_C32C) ? 6 : 1)
 k_count(const FkArgs a) {
   extern __shared__ __attribute__((aligned(16))) char lds[];
   constexpr bool F32 = (MODE == MODE_C32 || MODE == MODE_C32C);
@@ -618,18 +618,22 @@ k_count(const FkArgs a) {
     auto cell = [](const uint16_t *t, int len, int c) -> unsigned {
       return (c >= 0 && c < len) ? (unsigned) t[c] : FK_OUT;
     };
-    auto entry = [&](const uint16_t *t, int len, int nb, int c) -> unsigned {
+    /* alw_below: cells whose upper end lies below it are always re-evaluated */
+    auto entry = [&](const uint16_t *t, int len, int nb, int c, double off, double alw_below) -> unsigned {
       const unsigned v = cell(t, len, c);
       if (!F32) return v;
       unsigned f = 0;
       if (cell(t, len, c - 1) != v) f |= FK_F_LO;
       if (cell(t, len, c + 1) != v) f |= FK_F_HI;
-      if (v >= (unsigned) nb && v != FK_OUT) f |= FK_F_ALW;
+      if (v != FK_OUT && (v >= (unsigned) nb || off + c < alw_below)) f |= FK_F_ALW | FK_F_LO | FK_F_HI;
       return v | f;
     };
-    for (int i = tid; i <= bn.stab_len + 2; i += FK_NT) stab_raw[i] = entry(bn.stab, bn.stab_len, bn.ns, i - 1);
+    const double s2f = (F32 && BIN == FCFC_BIN_SMU) ? (double) a.jobs[0].win.s2_floor : 0.0;
+    for (int i = tid; i <= bn.stab_len + 2; i += FK_NT)
+      stab_raw[i] = entry(bn.stab, bn.stab_len, bn.ns, i - 1, bn.soff, s2f);
     if (BIN == FCFC_BIN_SPI)
-      for (int i = tid; i <= bn.ptab_len + 2; i += FK_NT) ptab_raw[i] = entry(bn.ptab, bn.ptab_len, bn.np, i - 1);
+      for (int i = tid; i <= bn.ptab_len + 2; i += FK_NT)
+        ptab_raw[i] = entry(bn.ptab, bn.ptab_len, bn.np, i - 1, bn.poff, 0.0);
   }
   if (HYB) {
     for (int i = tid; i <= bn.ns; i += FK_NT) s2e[i] = bn.s2bin[i];
@@ -1444,21 +1448,31 @@ static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int R
  * constant D, and the bin is certain when |fract(t32) - 1/2| <= 1/2 - D.
  * All bounds are inflated by 25%; pairs below s2_floor (about 1/512 of the
  * in-range volume) are always re-evaluated. */
-static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc,
+static double fp32_extent(const FkGrid &g, int zmaxc) {
+  return std::max(g.wx, std::max(g.wy, (zmaxc + 1) * g.wz));
+}
+
+/* s^2 below which a job's pairs are always re-evaluated (the LDS table flags
+ * those cells once per launch, so a launch uses the largest over its jobs) */
+static double fp32_s2_floor(const fcfc_bins *b, const FkGrid &g, int zmaxc) {
+  const double u = ldexp(1.0, -24);
+  return std::max(b->s2bin[b->ns] / 64.0, pow(1e4 * u * fp32_extent(g, zmaxc), 2));
+}
+
+static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double s2f,
     FkWin *win) {
   const double u = ldexp(1.0, -24);
   const double s2max = b->s2bin[b->ns];
   const double pmx = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
   const double S2 = s2max + pmx * pmx + 2.0;
   const double s = sqrt(S2);
-  const double E = std::max(g.wx, std::max(g.wy, (zmaxc + 1) * g.wz));
+  const double E = fp32_extent(g, zmaxc);
   const double tiny = ldexp(1.0, -44) * std::max(g.Lx, std::max(g.Ly, g.Lz)) * 4;
   const double ek = u * (2 * s + E) + tiny;
   const double es2 = 7 * u * S2 + 2 * sqrt(3.0) * u * E * s + 3 * ek * ek + 6 * ek * tiny;
   FkWin &w = *win;
   w.e_s2 = (float) (es2 * 1.25 + 1e-7);
   w.e_pi = (float) ((u * (2 * pmx + E) + tiny) * 1.25 + 1e-7);
-  const double s2f = std::max(s2max / 64.0, pow(1e4 * u * E, 2));
   const double nmu = b->nmu;
   const double D = 1.25 * (nmu * 10 * u + (sqrt(3.0) * u * E * nmu + nmu * (u * E + tiny)) / sqrt(s2f))
                    + 4 * nmu * u;
@@ -1647,6 +1661,9 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   std::vector<int> hstr;
   hstr.reserve(nstr);
   std::vector<FkJob> hj(jobs.size());
+  double s2f = 0;
+  if (m == MODE_C32)
+    for (const PJob &J : jobs) s2f = std::max(s2f, fp32_s2_floor(b, J.P.g, J.P.zmaxc));
   for (size_t k = 0; k < jobs.size(); k++) {
     const PJob &J = jobs[k];
     FkJob &f = hj[k];
@@ -1661,7 +1678,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     f.swap_all = J.swap_all;
     f.hist_off = J.hist_off;
     f.gA = J.gA;
-    if (m == MODE_C32) fp32_windows(b, J.P.g, J.P.zmaxc, &f.win);
+    if (m == MODE_C32) fp32_windows(b, J.P.g, J.P.zmaxc, s2f, &f.win);
     hstr.insert(hstr.end(), J.P.strips.begin(), J.P.strips.end());
   }
   djobs = (FkJob *) al.get(WS_JOBS, sizeof(FkJob) * jobs.size());
