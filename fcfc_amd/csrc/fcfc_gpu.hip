@@ -565,7 +565,7 @@ __device__ __forceinline__ unsigned fk_recheck(const FkBins &bn, const FkJob &J,
  * weight gradient.  Template parameters: binning, geometry, weighted
  * histogram support, mode, hybrid tables, A objects per lane. */
 template <int BIN, int GEOM, bool WT, int MODE, bool HYB, int R>
-__global__ void __launch_bounds__(FK_NT, (MODE == MODE_C32 || MODE == MODE_C32C) ? 6 : 1)
+__global__ void __launch_bounds__(FK_NT, (MODE == MODE_GRAD) ? 1 : ((WT || MODE == MODE_C32 || MODE == MODE_C32C) ? 6 : 8))
 k_count(const FkArgs a) {
   extern __shared__ __attribute__((aligned(16))) char lds[];
   constexpr bool F32 = (MODE == MODE_C32 || MODE == MODE_C32C);
