@@ -286,7 +286,6 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
  * its own: an out-of-range s (or pi) gives sb >= ntot, a dropped mu gives
  * mb = nmu and sb + ns * nmu >= ntot. */
 #define FK_NONE 0xFFFFFFFFu
-#define FK_AMB  0xFFFFFFFEu   /* FP32 path: within a rounding bound of an edge */
 #define FK_DBL_EPS 2.220446049250313e-16
 #define FK_FAR 1.0e30          /* coordinates of idle lanes: every pair out */
 
@@ -483,15 +482,15 @@ __device__ unsigned fk_box_variant(int var, const double a[3], const double bo[3
 
 /* ---------------- FP32 pair evaluation with exactness windows ----------- */
 /* Coordinates relative to the A tile centre in FP32.  Returns the histogram
- * index (>= ntot: not counted) or FK_AMB when the pair lies within the
+ * index (>= ntot: not counted) and sets ambig when the pair lies within the
  * rounding bound of a bin edge that matters (it is then re-evaluated in
- * FP64).  The s^2 cell flags say which ends of the integer cell are edges,
+ * FP64 and the index is ignored).  The s^2 cell flags say which ends of the integer cell are edges,
  * so the window test is branch-free; a pair whose s (or pi) is certainly
  * out of range is never AMB. */
 template <int BIN>
 __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     const FkBins &bn, const FkWin &wn, const unsigned *stab,
-    const unsigned *ptab) {
+    const unsigned *ptab, bool &ambig) {
   float s2;
   if (BIN == FCFC_BIN_SPI) s2 = __builtin_fmaf(dy, dy, dx * dx);
   else s2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, dz * dz));
@@ -501,7 +500,7 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
   const unsigned msk = FK_F_ALW | (f < wn.e_s2 ? FK_F_LO : 0u) | (f > 1.0f - wn.e_s2 ? FK_F_HI : 0u);
   const bool amb = (e & msk) != 0;
   const unsigned sb = e & 0xFFFFu;
-  if (BIN == FCFC_BIN_ISO) return amb ? FK_AMB : sb;
+  if (BIN == FCFC_BIN_ISO) { ambig = amb; return sb; }
   if (BIN == FCFC_BIN_SPI) {
     const float pi = __builtin_fabsf(dz);
     const float fp = __builtin_floorf(pi);
@@ -510,16 +509,17 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     const unsigned mp = FK_F_ALW | (g < wn.e_pi ? FK_F_LO : 0u) | (g > 1.0f - wn.e_pi ? FK_F_HI : 0u);
     const bool ambp = (ep & mp) != 0;
     const unsigned pb = ep & 0xFFFFu;
-    const bool out = (sb == FK_OUT && !amb) || (pb == FK_OUT && !ambp);
-    return ((amb || ambp) && !out) ? FK_AMB : sb + __umul24((unsigned) bn.ns, pb);
+    const bool out = ((sb == FK_OUT) & !amb) | ((pb == FK_OUT) & !ambp);
+    ambig = (amb | ambp) & !out;
+    return sb + __umul24((unsigned) bn.ns, pb);
   }
   /* mu bin = floor(t), t = nmu |dz| / s; above s2_floor the FP32 error of t
    * is below 1/2 - mu_w (fp32_windows) and t <= nmu (1 + 2^-20); the cells
    * below s2_floor carry FK_F_ALW (every pair there is re-evaluated) */
   const float t = (wn.nmu_f * __builtin_fabsf(dz)) * __builtin_amdgcn_rsqf(s2);
   const bool nearm = !(__builtin_fabsf(__builtin_amdgcn_fractf(t) - 0.5f) <= wn.mu_w);
-  const bool ambt = amb || (nearm && sb != FK_OUT);
-  return ambt ? FK_AMB : sb + __umul24((unsigned) bn.ns, (unsigned) (int) t);
+  ambig = amb | (nearm & (sb != FK_OUT));      /* no short-circuit branches */
+  return sb + __umul24((unsigned) bn.ns, (unsigned) (int) t);
 }
 
 /* Exact FP64 re-evaluation of a pair flagged by the FP32 windows (rare: the
@@ -763,11 +763,11 @@ k_count(const FkArgs a) {
         for (int r = 0; r < R; r++) {
           unsigned idx;
           if (F32) {
-            idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw, ptab_raw);
-            if (SELF && !(jj > ai[r])) idx = FK_NONE;
+            bool amb;
+            idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw, ptab_raw, amb);
+            if (SELF && !(jj > ai[r])) { idx = FK_NONE; amb = false; }
             /* pairs near an edge: queue (A slot, chunk entry) for FP64 */
-            const bool amb = (idx == FK_AMB);
-            const unsigned long long m = __ballot(amb);
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(amb);
             if (m) {
               if (amb) {
                 const int pos = qn + __popcll(m & lanemask_lt);
