@@ -274,6 +274,77 @@ def survey_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def grad_workload(args):
+    """Differentiable weights: box L = 1000, D = 1e6 clustered (C2 recipe at
+    half size), R = 5e6 uniform; float32 per-object weights with gradient;
+    weighted (s,mu) 40x100 counts -> xi_0,2 (Landy-Szalay); loss = sum of
+    squared differences to a seeded target.  One step = forward (DD and DR
+    recounted through a prepared plan, HIP graph replay) + backward (the
+    gradient kernel for DD and DR) + a weight update."""
+    import torch
+    import fcfc_amd as F
+    from fcfc_amd.xi import WeightFitCounts, WeightedPairCount, xi_multipoles_torch
+    L = 1000.0
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=100, bsize=L)
+    nh = max(1, int(round(1000 * args.scale)))
+    x, y, z, w = F.gen_clustered(20271, int(5e5 * args.scale), nh, 500, 2.0, L, 0.3)
+    xr, yr, zr = F.gen_uniform(20272, int(5e6 * args.scale), L)
+    D = to_device(b, (x, y, z), F, torch)
+    R = to_device(b, (xr, yr, zr), F, torch)
+    wr = torch.ones(R.n, dtype=torch.float64, device="cuda")
+    wd = torch.tensor(w, dtype=torch.float32, device="cuda", requires_grad=True)
+    eng = F.Engine(0)
+    t0 = time.time()
+    rr = WeightedPairCount.apply(wr, None, eng, R, None, b, True).detach()
+    log(f"RR once: {time.time() - t0:.2f} s")
+    fit = WeightFitCounts(eng, D, R, b)
+    rng = np.random.default_rng(20273)
+    target = torch.tensor(rng.normal(0, 0.01, (2, b.ns)), dtype=torch.float64, device="cuda")
+    opt = torch.optim.SGD([wd], lr=1e-3)
+
+    def step():
+        opt.zero_grad()
+        t1 = time.time()
+        xi = xi_multipoles_torch(wd, wr, eng, D, R, b, rr_cnt=rr, poles=(0, 2), fit=fit)
+        loss = ((xi - target) ** 2).sum()
+        torch.cuda.synchronize()
+        tf = time.time() - t1
+        ev_f = eng.stats()["evals"]
+        t2 = time.time()
+        loss.backward()
+        torch.cuda.synchronize()
+        tb = time.time() - t2
+        opt.step()
+        return float(loss.item()), tf, tb, ev_f
+
+    for i in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t3 = time.time()
+    tfs, tbs = 0.0, 0.0
+    for i in range(args.steps):
+        loss, tf, tb, _ = step()
+        tfs += tf
+        tbs += tb
+        log(f"fit step {i}: loss {loss:.6g} forward {tf:.2f} s backward {tb:.2f} s")
+    torch.cuda.synchronize()
+    dt = (time.time() - t3) / args.steps
+    line = {
+        "metric": METRIC, "value": 1.0 / dt, "unit": "fit steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "box L=1000; D=1e6 clustered, float32 weights with gradient; "
+                               "R=5e6; weighted (s,mu) 40x100 -> xi_0,2; loss to a seeded target; "
+                               "forward (plan replay) + backward (gradient kernel) + SGD step",
+                   "scale": args.scale, "n_data": D.n, "n_random": R.n},
+        "forward_ms": tfs / args.steps * 1e3, "backward_ms": tbs / args.steps * 1e3,
+    }
+    fit.close()
+    eng.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,9 +359,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mock-scale", type=float, default=1.0,
                     help="mocks workload: factor on the catalogue sizes")
-    ap.add_argument("--workload", choices=["c2", "mocks", "survey"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "mocks", "survey", "grad"], default="c2",
                     help="c2: the headline clustered box DD/DR/RR; mocks: 256 batched mocks; "
-                         "survey: the survey-shell (s_perp, pi) counts")
+                         "survey: the survey-shell (s_perp, pi) counts; grad: weight fitting "
+                         "with the backward kernel")
     ap.add_argument("--copy-calib", action="store_true",
                     help="start with a 1 GiB device-to-device copy (calibrates HBM byte "
                          "counters of a profiler run)")
@@ -322,6 +394,9 @@ def main():
         return
     if args.workload == "survey":
         survey_workload(args)
+        return
+    if args.workload == "grad":
+        grad_workload(args)
         return
 
     t0 = time.time()

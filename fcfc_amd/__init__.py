@@ -300,8 +300,9 @@ class Engine:
         self._check(e, "fcfc_gpu_count")
         return out
 
-    def count_backward(self, a, b, bins, grad_cnt, isauto=False):
-        """d(sum_k G_k cnt_k)/dw for the objects of a (and b for cross)."""
+    def count_backward(self, a, b, bins, grad_cnt, isauto=False, need_b=True):
+        """d(sum_k G_k cnt_k)/dw for the objects of a (and of b for cross
+        counts unless need_b is False, which skips the pass over b)."""
         g = _f64(grad_cnt)
         ga = np.zeros(a.n)
         gb = np.zeros((b if b is not None else a).n)
@@ -312,9 +313,9 @@ class Engine:
         e = lib().fcfc_gpu_count_backward(self._h, C.byref(ca), C.byref(cb), C.byref(bins._b),
                                           _dptr(bs) if bs is not None else None,
                                           L.FCFC_GPU_AUTO if isauto else 0, _dptr(g),
-                                          _dptr(ga), None if isauto else _dptr(gb))
+                                          _dptr(ga), None if (isauto or not need_b) else _dptr(gb))
         self._check(e, "fcfc_gpu_count_backward")
-        return (ga, None) if isauto else (ga, gb)
+        return (ga, None) if (isauto or not need_b) else (ga, gb)
 
     def plan(self, jobs, bins, precision="fp64"):
         """A prepared batch (see fcfc_gpu_plan_create): jobs as for

@@ -565,7 +565,7 @@ __device__ __forceinline__ unsigned fk_recheck(const FkBins &bn, const FkJob &J,
  * weight gradient.  Template parameters: binning, geometry, weighted
  * histogram support, mode, hybrid tables, A objects per lane. */
 template <int BIN, int GEOM, bool WT, int MODE, bool HYB, int R>
-__global__ void __launch_bounds__(FK_NT, (MODE == MODE_GRAD) ? 1 : ((WT || MODE == MODE_C32 || MODE == MODE_C32C) ? 6 : 8))
+__global__ void __launch_bounds__(FK_NT, (WT || MODE == MODE_C32 || MODE == MODE_C32C) ? 6 : 8)
 k_count(const FkArgs a) {
   extern __shared__ __attribute__((aligned(16))) char lds[];
   constexpr bool F32 = (MODE == MODE_C32 || MODE == MODE_C32C);
@@ -1166,9 +1166,8 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
   }
   /* objects per tile: two per lane when the rows are long enough to fill
    * such tiles, one per lane otherwise (and always for the gradient) */
-  P->T_auto = (mode != MODE_GRAD && tT <= 0);
-  if (mode == MODE_GRAD) P->T = FK_NT;
-  else if (tT > 0) P->T = std::min(tT, 2 * FK_NT);
+  P->T_auto = (tT <= 0);
+  if (tT > 0) P->T = std::min(tT, 2 * FK_NT);
   else {
     /* two objects per lane only when the z extent allowed to a tile holds
      * enough objects to fill such tiles (idle lanes cost full issue slots) */
@@ -1402,7 +1401,8 @@ typedef void (*kfn_t)(const FkArgs);
 template <int BIN, int GEOM, bool WT, int MODE>
 static kfn_t pick5(int hyb, int R) {
   if constexpr (MODE == MODE_GRAD) {
-    return hyb ? k_count<BIN, GEOM, WT, MODE, true, 1> : k_count<BIN, GEOM, WT, MODE, false, 1>;
+    if (hyb) return k_count<BIN, GEOM, WT, MODE, true, 1>;
+    return R == 1 ? k_count<BIN, GEOM, WT, MODE, false, 1> : k_count<BIN, GEOM, WT, MODE, false, 2>;
   }
   else {
     if (hyb) return k_count<BIN, GEOM, WT, MODE, true, 2>;
@@ -1412,7 +1412,7 @@ static kfn_t pick5(int hyb, int R) {
 
 template <int BIN, int GEOM>
 static kfn_t pick3(int wt, int mode, int hyb, int R) {
-  if (mode == MODE_GRAD) return pick5<BIN, GEOM, true, MODE_GRAD>(hyb, 1);
+  if (mode == MODE_GRAD) return pick5<BIN, GEOM, true, MODE_GRAD>(hyb, R);
   if constexpr (GEOM == FCFC_GEOM_BOX) {
     if (mode == MODE_C32)
       return wt ? pick5<BIN, GEOM, true, MODE_C32>(hyb, R) : pick5<BIN, GEOM, false, MODE_C32>(hyb, R);
@@ -1575,7 +1575,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   {
     int R = 1;
     for (auto &j : jobs) R = std::max(R, (j.P.T + FK_NT - 1) / FK_NT);
-    if (m == MODE_GRAD) R = 1;
+    if (m == MODE_GRAD && b->tabtype) R = 1;
     for (auto &j : jobs) { j.P.R = R; if (j.P.T_auto) j.P.T = R * FK_NT; }
   }
   /* ---- kernel and LDS plan ---- */

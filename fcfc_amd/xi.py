@@ -186,7 +186,7 @@ def _make_fit_fn():
             if gdr is not None:
                 ga, _ = fit.eng.count_backward(fit.D, fit.R, fit.bins,
                                                gdr.detach().to(torch.float64).cpu().numpy(),
-                                               isauto=False)
+                                               isauto=False, need_b=False)
                 g += ga
             return torch.from_numpy(g).to(fit._wd.device).to(ctx.wdtype), None
 
