@@ -373,7 +373,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this "
-              "build (the pair count would shard by cell-pair ranges; see DESIGN.md)",
+              "build (the pair count would shard by ranges of the tile list, each GPU "
+              "holding the sorted catalogues, with one histogram all-reduce at the end)",
               file=sys.stderr)
         sys.exit(2)
 
