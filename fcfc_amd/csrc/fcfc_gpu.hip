@@ -759,6 +759,34 @@ k_count(const FkArgs a) {
         const double wj = NEED_W ? cw[j] : 1.0;
         const int jj = (SELF || grad_self || F32) ? cj[j] : 0;
         const int code = NEED_C ? cc[j] : 13;
+        if (GEOM == FCFC_GEOM_BOX && !PLAIN && !(GRAD && (code & 32))) {
+          /* the wavefront's A objects are two runs of consecutive objects
+           * of one cell column: when this B object is out of range of all
+           * of them, skip the binning (same expressions as the pair
+           * functions, so the compiler shares them; gradient pairs taken in
+           * the other orientation are staged unshifted and are not tested;
+           * the plain form, like the reference's leaf loops, bins every
+           * candidate pair) */
+          bool any = false;
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            if (F32) {
+              const float ex = gx - fax[r], ey = gy - fay[r], ez = gz - faz[r];
+              if (BIN == FCFC_BIN_SPI)
+                any |= (__builtin_fmaf(ey, ey, ex * ex) < bn.s2skip32) &
+                       (__builtin_fabsf(ez) < bn.pskip32);
+              else any |= __builtin_fmaf(ey, ey, __builtin_fmaf(ex, ex, ez * ez)) < bn.s2skip32;
+            }
+            else {
+              const double ex = bx - ax[r], ey = by - ay[r], ez = bz - az[r];
+              if (BIN == FCFC_BIN_SPI)
+                any |= (__builtin_fma(ey, ey, ex * ex) < bn.s2skip) &
+                       (__builtin_fabs(ez) < bn.pskip);
+              else any |= __builtin_fma(ey, ey, __builtin_fma(ex, ex, ez * ez)) < bn.s2skip;
+            }
+          }
+          if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
+        }
 #pragma unroll
         for (int r = 0; r < R; r++) {
           unsigned idx;
@@ -1708,6 +1736,12 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   fb.nmu2 = (double) b->nmu * b->nmu;
   fb.nmu_d = (double) b->nmu;
   fb.s2_exact = std::max(b->s2bin[b->ns] * ldexp(1.0, -12), 4 * FK_DBL_EPS);
+  /* no pair at or beyond these is counted (margins cover the other rounding
+   * orientation of the gradient and the FP32 windows) */
+  fb.s2skip = fb.s2max * (1 + ldexp(1.0, -30));
+  fb.pskip = (b->bintype == FCFC_BIN_SPI) ? fb.pmax * (1 + ldexp(1.0, -30)) : 1e300;
+  fb.s2skip32 = (float) (fb.s2max + 2.0);
+  fb.pskip32 = (b->bintype == FCFC_BIN_SPI) ? (float) (fb.pmax + 2.0) : 3.0e38f;
   fb.mu_w = 0.5 - ldexp((double) b->nmu, -20);
   fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
   a.tiles = tiles; a.ntiles = ntiles;

@@ -52,6 +52,8 @@ struct FkBins {
   double mu_w;               /* mu bin settled exactly when the fraction of  */
                              /* t is outside [0.5 - mu_w, 0.5 + mu_w]        */
   double s2_exact;           /* ... or s^2 is below this value               */
+  double s2skip, pskip;      /* box: a pair at s^2 >= s2skip or pi >= pskip  */
+  float s2skip32, pskip32;   /* is out of range (FP64 / FP32 values)         */
   const uint16_t *stab;      /* device tables (global), staged to LDS       */
   const uint16_t *ptab;
   const double *s2bin;       /* edges for the hybrid refinement              */
