@@ -706,7 +706,7 @@ k_count(const FkArgs a) {
     const double tcz = J.g.oz + 0.5 * (t.iz0 + t.iz1 + 1) * J.g.wz;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int p = t.p0 + tid + r * FK_NT;
+      const int p = t.p0 + tid * R + r;     /* a wavefront holds one run of 64 R objects */
       const bool v = p < t.p1;
       ai[r] = v ? p : 0x7fffffff;
       ax[r] = v ? J.A.x[p] : FK_FAR;
@@ -760,7 +760,7 @@ k_count(const FkArgs a) {
         const int jj = (SELF || grad_self || F32) ? cj[j] : 0;
         const int code = NEED_C ? cc[j] : 13;
         if (GEOM == FCFC_GEOM_BOX && !PLAIN && !(GRAD && (code & 32))) {
-          /* the wavefront's A objects are two runs of consecutive objects
+          /* the wavefront's A objects are one run of consecutive objects
            * of one cell column: when this B object is out of range of all
            * of them, skip the binning (same expressions as the pair
            * functions, so the compiler shares them; gradient pairs taken in
@@ -799,7 +799,7 @@ k_count(const FkArgs a) {
             if (m) {
               if (amb) {
                 const int pos = qn + __popcll(m & lanemask_lt);
-                wq[pos] = ((unsigned) (tid + r * FK_NT) << 8) | (unsigned) j;
+                wq[pos] = ((unsigned) (tid * R + r) << 8) | (unsigned) j;
                 idx = FK_NONE;
               }
               qn += __popcll(m);
