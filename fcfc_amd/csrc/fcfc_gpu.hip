@@ -787,6 +787,23 @@ k_count(const FkArgs a) {
           }
           if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
         }
+        if (GEOM == FCFC_GEOM_SURVEY && !PLAIN) {
+          /* the same skip for survey pairs: |a - b|^2 (iso) or s - t
+           * (= s^2 = s_perp^2 + pi^2) beyond the largest counted value */
+          bool any = false;
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            if (BIN == FCFC_BIN_ISO) {
+              const double ex = ax[r] - bx, ey = ay[r] - by, ez = az[r] - bz;
+              any |= __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex)) < bn.s2skip;
+            }
+            else {
+              const double tt = __builtin_fma(az[r], bz, __builtin_fma(ay[r], by, ax[r] * bx)) * 2.0;
+              any |= (as[r] + bs) - tt < bn.s2skip;
+            }
+          }
+          if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
+        }
 #pragma unroll
         for (int r = 0; r < R; r++) {
           unsigned idx;
@@ -1447,10 +1464,8 @@ static kfn_t pick3(int wt, int mode, int hyb, int R) {
     if (mode == MODE_C32C)
       return wt ? pick5<BIN, GEOM, true, MODE_C32C>(hyb, R) : pick5<BIN, GEOM, false, MODE_C32C>(hyb, R);
   }
-  if constexpr (BIN == FCFC_BIN_SMU) {
-    if (mode == MODE_P64)
-      return wt ? pick5<BIN, GEOM, true, MODE_P64>(hyb, R) : pick5<BIN, GEOM, false, MODE_P64>(hyb, R);
-  }
+  if (mode == MODE_P64)
+    return wt ? pick5<BIN, GEOM, true, MODE_P64>(hyb, R) : pick5<BIN, GEOM, false, MODE_P64>(hyb, R);
   return wt ? pick5<BIN, GEOM, true, MODE_C64>(hyb, R) : pick5<BIN, GEOM, false, MODE_C64>(hyb, R);
 }
 
@@ -1738,7 +1753,10 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   fb.s2_exact = std::max(b->s2bin[b->ns] * ldexp(1.0, -12), 4 * FK_DBL_EPS);
   /* no pair at or beyond these is counted (margins cover the other rounding
    * orientation of the gradient and the FP32 windows) */
-  fb.s2skip = fb.s2max * (1 + ldexp(1.0, -30));
+  /* survey (s_perp, pi): s^2 = s_perp^2 + pi^2 (pbin holds pi^2 edges) */
+  fb.s2skip = (geom == FCFC_GEOM_SURVEY && b->bintype == FCFC_BIN_SPI)
+      ? (fb.s2max + fb.pmax) * (1 + ldexp(1.0, -30)) + 1e-6 * (fb.s2max + fb.pmax + 1)
+      : fb.s2max * (1 + ldexp(1.0, -30));
   fb.pskip = (b->bintype == FCFC_BIN_SPI) ? fb.pmax * (1 + ldexp(1.0, -30)) : 1e300;
   fb.s2skip32 = (float) (fb.s2max + 2.0);
   fb.pskip32 = (b->bintype == FCFC_BIN_SPI) ? (float) (fb.pmax + 2.0) : 3.0e38f;
