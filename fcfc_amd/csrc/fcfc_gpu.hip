@@ -163,7 +163,7 @@ __global__ void k_tile_emit(int nrow, int ncz, const int *__restrict__ cstart,
 struct FkWin {
   float e_s2;                 /* absolute window on s^2 (or s_perp^2)       */
   float e_pi;                 /* absolute window on pi                      */
-  float mu_w;                 /* mu bin certain if |fract(t) - 1/2| <= mu_w */
+  float mu_sh, mu_hi;         /* mu bin certain if fract(t - mu_sh) < mu_hi */
   float s2_floor;             /* pairs below this s^2 are re-evaluated      */
   float nmu_f;
 };
@@ -204,6 +204,7 @@ struct FkArgs {
   int nh;                     /* LDS histogram copies */
   /* LDS byte offsets */
   int off_hist, off_stab, off_ptab, off_edges, off_strip, off_chunk, off_queue;
+  int stab_c0, ptab_c0;       /* table entries before cell 0 */
   int census;
   const double *G;            /* GRAD: dL/dcnt */
 };
@@ -290,10 +291,12 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
 #define FK_FAR 1.0e30          /* coordinates of idle lanes: every pair out */
 
 /* s^2 (or s_perp^2) bin from the LDS table, FK_OUT outside [s2min, s2max).
- * Integer tables (setup_cf.c:385-410): index (int) s2 - (int) s2min, exactly
- * the reference's cvtt lookup (metric_common.c:500-510); the two sentinel
- * entries perform the range check.  Hybrid tables refine downwards against
- * the edges (FCFC_LOOKUP_HYBRID, metric_common.c:56-64). */
+ * Integer tables (setup_cf.c:385-410): cell (int) s2 - (int) s2min, exactly
+ * the reference's cvtt lookup (metric_common.c:500-510).  The LDS copy is
+ * zero-based: entry k belongs to (int) s2 = k, the entries below (int) s2min
+ * and the one at (int) s2min + len are FK_OUT, so with s2 >= 0 the range
+ * check is one min.  Hybrid tables (pointer at cell 0) refine downwards
+ * against the edges (FCFC_LOOKUP_HYBRID, metric_common.c:56-64). */
 template <bool HYB, bool MK = false>
 __device__ __forceinline__ unsigned fk_sbin(double s2, const FkBins &bn,
     const unsigned *stab, const double *s2e) {
@@ -308,9 +311,7 @@ __device__ __forceinline__ unsigned fk_sbin(double s2, const FkBins &bn,
     }
     return sb;
   }
-  /* (int) s2 - soff below zero wraps to a large unsigned value: clamped to
-   * the upper sentinel like s2 >= s2max */
-  const unsigned k = min((unsigned) ((int) s2 - bn.soff), (unsigned) bn.stab_len);
+  const int k = min((int) s2, bn.soff + bn.stab_len);
   return MK ? (stab[k] & 0xFFFFu) : stab[k];
 }
 
@@ -328,7 +329,7 @@ __device__ __forceinline__ unsigned fk_pbin(double p, const FkBins &bn,
     }
     return pb;
   }
-  const unsigned k = min((unsigned) ((int) p - bn.poff), (unsigned) bn.ptab_len);
+  const int k = min((int) p, bn.poff + bn.ptab_len);
   return MK ? (ptab[k] & 0xFFFFu) : ptab[k];
 }
 
@@ -338,17 +339,20 @@ __device__ __forceinline__ unsigned fk_pbin(double p, const FkBins &bn,
  * when m >= nmu^2 and bins it in mutab[m] = isqrt(m) (setup_cf.c:513-531).
  * Since isqrt(floor(q)) = floor(sqrt(q)) for q >= 0, the bin is floor(t),
  * t = sqrt(a / s2), and the drop rule is floor(t) >= nmu.
- * The caller passes t_apx, t from hardware reciprocal square roots (relative
- * error below 2^-21 including the products).  When t_apx is farther than
- * nmu 2^-20 from every integer its floor is the bin; otherwise, and for all
- * s2 below s2_exact = s2max 2^-12 (which covers s2 < DBL_EPSILON and bounds
- * t_apx by nmu (1 + 2^-21) on the fast path), the bin is settled exactly:
- * k = rint(t_apx) is the bin if a - k^2 s2 >= 0 (one fma, exact sign; k^2
- * is exact), else k - 1. */
-__device__ __forceinline__ unsigned fk_mu_exact(double a, double s2, double t_apx,
+ * The caller evaluates t from hardware reciprocal square roots (relative
+ * error below 2^-21 including the products) and passes the shifted value
+ * t_sh = t - w, w = nmu 2^-20, formed by the fma of the last product.  When
+ * t is farther than w from every integer, i.e. fract(t_sh) < 1 - 2w, then
+ * floor(t_sh) is the bin (t_sh in [k, k + 1 - 2w) puts the exact t in
+ * (k, k + 1)).  Otherwise, and for all s2 below s2_exact = s2max 2^-12
+ * (which covers s2 < DBL_EPSILON and bounds t by nmu (1 + 2^-21) on the
+ * fast path), the bin is settled exactly: k = rint(t_sh), which is within
+ * 1/2 of t, is the bin if a - k^2 s2 >= 0 (one fma, exact sign; k^2 is
+ * exact), else k - 1. */
+__device__ __forceinline__ unsigned fk_mu_exact(double a, double s2, double t_sh,
     const FkBins &bn) {
   if (!(s2 >= FK_DBL_EPS)) return 0u;
-  double k = __builtin_rint(t_apx);
+  double k = __builtin_rint(t_sh);
   if (__builtin_fma(-(k * k), s2, a) < 0.0) k -= 1.0;
   return (unsigned) (int) __builtin_fmin(k, bn.nmu_d);
 }
@@ -368,9 +372,9 @@ __device__ __forceinline__ unsigned fk_mu_plain(double num, double s2, const FkB
   return (unsigned) fk_isqrt((int) m);
 }
 
-__device__ __forceinline__ bool fk_mu_near(double t_apx, double s2, const FkBins &bn) {
-  const double fr = __builtin_amdgcn_fract(t_apx);
-  return !(__builtin_fabs(fr - 0.5) <= bn.mu_w) || s2 < bn.s2_exact;
+/* t_sh = t - w (negative t_sh: t below w, settled exactly as well) */
+__device__ __forceinline__ bool fk_mu_near(double t_sh, double s2, const FkBins &bn) {
+  return !(__builtin_amdgcn_fract(t_sh) < bn.mu_hi) || s2 < bn.s2_exact;
 }
 
 /* Box pair, B already at its periodic image (b' = fl(b + D)). */
@@ -390,7 +394,7 @@ __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
   const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
   if (BIN == FCFC_BIN_ISO) return sb;
   if (PLAIN) return sb + __umul24((unsigned) bn.ns, fk_mu_plain(dz2, s2, bn));
-  const double t = (__builtin_fabs(dz) * bn.nmu_d) * __builtin_amdgcn_rsq(s2);
+  const double t = __builtin_fma(__builtin_fabs(dz) * bn.nmu_d, __builtin_amdgcn_rsq(s2), -bn.mu_sh);
   unsigned mb = (unsigned) (int) t;
   if (fk_mu_near(t, s2, bn)) mb = fk_mu_exact(dz2 * bn.nmu2, s2, t, bn);
   return sb + __umul24((unsigned) bn.ns, mb);
@@ -416,8 +420,8 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
     const double s2 = s - t;
     const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
     if (PLAIN) return sb + __umul24((unsigned) bn.ns, fk_mu_plain((d * d) / (t + s), s2, bn));
-    const double tm = (__builtin_fabs(d) * bn.nmu_d) *
-        (__builtin_amdgcn_rsq(t + s) * __builtin_amdgcn_rsq(s2));
+    const double tm = __builtin_fma(__builtin_fabs(d) * bn.nmu_d,
+        __builtin_amdgcn_rsq(t + s) * __builtin_amdgcn_rsq(s2), -bn.mu_sh);
     unsigned mb = (unsigned) (int) tm;
     if (fk_mu_near(tm, s2, bn)) mb = fk_mu_exact(((d * d) / (t + s)) * bn.nmu2, s2, tm, bn);
     return sb + __umul24((unsigned) bn.ns, mb);
@@ -494,8 +498,11 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
   float s2;
   if (BIN == FCFC_BIN_SPI) s2 = __builtin_fmaf(dy, dy, dx * dx);
   else s2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, dz * dz));
-  /* s2 >= 0: truncation is the floor */
-  const unsigned e = stab[min((unsigned) ((int) s2 - bn.soff_m1), (unsigned) (bn.stab_len + 2))];
+  /* s2 >= 0: truncation is the floor.  The zero-based FP32 tables run to
+   * (int) s2min + len + 1; the entries next to the cells flag the edges at
+   * s2min and s2max, the others are plain FK_OUT for pairs certainly out of
+   * range */
+  const unsigned e = stab[min((int) s2, bn.soff + bn.stab_len + 1)];
   const float f = __builtin_amdgcn_fractf(s2);
   const unsigned msk = FK_F_ALW | (f < wn.e_s2 ? FK_F_LO : 0u) | (f > 1.0f - wn.e_s2 ? FK_F_HI : 0u);
   const bool amb = (e & msk) != 0;
@@ -504,7 +511,7 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
   if (BIN == FCFC_BIN_SPI) {
     const float pi = __builtin_fabsf(dz);
     const float fp = __builtin_floorf(pi);
-    const unsigned ep = ptab[min((unsigned) ((int) fp - bn.poff_m1), (unsigned) (bn.ptab_len + 2))];
+    const unsigned ep = ptab[min((int) fp, bn.poff + bn.ptab_len + 1)];
     const float g = pi - fp;
     const unsigned mp = FK_F_ALW | (g < wn.e_pi ? FK_F_LO : 0u) | (g > 1.0f - wn.e_pi ? FK_F_HI : 0u);
     const bool ambp = (ep & mp) != 0;
@@ -513,11 +520,14 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     ambig = (amb | ambp) & !out;
     return sb + __umul24((unsigned) bn.ns, pb);
   }
-  /* mu bin = floor(t), t = nmu |dz| / s; above s2_floor the FP32 error of t
-   * is below 1/2 - mu_w (fp32_windows) and t <= nmu (1 + 2^-20); the cells
-   * below s2_floor carry FK_F_ALW (every pair there is re-evaluated) */
-  const float t = (wn.nmu_f * __builtin_fabsf(dz)) * __builtin_amdgcn_rsqf(s2);
-  const bool nearm = !(__builtin_fabsf(__builtin_amdgcn_fractf(t) - 0.5f) <= wn.mu_w);
+  /* mu bin = floor(t), t = nmu |dz| / s; above s2_floor the FP32 error of
+   * t (the fma's rounding included) is below mu_sh (fp32_windows) and
+   * t <= nmu (1 + 2^-20); the cells below s2_floor carry FK_F_ALW (every
+   * pair there is re-evaluated).  With t_sh = t - mu_sh, fract(t_sh) <
+   * mu_hi <= 1 - 2 mu_sh puts the exact t strictly inside (k, k + 1),
+   * k = floor(t_sh). */
+  const float t = __builtin_fmaf(wn.nmu_f * __builtin_fabsf(dz), __builtin_amdgcn_rsqf(s2), -wn.mu_sh);
+  const bool nearm = !(__builtin_amdgcn_fractf(t) < wn.mu_hi);
   ambig = amb | (nearm & (sb != FK_OUT));      /* no short-circuit branches */
   return sb + __umul24((unsigned) bn.ns, (unsigned) (int) t);
 }
@@ -582,13 +592,15 @@ k_count(const FkArgs a) {
   /* ---- LDS carve-up (lds_plan on the host computes the same offsets) ---- */
   unsigned int *hist_u = (unsigned int *) (lds + a.off_hist);
   double *hist_d = (double *) (lds + a.off_hist);
-  /* tables: raw[0] and raw[len + 1], raw[len + 2] are FK_OUT sentinels,
-   * raw[1 + c] is integer cell c; the FP64 lookups index from raw + 1, the
-   * FP32 ones (which carry edge flags) from raw */
+  /* tables: raw[c0 + c] is integer cell c, c0 = a.stab_c0 (a.ptab_c0)
+   * entries precede cell 0 and two follow the last one.  Integer tables and
+   * all FP32 ones are zero-based (c0 = (int) s2min: entry k is the one of
+   * (int) s2 = k); hybrid ones for the FP64 forms may have c0 = 1.  Hybrid
+   * lookups index cells from raw + c0 */
   unsigned *stab_raw = (unsigned *) (lds + a.off_stab);
   unsigned *ptab_raw = (unsigned *) (lds + a.off_ptab);
-  unsigned *stab = stab_raw + 1;
-  unsigned *ptab = ptab_raw + 1;
+  const unsigned *stab = HYB ? stab_raw + a.stab_c0 : stab_raw;
+  const unsigned *ptab = HYB ? ptab_raw + a.ptab_c0 : ptab_raw;
   unsigned *wq = (unsigned *) (lds + a.off_queue) + wave * FK_QCAP;   /* FP32 re-evaluation queue */
   double *s2e = (double *) (lds + a.off_edges);
   double *pe = s2e + (bn.ns + 1);
@@ -629,11 +641,11 @@ k_count(const FkArgs a) {
       return v | f;
     };
     const double s2f = (F32 && BIN == FCFC_BIN_SMU) ? (double) a.jobs[0].win.s2_floor : 0.0;
-    for (int i = tid; i <= bn.stab_len + 2; i += FK_NT)
-      stab_raw[i] = entry(bn.stab, bn.stab_len, bn.ns, i - 1, bn.soff, s2f);
+    for (int i = tid; i < a.stab_c0 + bn.stab_len + 2; i += FK_NT)
+      stab_raw[i] = entry(bn.stab, bn.stab_len, bn.ns, i - a.stab_c0, bn.soff, s2f);
     if (BIN == FCFC_BIN_SPI)
-      for (int i = tid; i <= bn.ptab_len + 2; i += FK_NT)
-        ptab_raw[i] = entry(bn.ptab, bn.ptab_len, bn.np, i - 1, bn.poff, 0.0);
+      for (int i = tid; i < a.ptab_c0 + bn.ptab_len + 2; i += FK_NT)
+        ptab_raw[i] = entry(bn.ptab, bn.ptab_len, bn.np, i - a.ptab_c0, bn.poff, 0.0);
   }
   if (HYB) {
     for (int i = tid; i <= bn.ns; i += FK_NT) s2e[i] = bn.s2bin[i];
@@ -1407,11 +1419,23 @@ static int stage_cat(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, int need_s,
   return 0;
 }
 
-struct LdsPlan { int off_hist, off_stab, off_ptab, off_edges, off_strip, off_chunk, off_queue, total, nh; };
+struct LdsPlan { int off_hist, off_stab, off_ptab, off_edges, off_strip, off_chunk, off_queue, total, nh, stab_c0, ptab_c0; };
 
 static int align16(int v) { return (v + 15) & ~15; }
 
-static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh) {
+/* Table entries staged before cell 0: the integer tables and every FP32
+ * table are zero-based (entry k is the one of (int) s2 = k, so the lookup
+ * needs no lower range check), the hybrid tables of the FP64 forms keep one
+ * sentinel.  A zero-based table costs 4 bytes per entry below s2min; tables
+ * that start beyond FK_ZB_MAXOFF entries run through the hybrid lookups
+ * instead, which check the range against the edges and give the same bins
+ * for integer edges. */
+#define FK_ZB_MAXOFF 4096
+static bool zb_fits(const fcfc_bins *b) {
+  return b->soff <= FK_ZB_MAXOFF && (b->bintype != FCFC_BIN_SPI || b->poff <= FK_ZB_MAXOFF);
+}
+
+static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh, int hyb) {
   LdsPlan L;
   int o = 0;
   L.nh = nh;
@@ -1419,10 +1443,8 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   if (mode == MODE_GRAD) o += b->ntot * 8;
   else o += nh * b->ntot * (wt ? 8 : 4);
   o = align16(o);
-  L.off_stab = o; o = align16(o + (b->stab_len + 3) * 4);
-  L.off_ptab = o; if (b->bintype == FCFC_BIN_SPI) o = align16(o + (b->ptab_len + 3) * 4);
   L.off_edges = o;
-  if (b->tabtype) o = align16(o + 8 * (b->ns + 1 + (b->bintype == FCFC_BIN_SPI ? b->np + 1 : 0)));
+  if (hyb) o = align16(o + 8 * (b->ns + 1 + (b->bintype == FCFC_BIN_SPI ? b->np + 1 : 0)));
   L.off_strip = o; o = align16(o + (int) sizeof(FkStripLds));
   L.off_chunk = o;
   /* per staged B object: coordinates, |r|^2 (survey), weight, index, image code */
@@ -1434,6 +1456,13 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   o = align16(o + FK_CH * per);
   L.off_queue = o;
   if (mode == MODE_C32) o += (FK_NT / 64) * FK_QCAP * 4;
+  o = align16(o);
+  const int zero_based = (!hyb || mode == MODE_C32);
+  L.stab_c0 = zero_based ? b->soff : 1;
+  L.ptab_c0 = zero_based ? b->poff : 1;
+  L.off_stab = o; o = align16(o + (L.stab_c0 + b->stab_len + 2) * 4);
+  L.off_ptab = o;
+  if (b->bintype == FCFC_BIN_SPI) o = align16(o + (L.ptab_c0 + b->ptab_len + 2) * 4);
   L.total = o;
   return L;
 }
@@ -1519,7 +1548,14 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double 
   const double nmu = b->nmu;
   const double D = 1.25 * (nmu * 10 * u + (sqrt(3.0) * u * E * nmu + nmu * (u * E + tiny)) / sqrt(s2f))
                    + 4 * nmu * u;
-  w.mu_w = (float) std::max(0.0, 0.5 - D);
+  /* mu_sh >= D; mu_hi <= 1 - 2 mu_sh (0 when the windows cover everything) */
+  float sh = (float) std::min(D, 0.5);
+  if ((double) sh < std::min(D, 0.5)) sh = nextafterf(sh, 1.0f);
+  const double hi = 1.0 - 2.0 * (double) sh;
+  float hf = (float) hi;
+  if ((double) hf > hi) hf = nextafterf(hf, 0.0f);
+  w.mu_sh = sh;
+  w.mu_hi = hf;
   w.s2_floor = (float) s2f;
   w.nmu_f = (float) b->nmu;
 }
@@ -1610,6 +1646,9 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   const int geom = b->geometry;
   int m = mode;
   if (m == MODE_C32 && geom != FCFC_GEOM_BOX) m = MODE_C64;    /* survey FP32: FP64 kernel */
+  const bool zb = zb_fits(b);
+  if (m == MODE_C32 && !zb) m = MODE_C64;                      /* FP32 tables are zero-based */
+  const int hyb = (b->tabtype || !zb) ? 1 : 0;
   const int need_s = (geom == FCFC_GEOM_SURVEY && b->bintype != FCFC_BIN_ISO);
   int anywt = 0;
   for (auto &j : jobs) anywt |= j.wt;
@@ -1618,16 +1657,15 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   {
     int R = 1;
     for (auto &j : jobs) R = std::max(R, (j.P.T + FK_NT - 1) / FK_NT);
-    if (m == MODE_GRAD && b->tabtype) R = 1;
+    if (m == MODE_GRAD && hyb) R = 1;
     for (auto &j : jobs) { j.P.R = R; if (j.P.T_auto) j.P.T = R * FK_NT; }
   }
   /* ---- kernel and LDS plan ---- */
-  const int hyb = b->tabtype ? 1 : 0;
   const int R = jobs.empty() ? 1 : jobs[0].P.R;
   kfn_t fn = pick_kernel(b->bintype, geom, anywt, (m == MODE_C32 && c->census) ? MODE_C32C : m, hyb, R);
   /* histogram copies: as many as keep the occupancy of a single copy */
   const int lds_max = 160 * 1024 - 2048;       /* room for the static LDS */
-  LdsPlan L = lds_plan(b, m, anywt, need_s, 1);
+  LdsPlan L = lds_plan(b, m, anywt, need_s, 1, hyb);
   if (L.total > lds_max) {
     set_err(c, "binning tables need %d bytes of LDS (limit %d)", L.total, lds_max);
     return FCFC_ERR_CFG;
@@ -1642,7 +1680,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     if (int e = occ_of(L, &occ1)) return e;
     const int nhmax = c->t_nh > 0 ? c->t_nh : 8;
     for (int nh = 2; nh <= nhmax; nh *= 2) {
-      LdsPlan L2 = lds_plan(b, m, anywt, need_s, nh);
+      LdsPlan L2 = lds_plan(b, m, anywt, need_s, nh, hyb);
       int occ = 0;
       if (L2.total > lds_max) break;
       if (int e = occ_of(L2, &occ)) return e;
@@ -1743,7 +1781,6 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   fb.bintype = b->bintype; fb.geometry = geom;
   fb.ns = b->ns; fb.nmu = b->nmu; fb.np = b->np; fb.ntot = b->ntot;
   fb.tabtype = b->tabtype; fb.soff = b->soff; fb.poff = b->poff;
-  fb.soff_m1 = b->soff - 1; fb.poff_m1 = b->poff - 1;
   fb.stab_len = b->stab_len; fb.ptab_len = b->ptab_len;
   fb.s2min = b->s2bin[0]; fb.s2max = b->s2bin[b->ns];
   fb.pmin = (b->bintype == FCFC_BIN_SPI) ? b->pbin[0] : 0;
@@ -1760,7 +1797,8 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   fb.pskip = (b->bintype == FCFC_BIN_SPI) ? fb.pmax * (1 + ldexp(1.0, -30)) : 1e300;
   fb.s2skip32 = (float) (fb.s2max + 2.0);
   fb.pskip32 = (b->bintype == FCFC_BIN_SPI) ? (float) (fb.pmax + 2.0) : 3.0e38f;
-  fb.mu_w = 0.5 - ldexp((double) b->nmu, -20);
+  fb.mu_sh = ldexp((double) b->nmu, -20);
+  fb.mu_hi = 1.0 - 2.0 * fb.mu_sh;
   fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
   a.tiles = tiles; a.ntiles = ntiles;
   a.queue = dqueue;
@@ -1768,6 +1806,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   a.stats = dstats;
   a.nh = L.nh;
   a.off_hist = L.off_hist; a.off_stab = L.off_stab; a.off_ptab = L.off_ptab;
+  a.stab_c0 = L.stab_c0; a.ptab_c0 = L.ptab_c0;
   a.off_edges = L.off_edges; a.off_strip = L.off_strip;
   a.off_chunk = L.off_chunk; a.off_queue = L.off_queue;
   a.census = c->census;

@@ -43,14 +43,13 @@ struct FkBins {
   int ns, nmu, np, ntot;
   int tabtype;               /* 0 integer table, 1 hybrid table             */
   int soff, poff;            /* (int) s2min, (int) pmin (or pi^2 min)       */
-  int soff_m1, poff_m1;      /* soff - 1, poff - 1                          */
   int stab_len, ptab_len;    /* table lengths without sentinels             */
   double s2min, s2max;       /* squared s (or s_perp) range                 */
   double pmin, pmax;         /* pi range (box) or pi^2 range (survey)       */
   double nmu2;               /* nmu * nmu as double                          */
   double nmu_d;              /* nmu as double                                */
-  double mu_w;               /* mu bin settled exactly when the fraction of  */
-                             /* t is outside [0.5 - mu_w, 0.5 + mu_w]        */
+  double mu_sh, mu_hi;       /* mu bin settled exactly when t - mu_sh has a  */
+                             /* fraction >= mu_hi = 1 - 2 mu_sh              */
   double s2_exact;           /* ... or s^2 is below this value               */
   double s2skip, pskip;      /* box: a pair at s^2 >= s2skip or pi >= pskip  */
   float s2skip32, pskip32;   /* is out of range (FP64 / FP32 values)         */

@@ -153,6 +153,36 @@ def test_hybrid_bins(eng, cats):
         _cmp(g, ref, False)
 
 
+@pytest.mark.parametrize("bt,smin,smax,ds,pmin,pmax,dpi,soff", [
+    (F.BIN_SMU, 20.0, 200.0, 5.0, 0.0, 0.0, 1.0, 16),       # zero-based table, 16 cells below
+    (F.BIN_SPI, 20.0, 200.0, 5.0, 30.0, 150.0, 10.0, 16),
+    (F.BIN_SMU, 70.0, 100.0, 1.0, 0.0, 0.0, 1.0, 4900),     # far from zero: hybrid lookups
+    (F.BIN_SPI, 70.0, 100.0, 1.0, 80.0, 90.0, 1.0, 4900),
+])
+def test_offset_tables(eng, cats, bt, smin, smax, ds, pmin, pmax, dpi, soff):
+    """Integer tables whose range starts above zero: near zero they are
+    staged zero-based (cells below s2min are out of range), far from it the
+    hybrid lookups with explicit range checks take over; both give the
+    integer table's bins."""
+    (x, y, z, w), (xr, yr, zr) = cats
+    b = F.Bins.linear(F.GEOM_BOX, bt, smin, smax, ds, nmu=20 if bt == F.BIN_SMU else 1,
+                      pmin=pmin, pmax=pmax, dpi=dpi, bsize=L)
+    assert b.tabtype == 0 and int(b.s2bin[0]) == soff
+    d = F.Catalog(b, x, y, z, w)
+    r = F.Catalog(b, xr, yr, zr)
+    ref, cen = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None),
+                       (r.x, r.y, r.z, None, None), False, _obins(b), variant=1,
+                       census_mask=0b11111)
+    refa, cena = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None), None, True,
+                         _obins(b), variant=1, census_mask=0b11111)
+    for prec in ("fp64", "fp32", "plain"):
+        print(f"case offset table bt={bt} smin={smin} prec={prec}", flush=True)
+        g = eng.count_pairs(d, r, b, isauto=False, withwt=False, precision=prec)
+        _cmp(g, ref, False, cen)
+        ga = eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec)
+        _cmp(ga, refa, False, cena)
+
+
 def test_launch_shapes_identical(eng, cats):
     """Integer counts do not depend on the grid / tile / launch shape."""
     (x, y, z, w), (xr, yr, zr) = cats
