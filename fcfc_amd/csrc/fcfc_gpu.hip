@@ -161,8 +161,9 @@ __global__ void k_tile_emit(int nrow, int ncz, const int *__restrict__ cstart,
 
 /* FP32 exactness windows (see fp32_windows on the host). */
 struct FkWin {
-  float e_s2;                 /* absolute window on s^2 (or s_perp^2)       */
-  float e_pi;                 /* absolute window on pi                      */
+  float e_s2, e_s2_hi;        /* absolute window on s^2 (or s_perp^2), and  */
+                              /* 1 - e_s2                                   */
+  float e_pi, e_pi_hi;        /* absolute window on pi, and 1 - e_pi        */
   float mu_sh, mu_hi;         /* mu bin certain if fract(t - mu_sh) < mu_hi */
   float s2_floor;             /* pairs below this s^2 are re-evaluated      */
   float nmu_f;
@@ -504,7 +505,7 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
    * range */
   const unsigned e = stab[min((int) s2, bn.soff + bn.stab_len + 1)];
   const float f = __builtin_amdgcn_fractf(s2);
-  const unsigned msk = FK_F_ALW | (f < wn.e_s2 ? FK_F_LO : 0u) | (f > 1.0f - wn.e_s2 ? FK_F_HI : 0u);
+  const unsigned msk = FK_F_ALW | (f < wn.e_s2 ? FK_F_LO : 0u) | (f > wn.e_s2_hi ? FK_F_HI : 0u);
   const bool amb = (e & msk) != 0;
   const unsigned sb = e & 0xFFFFu;
   if (BIN == FCFC_BIN_ISO) { ambig = amb; return sb; }
@@ -513,7 +514,7 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     const float fp = __builtin_floorf(pi);
     const unsigned ep = ptab[min((int) fp, bn.poff + bn.ptab_len + 1)];
     const float g = pi - fp;
-    const unsigned mp = FK_F_ALW | (g < wn.e_pi ? FK_F_LO : 0u) | (g > 1.0f - wn.e_pi ? FK_F_HI : 0u);
+    const unsigned mp = FK_F_ALW | (g < wn.e_pi ? FK_F_LO : 0u) | (g > wn.e_pi_hi ? FK_F_HI : 0u);
     const bool ambp = (ep & mp) != 0;
     const unsigned pb = ep & 0xFFFFu;
     const bool out = ((sb == FK_OUT) & !amb) | ((pb == FK_OUT) & !ambp);
@@ -573,9 +574,13 @@ __device__ __forceinline__ unsigned fk_recheck(const FkBins &bn, const FkJob &J,
 
 /* One persistent kernel for counts (FP64 or FP32 + recheck) and for the
  * weight gradient.  Template parameters: binning, geometry, weighted
- * histogram support, mode, hybrid tables, A objects per lane. */
+ * histogram support, mode, hybrid tables, A objects per lane.  Unweighted
+ * kernels are held to 64 VGPRs (8 waves per SIMD, 4 workgroups per CU):
+ * occupancy pays more than the FP32 form's spills, which sit in its rare
+ * re-evaluation paths.  Weighted ones get 80 (6 waves per SIMD), as their
+ * FP64 LDS histograms limit them to 3 workgroups per CU anyway. */
 template <int BIN, int GEOM, bool WT, int MODE, bool HYB, int R>
-__global__ void __launch_bounds__(FK_NT, (WT || MODE == MODE_C32 || MODE == MODE_C32C) ? 6 : 8)
+__global__ void __launch_bounds__(FK_NT, WT ? 6 : 8)
 k_count(const FkArgs a) {
   extern __shared__ __attribute__((aligned(16))) char lds[];
   constexpr bool F32 = (MODE == MODE_C32 || MODE == MODE_C32C);
@@ -1545,6 +1550,8 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double 
   FkWin &w = *win;
   w.e_s2 = (float) (es2 * 1.25 + 1e-7);
   w.e_pi = (float) ((u * (2 * pmx + E) + tiny) * 1.25 + 1e-7);
+  w.e_s2_hi = 1.0f - w.e_s2;
+  w.e_pi_hi = 1.0f - w.e_pi;
   const double nmu = b->nmu;
   const double D = 1.25 * (nmu * 10 * u + (sqrt(3.0) * u * E * nmu + nmu * (u * E + tiny)) / sqrt(s2f))
                    + 4 * nmu * u;
