@@ -108,18 +108,36 @@ def cpu_baseline(target_s=15.0):
                       f"count phase {t:.2f} s, {pairs:.4g} unordered in-range pairs"}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_count_smu_box_RR_counters.txt")
+
+
 def traffic_from_profile():
     """HBM bytes per launch of the RR count kernel, from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy
     kernel calibration recorded there); None when absent."""
-    fn = os.path.join(ROOT, "profiles", "k_count_smu_box_RR_counters.txt")
     try:
-        for l in open(fn):
+        for l in open(PROFILE):
             if l.startswith("hbm_bytes_per_launch"):
                 return float(l.split()[1])
     except OSError:
         pass
     return None
+
+
+def issue_from_profile():
+    """VALU busy and VALU issue-slot fractions of the RR count kernel from the
+    same summary (rocprofv3 SQ counters); None when absent."""
+    out = {}
+    try:
+        for l in open(PROFILE):
+            t = l.split()
+            if t[:1] == ["VALUBusy"]:
+                out["valu_busy"] = float(t[-2]) / 100
+            elif t[:2] == ["issue", "slots"]:
+                out["valu_issue_slots"] = float(t[-2]) / 100
+    except (OSError, ValueError, IndexError):
+        return None
+    return out or None
 
 
 def mocks_workload(args):
@@ -461,7 +479,8 @@ def main():
                         "unit": "TFLOP/s", "frac": ach / peak, "traffic": traffic_from_profile(),
                         "kernel": f"k_count (s,mu) box {prec}, RR auto count", "kernel_ms": rr_ms,
                         "flops_per_pair": FLOPS_PAIR["smu"], "evals": rr_evals,
-                        "evals_per_s": rr_evals / (rr_ms * 1e-3)}
+                        "evals_per_s": rr_evals / (rr_ms * 1e-3),
+                        "issue_profile": issue_from_profile()}
     line["kernel_ms"] = {k: v / args.steps for k, v in kms.items()}
     # in-range pairs of RR (unordered), the algorithm-independent rate
     line["inrange_pairs_per_s_RR"] = float(res["RR"].sum()) / (rr_ms * 1e-3)

@@ -282,11 +282,19 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
 
 /* ---------------- FP64 pair evaluation (the arithmetic contract) -------- */
 
-/* Pair functions return a histogram index; every value >= ntot means "not
- * counted".  Table sentinels are FK_OUT = 0xFFFF >= ntot (the histogram
- * lives in LDS, so ntot < 40960), hence sb + ns * mb needs no range test of
- * its own: an out-of-range s (or pi) gives sb >= ntot, a dropped mu gives
- * mb = nmu and sb + ns * nmu >= ntot. */
+/* Pair functions return an LDS histogram index sb + nsh * mb; every value
+ * >= ntoh = nsh * rows means "not counted".  The row stride nsh is ns
+ * rounded up to an odd number: lanes of one histogram atomic that share an
+ * s bin but differ in mu (or pi) then fall on different LDS banks (an even
+ * stride such as 40 folds them onto a few).  Table sentinels are FK_OUT =
+ * 0xFFFF >= ntoh (the histogram lives in LDS, so ntoh < 40960), hence
+ * sb + nsh * mb needs no range test of its own: an out-of-range s (or pi)
+ * gives sb >= ntoh, a dropped mu gives mb = nmu and sb + nsh * nmu >= ntoh.
+ * fk_hslot maps an LDS slot to the reference layout sb + ns * mb. */
+__device__ __forceinline__ int fk_hslot(int i, int ns, int nsh) {
+  const int row = i / nsh, col = i - row * nsh;
+  return col < ns ? col + ns * row : -1;
+}
 #define FK_NONE 0xFFFFFFFFu
 #define FK_DBL_EPS 2.220446049250313e-16
 #define FK_FAR 1.0e30          /* coordinates of idle lanes: every pair out */
@@ -388,17 +396,17 @@ __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
     const double s2 = __builtin_fma(dy, dy, dx * dx);          /* :416-420 */
     const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
     const unsigned pb = fk_pbin<HYB, MK>(pi, bn, ptab, pe);
-    return sb + __umul24((unsigned) bn.ns, pb);
+    return sb + __umul24((unsigned) bn.nsh, pb);
   }
   const double dz2 = dz * dz;
   const double s2 = __builtin_fma(dy, dy, __builtin_fma(dx, dx, dz2));  /* :426-430 */
   const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
   if (BIN == FCFC_BIN_ISO) return sb;
-  if (PLAIN) return sb + __umul24((unsigned) bn.ns, fk_mu_plain(dz2, s2, bn));
+  if (PLAIN) return sb + __umul24((unsigned) bn.nsh, fk_mu_plain(dz2, s2, bn));
   const double t = __builtin_fma(__builtin_fabs(dz) * bn.nmu_d, __builtin_amdgcn_rsq(s2), -bn.mu_sh);
   unsigned mb = (unsigned) (int) t;
   if (fk_mu_near(t, s2, bn)) mb = fk_mu_exact(dz2 * bn.nmu2, s2, t, bn);
-  return sb + __umul24((unsigned) bn.ns, mb);
+  return sb + __umul24((unsigned) bn.nsh, mb);
 }
 
 /* Survey pair (src/fcfc/2pt/metric_common.c:283-460, FMA vector formulas):
@@ -420,12 +428,12 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
   if (BIN == FCFC_BIN_SMU) {
     const double s2 = s - t;
     const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
-    if (PLAIN) return sb + __umul24((unsigned) bn.ns, fk_mu_plain((d * d) / (t + s), s2, bn));
+    if (PLAIN) return sb + __umul24((unsigned) bn.nsh, fk_mu_plain((d * d) / (t + s), s2, bn));
     const double tm = __builtin_fma(__builtin_fabs(d) * bn.nmu_d,
         __builtin_amdgcn_rsq(t + s) * __builtin_amdgcn_rsq(s2), -bn.mu_sh);
     unsigned mb = (unsigned) (int) tm;
     if (fk_mu_near(tm, s2, bn)) mb = fk_mu_exact(((d * d) / (t + s)) * bn.nmu2, s2, tm, bn);
-    return sb + __umul24((unsigned) bn.ns, mb);
+    return sb + __umul24((unsigned) bn.nsh, mb);
   }
   /* (s_perp, pi): pbin holds pi^2 edges; with s2min == 0 the reference has
    * no lower bound on s_perp^2 (a rounding-negative value lands in bin 0) */
@@ -433,7 +441,7 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
   const double sp2 = (s - t) - p2;
   const unsigned sb = fk_sbin<HYB>(sp2, bn, stab, s2e);
   const unsigned pb = fk_pbin<HYB>(p2, bn, ptab, pe);
-  return sb + __umul24((unsigned) bn.ns, pb);
+  return sb + __umul24((unsigned) bn.nsh, pb);
 }
 
 /* The other arithmetic paths of the reference, for the rounding census
@@ -461,7 +469,7 @@ __device__ unsigned fk_box_variant(int var, const double a[3], const double bo[3
     const double s2 = scalar ? (d[0] * d[0] + d[1] * d[1]) : __builtin_fma(d[1], d[1], d[0] * d[0]);
     const unsigned sb = fk_sbin<HYB, true>(s2, bn, stab, s2e);
     const unsigned pb = fk_pbin<HYB, true>(pi, bn, ptab, pe);
-    return sb + __umul24((unsigned) bn.ns, pb);
+    return sb + __umul24((unsigned) bn.nsh, pb);
   }
   const double dz2 = d[2] * d[2];
   double s2;
@@ -482,7 +490,7 @@ __device__ unsigned fk_box_variant(int var, const double a[3], const double bo[3
       m = (int) mq;
     }
   }
-  return sb + __umul24((unsigned) bn.ns, (unsigned) fk_isqrt(m));
+  return sb + __umul24((unsigned) bn.nsh, (unsigned) fk_isqrt(m));
 }
 
 /* ---------------- FP32 pair evaluation with exactness windows ----------- */
@@ -519,7 +527,7 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     const unsigned pb = ep & 0xFFFFu;
     const bool out = ((sb == FK_OUT) & !amb) | ((pb == FK_OUT) & !ambp);
     ambig = (amb | ambp) & !out;
-    return sb + __umul24((unsigned) bn.ns, pb);
+    return sb + __umul24((unsigned) bn.nsh, pb);
   }
   /* mu bin = floor(t), t = nmu |dz| / s; above s2_floor the FP32 error of
    * t (the fma's rounding included) is below mu_sh (fp32_windows) and
@@ -530,7 +538,7 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
   const float t = __builtin_fmaf(wn.nmu_f * __builtin_fabsf(dz), __builtin_amdgcn_rsqf(s2), -wn.mu_sh);
   const bool nearm = !(__builtin_amdgcn_fractf(t) < wn.mu_hi);
   ambig = amb | (nearm & (sb != FK_OUT));      /* no short-circuit branches */
-  return sb + __umul24((unsigned) bn.ns, (unsigned) (int) t);
+  return sb + __umul24((unsigned) bn.nsh, (unsigned) (int) t);
 }
 
 /* Exact FP64 re-evaluation of a pair flagged by the FP32 windows (rare: the
@@ -542,7 +550,7 @@ template <int BIN, bool HYB>
 __device__ __forceinline__ bool fk_census(const FkBins &bn, const FkJob &J, double ax,
     double ay, double az, const double bo[3], const int c3[3], unsigned idx,
     const unsigned *stab, const unsigned *ptab, const double *s2e, const double *pe) {
-  const unsigned nt = (unsigned) bn.ntot;
+  const unsigned nt = (unsigned) bn.ntoh;
   const double av3[3] = {ax, ay, az};
   bool differ = false;
   for (int v = 0; v < 4; v++) {
@@ -564,7 +572,7 @@ __device__ __forceinline__ unsigned fk_recheck(const FkBins &bn, const FkJob &J,
   const double ey = bo[1] + c3[1] * J.g.Ly;
   const double ez = bo[2] + c3[2] * J.g.Lz;
   unsigned idx = fk_box64<BIN, HYB, true>(ex - ax, ey - ay, ez - az, bn, stab, ptab, s2e, pe);
-  if (idx >= (unsigned) bn.ntot) idx = FK_NONE;
+  if (idx >= (unsigned) bn.ntoh) idx = FK_NONE;
   if (CEN && fk_census<BIN, HYB>(bn, J, ax, ay, az, bo, c3, idx, stab, ptab, s2e, pe))
     (*ncensus)++;
   return idx;
@@ -592,7 +600,7 @@ k_count(const FkArgs a) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const FkBins &bn = a.bins;
-  const unsigned ntot = (unsigned) bn.ntot;
+  const unsigned ntot = (unsigned) bn.ntoh;   /* LDS histogram slots */
 
   /* ---- LDS carve-up (lds_plan on the host computes the same offsets) ---- */
   unsigned int *hist_u = (unsigned int *) (lds + a.off_hist);
@@ -624,9 +632,12 @@ k_count(const FkArgs a) {
   __shared__ int s_tile;
 
   /* ---- stage tables ---- */
-  const int hist_elems = a.nh * bn.ntot;
+  const int hist_elems = a.nh * bn.ntoh;
   if (GRAD) {
-    for (int i = tid; i < bn.ntot; i += FK_NT) hist_d[i] = a.G[i];
+    for (int i = tid; i < bn.ntoh; i += FK_NT) {
+      const int g = fk_hslot(i, bn.ns, bn.nsh);
+      hist_d[i] = g >= 0 ? a.G[g] : 0.0;
+    }
   }
   else if (WT) { for (int i = tid; i < hist_elems; i += FK_NT) hist_d[i] = 0.0; }   /* covers the u32 view */
   else { for (int i = tid; i < hist_elems; i += FK_NT) hist_u[i] = 0u; }
@@ -660,8 +671,8 @@ k_count(const FkArgs a) {
   double evals = 0;
   unsigned int nrecheck = 0, ncensus = 0;
   double since_flush = 0;
-  unsigned int *myhist_u = hist_u + (wave % a.nh) * bn.ntot;
-  double *myhist_d = hist_d + (wave % a.nh) * bn.ntot;
+  unsigned int *myhist_u = hist_u + (wave % a.nh) * bn.ntoh;
+  double *myhist_d = hist_d + (wave % a.nh) * bn.ntoh;
   __shared__ FkJob sJ;
   const FkJob &J = sJ;
   int cur = -1;
@@ -671,16 +682,18 @@ k_count(const FkArgs a) {
     __syncthreads();
     if (cur >= 0) {
       const bool w = WT && J.wt;
-      for (int i = tid; i < bn.ntot; i += FK_NT) {
+      for (int i = tid; i < bn.ntoh; i += FK_NT) {
+        const int g = fk_hslot(i, bn.ns, bn.nsh);    /* padding slots stay 0 */
+        if (g < 0) continue;
         if (w) {
           double v = 0;
-          for (int c = 0; c < a.nh; c++) { v += hist_d[c * bn.ntot + i]; hist_d[c * bn.ntot + i] = 0.0; }
-          if (v != 0) unsafeAtomicAdd(a.gcnt_d + J.hist_off + i, v);
+          for (int c = 0; c < a.nh; c++) { v += hist_d[c * bn.ntoh + i]; hist_d[c * bn.ntoh + i] = 0.0; }
+          if (v != 0) unsafeAtomicAdd(a.gcnt_d + J.hist_off + g, v);
         }
         else {
           unsigned long long v = 0;
-          for (int c = 0; c < a.nh; c++) { v += hist_u[c * bn.ntot + i]; hist_u[c * bn.ntot + i] = 0u; }
-          if (v) atomicAdd(a.gcnt_u + J.hist_off + i, v);
+          for (int c = 0; c < a.nh; c++) { v += hist_u[c * bn.ntoh + i]; hist_u[c * bn.ntoh + i] = 0u; }
+          if (v) atomicAdd(a.gcnt_u + J.hist_off + g, v);
         }
       }
     }
@@ -1424,6 +1437,15 @@ static int stage_cat(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, int need_s,
   return 0;
 }
 
+/* LDS histogram geometry: row stride ns rounded up to an odd number (rows
+ * of mu or pi bins), see fk_hslot */
+static int hist_stride(const fcfc_bins *b) {
+  return b->bintype == FCFC_BIN_ISO ? b->ns : (b->ns | 1);
+}
+static int hist_slots(const fcfc_bins *b) {
+  return hist_stride(b) * (b->ntot / b->ns);
+}
+
 struct LdsPlan { int off_hist, off_stab, off_ptab, off_edges, off_strip, off_chunk, off_queue, total, nh, stab_c0, ptab_c0; };
 
 static int align16(int v) { return (v + 15) & ~15; }
@@ -1445,8 +1467,8 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   int o = 0;
   L.nh = nh;
   L.off_hist = o;
-  if (mode == MODE_GRAD) o += b->ntot * 8;
-  else o += nh * b->ntot * (wt ? 8 : 4);
+  if (mode == MODE_GRAD) o += hist_slots(b) * 8;
+  else o += nh * hist_slots(b) * (wt ? 8 : 4);
   o = align16(o);
   L.off_edges = o;
   if (hyb) o = align16(o + 8 * (b->ns + 1 + (b->bintype == FCFC_BIN_SPI ? b->np + 1 : 0)));
@@ -1787,6 +1809,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   FkBins &fb = a.bins;
   fb.bintype = b->bintype; fb.geometry = geom;
   fb.ns = b->ns; fb.nmu = b->nmu; fb.np = b->np; fb.ntot = b->ntot;
+  fb.nsh = hist_stride(b); fb.ntoh = hist_slots(b);
   fb.tabtype = b->tabtype; fb.soff = b->soff; fb.poff = b->poff;
   fb.stab_len = b->stab_len; fb.ptab_len = b->ptab_len;
   fb.s2min = b->s2bin[0]; fb.s2max = b->s2bin[b->ns];

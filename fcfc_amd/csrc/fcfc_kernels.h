@@ -41,6 +41,7 @@
 struct FkBins {
   int bintype, geometry;
   int ns, nmu, np, ntot;
+  int nsh, ntoh;             /* LDS histogram row stride (odd) and slots    */
   int tabtype;               /* 0 integer table, 1 hybrid table             */
   int soff, poff;            /* (int) s2min, (int) pmin (or pi^2 min)       */
   int stab_len, ptab_len;    /* table lengths without sentinels             */
