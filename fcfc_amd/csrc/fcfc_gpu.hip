@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -45,19 +46,6 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
-__global__ void k_cell_assign(FkGrid g, int n, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z,
-    int *__restrict__ cid, int *__restrict__ rank, int *__restrict__ count) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    int ix = fk_cell1(x[i], g.ox, g.wx, g.ncx);
-    int iy = fk_cell1(y[i], g.oy, g.wy, g.ncy);
-    int iz = fk_cell1(z[i], g.oz, g.wz, g.ncz);
-    int c = (ix * g.ncy + iy) * g.ncz + iz;
-    cid[i] = c;
-    rank[i] = atomicAdd(count + c, 1);
-  }
-}
-
 /* Exclusive scan of n ints into out[0..n] (out[n] = total), one workgroup
  * of 1024 threads, each scanning a contiguous slice. */
 __global__ void __launch_bounds__(1024) k_scan1(const int *__restrict__ in,
@@ -81,32 +69,74 @@ __global__ void __launch_bounds__(1024) k_scan1(const int *__restrict__ in,
   if (t == 1023) out[n] = part[1023];
 }
 
-__global__ void k_scatter(int n, const int *__restrict__ cid,
-    const int *__restrict__ rank, const int *__restrict__ cstart,
-    const double *__restrict__ x, const double *__restrict__ y,
-    const double *__restrict__ z, const double *__restrict__ s,
-    const double *__restrict__ w, double *__restrict__ ox,
-    double *__restrict__ oy, double *__restrict__ oz,
-    double *__restrict__ os, double *__restrict__ ow, int *__restrict__ oidx) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    int d = cstart[cid[i]] + rank[i];
-    ox[d] = x[i]; oy[d] = y[i]; oz[d] = z[i];
-    if (s) os[d] = s[i];
-    if (w) ow[d] = w[i];
-    oidx[d] = i;
+/* ---- grid build: every catalogue of a call sorted into its job's grid
+ * in one pass per stage (cell counting sort), so that a batch of hundreds
+ * of small catalogues costs a handful of launches ---- */
+struct FkStage {
+  int n, obj0, cell0, cs0;    /* objects; offsets in the concatenated object,
+                                 count and cell-start index spaces */
+  int need_s;                 /* survey |r|^2 wanted (computed when s is null) */
+  FkGrid g;
+  const double *x, *y, *z, *s, *w;      /* inputs; s, w may be null */
+  double *ox, *oy, *oz, *os, *ow;       /* sorted copies; os, ow null when unused */
+  int *oidx, *ocs;                      /* original index, cell starts [ncell + 1] */
+};
+
+/* request owning index v of the concatenated space whose starts are off */
+__device__ __forceinline__ int fk_stage_find(const FkStage *__restrict__ st, int nst, int v,
+    int FkStage::*off) {
+  int lo = 0, hi = nst - 1;                   /* last request with start <= v */
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (st[mid].*off <= v) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_stage_assign(int nobj, int nst, const FkStage *__restrict__ st,
+    int *__restrict__ cid, int *__restrict__ rank, int *__restrict__ count) {
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < nobj; o += gridDim.x * blockDim.x) {
+    const FkStage &q = st[fk_stage_find(st, nst, o, &FkStage::obj0)];
+    const int i = o - q.obj0;
+    const int ix = fk_cell1(q.x[i], q.g.ox, q.g.wx, q.g.ncx);
+    const int iy = fk_cell1(q.y[i], q.g.oy, q.g.wy, q.g.ncy);
+    const int iz = fk_cell1(q.z[i], q.g.oz, q.g.wz, q.g.ncz);
+    const int c = (ix * q.g.ncy + iy) * q.g.ncz + iz;
+    cid[o] = c;
+    rank[o] = atomicAdd(count + q.cell0 + c, 1);
   }
 }
 
-/* |r|^2 with FCFC's data_preprocess lane rule (2pt/build_tree.c:59-92):
- * FMA for the first floor(n/8)*8 objects, unfused for the tail. */
-__global__ void k_survey_norm(int n, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z,
-    double *__restrict__ s) {
-  const int nv = (n / 8) * 8;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double a = x[i], b = y[i], c = z[i];
-    if (i < nv) s[i] = __builtin_fma(c, c, __builtin_fma(b, b, a * a));
-    else { double t = a * a; t = t + b * b; s[i] = t + c * c; }
+/* cell starts of each catalogue from the scan of all counts */
+__global__ void k_stage_starts(int ncs, int nst, const FkStage *__restrict__ st,
+    const int *__restrict__ gstart) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ncs; t += gridDim.x * blockDim.x) {
+    const FkStage &q = st[fk_stage_find(st, nst, t, &FkStage::cs0)];
+    q.ocs[t - q.cs0] = gstart[q.cell0 + (t - q.cs0)] - gstart[q.cell0];
+  }
+}
+
+/* sorted copies; |r|^2 with FCFC's data_preprocess lane rule
+ * (2pt/build_tree.c:59-92: FMA for the first floor(n/8)*8 objects, unfused
+ * for the tail), unit weights where a weighted count has none
+ * (build_tree.c:143-156) */
+__global__ void k_stage_scatter(int nobj, int nst, const FkStage *__restrict__ st,
+    const int *__restrict__ cid, const int *__restrict__ rank, const int *__restrict__ gstart) {
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < nobj; o += gridDim.x * blockDim.x) {
+    const FkStage &q = st[fk_stage_find(st, nst, o, &FkStage::obj0)];
+    const int i = o - q.obj0;
+    const int d = gstart[q.cell0 + cid[o]] - gstart[q.cell0] + rank[o];
+    const double a = q.x[i], b = q.y[i], c = q.z[i];
+    q.ox[d] = a; q.oy[d] = b; q.oz[d] = c;
+    if (q.os) {
+      double v;
+      if (q.s) v = q.s[i];
+      else if (i < (q.n / 8) * 8) v = __builtin_fma(c, c, __builtin_fma(b, b, a * a));
+      else { double t = a * a; t = t + b * b; v = t + c * c; }
+      q.os[d] = v;
+    }
+    if (q.ow) q.ow[d] = q.w ? q.w[i] : 1.0;
+    q.oidx[d] = i;
   }
 }
 
@@ -142,17 +172,37 @@ __device__ int row_tiles(const int *__restrict__ cs, int ncz, int T, int zmaxc,
   return nt;
 }
 
-__global__ void k_tile_count(int nrow, int ncz, const int *__restrict__ cstart,
-    int T, int zmaxc, int *__restrict__ ntile) {
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
-    ntile[r] = row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, 0, 0, 1, nullptr);
+/* Tiling of all jobs of a launch in one pass: job k owns rows row0 ..
+ * row0 + ncx ncy - 1 of the launch's row space (row0 ascending). */
+struct FkTileJob { int row0, ncz, T, zmaxc, nstrips; const int *cs; };
+
+__device__ __forceinline__ int fk_tile_job(const FkTileJob *__restrict__ tj, int njob, int gr) {
+  int lo = 0, hi = njob - 1;                  /* last job with row0 <= gr */
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tj[mid].row0 <= gr) lo = mid; else hi = mid - 1;
+  }
+  return lo;
 }
-/* items of row r start at S * toff[r] */
-__global__ void k_tile_emit(int nrow, int ncz, const int *__restrict__ cstart,
-    int T, int zmaxc, const int *__restrict__ toff, int job, int nstrips, int S,
-    FkTile *__restrict__ tiles) {
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
-    row_tiles(cstart + (size_t) r * ncz, ncz, T, zmaxc, r, job, nstrips, S, tiles + (size_t) S * toff[r]);
+
+__global__ void k_tile_count(int nrow, int njob, const FkTileJob *__restrict__ tj,
+    int *__restrict__ ntile) {
+  for (int gr = blockIdx.x * blockDim.x + threadIdx.x; gr < nrow; gr += gridDim.x * blockDim.x) {
+    const FkTileJob t = tj[fk_tile_job(tj, njob, gr)];
+    const int r = gr - t.row0;
+    ntile[gr] = row_tiles(t.cs + (size_t) r * t.ncz, t.ncz, t.T, t.zmaxc, r, 0, 0, 1, nullptr);
+  }
+}
+/* items of row gr start at S * toff[gr] */
+__global__ void k_tile_emit(int nrow, int njob, const FkTileJob *__restrict__ tj,
+    const int *__restrict__ toff, int S, FkTile *__restrict__ tiles) {
+  for (int gr = blockIdx.x * blockDim.x + threadIdx.x; gr < nrow; gr += gridDim.x * blockDim.x) {
+    const int k = fk_tile_job(tj, njob, gr);
+    const FkTileJob t = tj[k];
+    const int r = gr - t.row0;
+    row_tiles(t.cs + (size_t) r * t.ncz, t.ncz, t.T, t.zmaxc, r, k, t.nstrips, S,
+              tiles + (size_t) S * toff[gr]);
+  }
 }
 
 /* ======================================================================== */
@@ -1034,7 +1084,7 @@ enum {
   WS_IN2_X, WS_IN2_Y, WS_IN2_Z, WS_IN2_S, WS_IN2_W,     /* uploads, catalogue B */
   WS_A_X, WS_A_Y, WS_A_Z, WS_A_S, WS_A_W, WS_A_IDX, WS_A_CS,
   WS_B_X, WS_B_Y, WS_B_Z, WS_B_S, WS_B_W, WS_B_IDX, WS_B_CS,
-  WS_CID, WS_RANK, WS_COUNT, WS_ROWN, WS_ROWOFF, WS_TILES,
+  WS_CID, WS_RANK, WS_COUNT, WS_GSTART, WS_STAGE, WS_TJOBS, WS_ROWN, WS_ROWOFF, WS_TILES,
   WS_STRIPS, WS_STAB, WS_PTAB, WS_S2E, WS_PE, WS_HIST, WS_MISC, WS_G, WS_GA, WS_GB, WS_JOBS,
   WS_NSLOT
 };
@@ -1297,20 +1347,31 @@ struct Alloc {
   fcfc_gpu_ctx *c;
   bool own;
   std::vector<void *> blocks;
-  int err = 0;
+  char *cur = nullptr;         /* own: buffers are carved from 64 MiB chunks */
+  size_t left = 0;             /* (a batch of hundreds of small catalogues   */
+  int err = 0;                 /* would otherwise pay a hipMalloc per array) */
   Alloc(fcfc_gpu_ctx *c_, bool own_) : c(c_), own(own_) {}
   ~Alloc() { for (void *p : blocks) (void) hipFree(p); }
   void *get(int slot, size_t bytes) {
     if (bytes == 0) bytes = 8;
     if (!own) return ws(c, slot, bytes, &err);
-    void *p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-      set_err(c, "hipMalloc of %zu bytes failed", bytes);
-      err = FCFC_ERR_MEMORY;
-      return nullptr;
+    bytes = (bytes + 255) & ~(size_t) 255;
+    if (bytes > left) {
+      const size_t chunk = std::max(bytes, (size_t) 64 << 20);
+      void *p = nullptr;
+      if (hipMalloc(&p, chunk) != hipSuccess) {
+        set_err(c, "hipMalloc of %zu bytes failed", chunk);
+        err = FCFC_ERR_MEMORY;
+        return nullptr;
+      }
+      blocks.push_back(p);
+      cur = (char *) p;
+      left = chunk;
     }
-    blocks.push_back(p);
-    return p;
+    void *r = cur;
+    cur += bytes;
+    left -= bytes;
+    return r;
   }
 };
 
@@ -1371,15 +1432,15 @@ static int bbox(fcfc_gpu_ctx *c, const fcfc_cat *k, double lo[3], double hi[3]) 
   return 0;
 }
 
-/* Sort one catalogue into the grid of a job (cell counting sort), with the
- * survey |r|^2 and unit weights filled in where needed.  slot: first of 7
+/* Prepare the sort of one catalogue into the grid of a job: upload host
+ * inputs, allocate the sorted copies (so the job's descriptors can point
+ * at them at once) and queue the work for stage_run.  slot: first of 7
  * consecutive workspace slots (single-job calls). */
-static int stage_cat(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, int need_s,
-    int wt, const FkGrid &g, int slot, int upslot, DevCat *out) {
+static int stage_prep(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, int need_s,
+    int wt, const FkGrid &g, int slot, int upslot, DevCat *out, std::vector<FkStage> &st) {
   const int n = (int) k->n;
   const size_t nb = sizeof(double) * (size_t) (n > 0 ? n : 1);
   const int ncell = g.ncx * g.ncy * g.ncz;
-  int err = 0;
   /* inputs on the device */
   const double *x = k->x, *y = k->y, *z = k->z, *s = need_s ? k->s : nullptr, *w = wt ? k->w : nullptr;
   if (!k->on_device && n > 0) {
@@ -1411,29 +1472,50 @@ static int stage_cat(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, int need_s,
   out->w = wt ? (double *) al.get(slot + 4, nb) : nullptr;
   out->idx = (int *) al.get(slot + 5, sizeof(int) * (size_t) (n > 0 ? n : 1));
   out->cs = (int *) al.get(slot + 6, sizeof(int) * ((size_t) ncell + 1));
-  int *cid = (int *) ws(c, WS_CID, sizeof(int) * (size_t) (n > 0 ? n : 1), &err);
-  int *rank = (int *) ws(c, WS_RANK, sizeof(int) * (size_t) (n > 0 ? n : 1), &err);
-  int *count = (int *) ws(c, WS_COUNT, sizeof(int) * (size_t) ncell, &err);
   if (al.err) return al.err;
-  if (err) return err;
-  double *stmp = nullptr;
-  if (need_s && !s && n > 0) {
-    stmp = (double *) al.get(upslot + 3, nb);
-    if (al.err) return al.err;
-    if (err) return err;
-    hipLaunchKernelGGL(k_survey_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, x, y, z, stmp);
-    s = stmp;
+  FkStage q;
+  memset(&q, 0, sizeof q);
+  q.n = n; q.need_s = need_s; q.g = g;
+  q.x = x; q.y = y; q.z = z; q.s = s; q.w = w;
+  q.ox = out->x; q.oy = out->y; q.oz = out->z; q.os = out->s; q.ow = out->w;
+  q.oidx = out->idx; q.ocs = out->cs;
+  st.push_back(q);
+  return 0;
+}
+
+/* Run the queued sorts of a call: one launch per stage for all of them. */
+static int stage_run(fcfc_gpu_ctx *c, std::vector<FkStage> &st) {
+  if (st.empty()) return 0;
+  long long nobj = 0, ncell = 0, ncs = 0;
+  for (FkStage &q : st) {
+    q.obj0 = (int) nobj; q.cell0 = (int) ncell; q.cs0 = (int) ncs;
+    const long long nc = (long long) q.g.ncx * q.g.ncy * q.g.ncz;
+    nobj += q.n; ncell += nc; ncs += nc + 1;
+    if (nobj > INT_MAX / 2 || ncs > INT_MAX / 2) { set_err(c, "catalogues of one call too large"); return FCFC_ERR_CFG; }
   }
-  hipLaunchKernelGGL(k_fill_int, dim3(nblk(ncell, 256)), dim3(256), 0, c->stream, count, ncell, 0);
-  if (n > 0)
-    hipLaunchKernelGGL(k_cell_assign, dim3(nblk(n, 256)), dim3(256), 0, c->stream, g, n, x, y, z, cid, rank, count);
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, count, out->cs, ncell);
-  if (n > 0)
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, cid, rank, out->cs,
-        x, y, z, need_s ? s : nullptr, w, out->x, out->y, out->z, out->s, out->w, out->idx);
-  if (wt && !w && n > 0)     /* unit weights (build_tree.c:143-156) */
-    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n, 256)), dim3(256), 0, c->stream, out->w, (int64_t) n, 1.0);
+  int err = 0;
+  const int nst = (int) st.size();
+  FkStage *dst = (FkStage *) ws(c, WS_STAGE, sizeof(FkStage) * st.size(), &err);
+  int *cid = (int *) ws(c, WS_CID, sizeof(int) * (size_t) (nobj > 0 ? nobj : 1), &err);
+  int *rank = (int *) ws(c, WS_RANK, sizeof(int) * (size_t) (nobj > 0 ? nobj : 1), &err);
+  int *count = (int *) ws(c, WS_COUNT, sizeof(int) * (size_t) ncell, &err);
+  int *gstart = (int *) ws(c, WS_GSTART, sizeof(int) * ((size_t) ncell + 1), &err);
+  if (err) return err;
+  HIPCHK(c, hipMemcpyAsync(dst, st.data(), sizeof(FkStage) * st.size(), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(ncell, 256)), dim3(256), 0, c->stream, count, (int) ncell, 0);
+  if (nobj > 0)
+    hipLaunchKernelGGL(k_stage_assign, dim3(nblk(nobj, 256)), dim3(256), 0, c->stream, (int) nobj, nst, dst,
+                       cid, rank, count);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, count, gstart, (int) ncell);
+  hipLaunchKernelGGL(k_stage_starts, dim3(nblk(ncs, 256)), dim3(256), 0, c->stream, (int) ncs, nst, dst, gstart);
+  if (nobj > 0)
+    hipLaunchKernelGGL(k_stage_scatter, dim3(nblk(nobj, 256)), dim3(256), 0, c->stream, (int) nobj, nst, dst,
+                       cid, rank, gstart);
   HIPCHK(c, hipGetLastError());
+  /* the descriptors are read by the kernels above: keep the host copy until
+   * the copy has completed */
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  st.clear();
   return 0;
 }
 
@@ -1624,7 +1706,8 @@ struct Gather { const double *src; const int *idx; double *dst; int n; };
 
 static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
     const fcfc_cat *a, const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
-    int autoc, int wt, int mode, PJob *J, std::vector<Gather> *gathers = nullptr) {
+    int autoc, int wt, int mode, PJob *J, std::vector<FkStage> &stages,
+    std::vector<Gather> *gathers = nullptr) {
   const int need_s = (bins->geometry == FCFC_GEOM_SURVEY && bins->bintype != FCFC_BIN_ISO);
   /* cross counts are symmetric in the two catalogues: the larger one goes to
    * the lanes (fuller tiles, tighter windows and fewer lanes sharing one
@@ -1651,7 +1734,7 @@ static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
     if (found) continue;
     const int slot = (q == 0) ? WS_A_X : WS_B_X;
     const int upslot = (q == 0) ? WS_IN_X : WS_IN2_X;
-    if ((e = stage_cat(c, al, k, need_s, wt, J->P.g, slot, upslot, dst[q]))) return e;
+    if ((e = stage_prep(c, al, k, need_s, wt, J->P.g, slot, upslot, dst[q], stages))) return e;
     if (al.own) copies.push_back(CopyKey{k->x, k->n, wt, J->P.g, *dst[q]});
     if (gathers && wt && k->w && k->on_device && k->n > 0)
       gathers->push_back(Gather{k->w, dst[q]->idx, dst[q]->w, (int) k->n});
@@ -1724,13 +1807,17 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   const int nrow = rowoff.back();
   int *rown = (int *) al.get(WS_ROWN, sizeof(int) * (size_t) nrow);
   int *rowscan = (int *) al.get(WS_ROWOFF, sizeof(int) * ((size_t) nrow + 1));
+  FkTileJob *dtj = (FkTileJob *) al.get(WS_TJOBS, sizeof(FkTileJob) * jobs.size());
   if (al.err) return al.err;
+  std::vector<FkTileJob> htj(jobs.size());
   for (size_t k = 0; k < jobs.size(); k++) {
     const PJob &J = jobs[k];
-    const int nr = J.P.g.ncx * J.P.g.ncy;
-    hipLaunchKernelGGL(k_tile_count, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
-        J.P.T, J.P.zmaxc, rown + rowoff[k]);
+    htj[k] = FkTileJob{rowoff[k], J.P.g.ncz, J.P.T, J.P.zmaxc, J.P.nstrips, J.A.cs};
   }
+  HIPCHK(c, hipMemcpyAsync(dtj, htj.data(), sizeof(FkTileJob) * jobs.size(), hipMemcpyHostToDevice, c->stream));
+  const int njob = (int) jobs.size();
+  if (nrow > 0)
+    hipLaunchKernelGGL(k_tile_count, dim3(nblk(nrow, 64)), dim3(64), 0, c->stream, nrow, njob, dtj, rown);
   hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, rown, rowscan, nrow);
   int ntile0 = 0;
   HIPCHK(c, hipMemcpyAsync(&ntile0, rowscan + nrow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1752,12 +1839,9 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   const int ntiles = ntile0 * S;
   FkTile *tiles = (FkTile *) al.get(WS_TILES, sizeof(FkTile) * (size_t) (ntiles > 0 ? ntiles : 1));
   if (al.err) return al.err;
-  for (size_t k = 0; k < jobs.size(); k++) {
-    const PJob &J = jobs[k];
-    const int nr = J.P.g.ncx * J.P.g.ncy;
-    hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nr, 64)), dim3(64), 0, c->stream, nr, J.P.g.ncz, J.A.cs,
-        J.P.T, J.P.zmaxc, rowscan + rowoff[k], (int) k, J.P.nstrips, S, tiles);
-  }
+  if (nrow > 0)
+    hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nrow, 64)), dim3(64), 0, c->stream, nrow, njob, dtj,
+                       rowscan, S, tiles);
   /* ---- tables, strips, job descriptors ---- */
   size_t nstr = 0;
   for (auto &j : jobs) nstr += j.P.strips.size();
@@ -1939,16 +2023,18 @@ static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
                    ((flags & FCFC_GPU_PREC_PLAIN) ? MODE_P64 : MODE_C64);
   Alloc al(c, njob > 1);
   std::vector<CopyKey> copies;
+  std::vector<FkStage> stages;
   std::vector<PJob> jobs(njob);
   for (int k = 0; k < njob; k++) {
     const int autoc = (fl[k] & FCFC_GPU_AUTO) ? 1 : 0;
     const int wt = (fl[k] & FCFC_GPU_WEIGHTED) ? 1 : 0;
     if (!cat_ok(as[k]) || (!autoc && !cat_ok(bs[k]))) { set_err(c, "invalid catalogue in job %d", k); return FCFC_ERR_ARG; }
     if (bins->geometry == FCFC_GEOM_BOX && !bsz[k]) { set_err(c, "box size required"); return FCFC_ERR_ARG; }
-    int e = setup_job(c, al, copies, as[k], bs[k], bins, bsz[k], autoc, wt, mode, &jobs[k]);
+    int e = setup_job(c, al, copies, as[k], bs[k], bins, bsz[k], autoc, wt, mode, &jobs[k], stages);
     if (e) return e;
     jobs[k].hist_off = (long long) k * bins->ntot;
   }
+  if (int e = stage_run(c, stages)) return e;
   int err = 0;
   const size_t hbytes = 8 * (size_t) bins->ntot * njob;
   void *dh = ws(c, WS_HIST, hbytes, &err);
@@ -2025,11 +2111,12 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
   const int npass = (autoc || !gb) ? 1 : 2;
   Alloc al(c, npass > 1);
   std::vector<CopyKey> copies;
+  std::vector<FkStage> stages;
   std::vector<PJob> jobs(npass);
   int err = 0;
   for (int p = 0; p < npass; p++) {
     const fcfc_cat *pa = p ? b : a, *pb = p ? a : b;
-    int e = setup_job(c, al, copies, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &jobs[p]);
+    int e = setup_job(c, al, copies, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &jobs[p], stages);
     if (e) return e;
     /* second pass of a cross gradient: the count evaluated every pair with
      * the other catalogue as A */
@@ -2038,6 +2125,7 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
     if (al.err) return al.err;
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, jobs[p].gA, (int64_t) pa->n, 0.0);
   }
+  if (int e = stage_run(c, stages)) return e;
   double *dG = (double *) ws(c, WS_G, sizeof(double) * bins->ntot, &err);
   unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
   if (err) return err;
@@ -2108,15 +2196,18 @@ extern "C" int fcfc_gpu_plan_create(fcfc_gpu_ctx *c, int njob, const fcfc_batch_
   P->flags = flags;
   P->jobs.resize(njob);
   std::vector<CopyKey> copies;
+  std::vector<FkStage> stages;
   for (int k = 0; k < njob; k++) {
     const fcfc_batch_job &j = jobs[k];
     const int autoc = (j.flags & FCFC_GPU_AUTO) ? 1 : 0;
     const int wt = (j.flags & FCFC_GPU_WEIGHTED) ? 1 : 0;
     if (!cat_ok(&j.a) || (!autoc && !cat_ok(&j.b))) { set_err(c, "invalid catalogue in job %d", k); return FCFC_ERR_ARG; }
-    int e = setup_job(c, P->al, copies, &j.a, &j.b, bins, j.bsize, autoc, wt, mode, &P->jobs[k], &P->gathers);
+    int e = setup_job(c, P->al, copies, &j.a, &j.b, bins, j.bsize, autoc, wt, mode, &P->jobs[k], stages,
+                      &P->gathers);
     if (e) return e;
     P->jobs[k].hist_off = (long long) k * bins->ntot;
   }
+  if (int e = stage_run(c, stages)) return e;
   P->dh = P->al.get(WS_HIST, 8 * (size_t) bins->ntot * njob);
   P->misc = (unsigned long long *) P->al.get(WS_MISC, 64 * 8);
   if (P->al.err) return P->al.err;
