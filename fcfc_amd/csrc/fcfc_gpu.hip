@@ -211,9 +211,9 @@ __global__ void k_tile_emit(int nrow, int njob, const FkTileJob *__restrict__ tj
 
 /* FP32 exactness windows (see fp32_windows on the host). */
 struct FkWin {
-  float e_s2, e_s2_hi;        /* absolute window on s^2 (or s_perp^2), and  */
-                              /* 1 - e_s2                                   */
-  float e_pi, e_pi_hi;        /* absolute window on pi, and 1 - e_pi        */
+  float e_s2, e_s2x2;         /* absolute window on s^2 (or s_perp^2), 2 e_s2 */
+  float e_pi, e_pix2;         /* absolute window on pi, 2 e_pi                */
+  float idle_x, idle_z;       /* FP32 coordinates of idle lanes             */
   float mu_sh, mu_hi;         /* mu bin certain if fract(t - mu_sh) < mu_hi */
   float s2_floor;             /* pairs below this s^2 are re-evaluated      */
   float nmu_f;
@@ -222,9 +222,8 @@ struct FkWin {
 /* FP32 tables: entry = bin (low 16 bits) | flags; index (int) s2 - soff + 1
  * (entry 0: the integer cell below the table, entries len + 1 and len + 2:
  * above it, the last one without flags). */
-#define FK_F_LO   0x10000u    /* the lower end of the integer cell is an edge */
-#define FK_F_HI   0x20000u    /* the upper end is an edge                     */
-#define FK_F_ALW  0x40000u    /* hybrid cell: always re-evaluate              */
+#define FK_F_EDGE 0x10000u    /* a bin edge at the lower end of the integer cell */
+#define FK_F_ALW  0x20000u    /* always re-evaluate (with FK_F_EDGE set too)     */
 
 /* One counting job: a catalogue pair and its histogram.  A launch counts the
  * tiles of one or many jobs (batched mocks); tiles carry their job index. */
@@ -547,33 +546,36 @@ __device__ unsigned fk_box_variant(int var, const double a[3], const double bo[3
 /* Coordinates relative to the A tile centre in FP32.  Returns the histogram
  * index (>= ntot: not counted) and sets ambig when the pair lies within the
  * rounding bound of a bin edge that matters (it is then re-evaluated in
- * FP64 and the index is ignored).  The s^2 cell flags say which ends of the integer cell are edges,
- * so the window test is branch-free; a pair whose s (or pi) is certainly
- * out of range is never AMB. */
+ * FP64 and the index is ignored).  Window test of s^2 (and pi): with
+ * u = s2 + e, s2 is within e of an integer k exactly when fract(u) < 2e,
+ * and then k = (int) u; otherwise (int) u = (int) s2.  So the table entry
+ * at (int) u is the cell of s2 when s2 is clear of the integers, and the
+ * cell above the nearby integer otherwise, whose FK_F_EDGE flag says
+ * whether that integer is a bin edge (if not, both cells give the same
+ * bin).  Flags sit above the bin bits, ordered so the test is one compare:
+ * entry >= (near ? FK_F_EDGE : FK_F_ALW).  A pair whose s (or pi) is
+ * certainly out of range reads a flagless sentinel. */
 template <int BIN>
 __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     const FkBins &bn, const FkWin &wn, const unsigned *stab,
-    const unsigned *ptab, bool &ambig) {
+    const unsigned *ptab, bool &ambig, bool &ambig_mu) {
+  ambig_mu = false;
   float s2;
   if (BIN == FCFC_BIN_SPI) s2 = __builtin_fmaf(dy, dy, dx * dx);
   else s2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, dz * dz));
-  /* s2 >= 0: truncation is the floor.  The zero-based FP32 tables run to
-   * (int) s2min + len + 1; the entries next to the cells flag the edges at
-   * s2min and s2max, the others are plain FK_OUT for pairs certainly out of
-   * range */
-  const unsigned e = stab[min((int) s2, bn.soff + bn.stab_len + 1)];
-  const float f = __builtin_amdgcn_fractf(s2);
-  const unsigned msk = FK_F_ALW | (f < wn.e_s2 ? FK_F_LO : 0u) | (f > wn.e_s2_hi ? FK_F_HI : 0u);
-  const bool amb = (e & msk) != 0;
+  /* u >= 0: truncation is the floor.  The zero-based FP32 tables run to
+   * (int) s2min + len + 1: the entry after the last cell flags the edge at
+   * s2max, the last one is a plain FK_OUT for pairs certainly out of range
+   * (as are the entries below s2min) */
+  const float u = s2 + wn.e_s2;
+  const unsigned e = stab[min((int) u, bn.scap32)];
+  const bool amb = e >= (__builtin_amdgcn_fractf(u) < wn.e_s2x2 ? FK_F_EDGE : FK_F_ALW);
   const unsigned sb = e & 0xFFFFu;
   if (BIN == FCFC_BIN_ISO) { ambig = amb; return sb; }
   if (BIN == FCFC_BIN_SPI) {
-    const float pi = __builtin_fabsf(dz);
-    const float fp = __builtin_floorf(pi);
-    const unsigned ep = ptab[min((int) fp, bn.poff + bn.ptab_len + 1)];
-    const float g = pi - fp;
-    const unsigned mp = FK_F_ALW | (g < wn.e_pi ? FK_F_LO : 0u) | (g > wn.e_pi_hi ? FK_F_HI : 0u);
-    const bool ambp = (ep & mp) != 0;
+    const float up = __builtin_fabsf(dz) + wn.e_pi;
+    const unsigned ep = ptab[min((int) up, bn.pcap32)];
+    const bool ambp = ep >= (__builtin_amdgcn_fractf(up) < wn.e_pix2 ? FK_F_EDGE : FK_F_ALW);
     const unsigned pb = ep & 0xFFFFu;
     const bool out = ((sb == FK_OUT) & !amb) | ((pb == FK_OUT) & !ambp);
     ambig = (amb | ambp) & !out;
@@ -584,10 +586,12 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
    * t <= nmu (1 + 2^-20); the cells below s2_floor carry FK_F_ALW (every
    * pair there is re-evaluated).  With t_sh = t - mu_sh, fract(t_sh) <
    * mu_hi <= 1 - 2 mu_sh puts the exact t strictly inside (k, k + 1),
-   * k = floor(t_sh). */
+   * k = floor(t_sh).  A pair out of range in s whose t is near an integer
+   * is re-evaluated too (rare, and cheaper than testing sb here); idle
+   * lanes sit where t = 1/2. */
   const float t = __builtin_fmaf(wn.nmu_f * __builtin_fabsf(dz), __builtin_amdgcn_rsqf(s2), -wn.mu_sh);
-  const bool nearm = !(__builtin_amdgcn_fractf(t) < wn.mu_hi);
-  ambig = amb | (nearm & (sb != FK_OUT));      /* no short-circuit branches */
+  ambig = amb;                                 /* the caller ORs the two */
+  ambig_mu = !(__builtin_amdgcn_fractf(t) < wn.mu_hi);
   return sb + __umul24((unsigned) bn.nsh, (unsigned) (int) t);
 }
 
@@ -696,14 +700,16 @@ k_count(const FkArgs a) {
     auto cell = [](const uint16_t *t, int len, int c) -> unsigned {
       return (c >= 0 && c < len) ? (unsigned) t[c] : FK_OUT;
     };
-    /* alw_below: cells whose upper end lies below it are always re-evaluated */
+    /* alw_below > 0: cells whose lower end lies below alw_below + 1 are
+     * always re-evaluated (the extra cell covers pairs just below the first
+     * other cell, which the window test reads through that cell) */
     auto entry = [&](const uint16_t *t, int len, int nb, int c, double off, double alw_below) -> unsigned {
       const unsigned v = cell(t, len, c);
       if (!F32) return v;
       unsigned f = 0;
-      if (cell(t, len, c - 1) != v) f |= FK_F_LO;
-      if (cell(t, len, c + 1) != v) f |= FK_F_HI;
-      if (v != FK_OUT && (v >= (unsigned) nb || off + c < alw_below)) f |= FK_F_ALW | FK_F_LO | FK_F_HI;
+      if (cell(t, len, c - 1) != v) f |= FK_F_EDGE;
+      if (v != FK_OUT && (v >= (unsigned) nb || (alw_below > 0 && off + c < alw_below + 1.0)))
+        f |= FK_F_ALW | FK_F_EDGE;
       return v | f;
     };
     const double s2f = (F32 && BIN == FCFC_BIN_SMU) ? (double) a.jobs[0].win.s2_floor : 0.0;
@@ -795,9 +801,11 @@ k_count(const FkArgs a) {
       as[r] = NEED_S ? (v ? J.A.s[p] : 3.0 * FK_FAR * FK_FAR) : 0.0;
       aw[r] = (jw && v) ? J.A.w[p] : 1.0;
       gacc[r] = 0.0;
-      fax[r] = v ? (float) (ax[r] - tcx) : 1.0e18f;
-      fay[r] = v ? (float) (ay[r] - tcy) : 1.0e18f;
-      faz[r] = v ? (float) (az[r] - tcz) : 1.0e18f;
+      /* idle lanes: |a| = 1e18 and |dz| / s = 1 / (2 nmu), so that the mu
+       * estimate of their pairs is 1/2, clear of the windows */
+      fax[r] = v ? (float) (ax[r] - tcx) : J.win.idle_x;
+      fay[r] = v ? (float) (ay[r] - tcy) : 0.0f;
+      faz[r] = v ? (float) (az[r] - tcz) : J.win.idle_z;
     }
     const int nva = min(t.p1 - t.p0, R * FK_NT);
     const bool jwt = WT && J.wt;
@@ -809,9 +817,11 @@ k_count(const FkArgs a) {
     const unsigned long long lanemask_lt = (1ull << lane) - 1;
     auto drain = [&](int n) {
       for (int e0 = 0; e0 < n; e0 += 64) {
-        if (e0 + lane < n) {
-          const unsigned v = wq[e0 + lane];
-          const int p = t.p0 + (int) (v >> 8);
+        const unsigned v = (e0 + lane < n) ? wq[e0 + lane] : 0xFFFFFFFFu;
+        const int p = t.p0 + (int) (v >> 8);
+        /* entries of idle lanes (possible only when the windows cover
+         * everything) are dropped */
+        if (e0 + lane < n && p < t.p1) {
           const int j = (int) (v & 255u);
           const double qx = J.A.x[p], qy = J.A.y[p], qz = J.A.z[p];
           const unsigned idx = fk_recheck<BIN, HYB, MODE == MODE_C32C>(bn, J, qx, qy, qz, cj[j], cc[j],
@@ -888,16 +898,23 @@ k_count(const FkArgs a) {
         for (int r = 0; r < R; r++) {
           unsigned idx;
           if (F32) {
-            bool amb;
-            idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw, ptab_raw, amb);
-            if (SELF && !(jj > ai[r])) { idx = FK_NONE; amb = false; }
-            /* pairs near an edge: queue (A slot, chunk entry) for FP64 */
-            const unsigned long long m = __builtin_amdgcn_ballot_w64(amb);
+            bool amb, ambm;
+            idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw, ptab_raw,
+                                amb, ambm);
+            if (SELF) {
+              const bool keep = jj > ai[r];
+              amb = amb & keep; ambm = ambm & keep; idx = keep ? idx : FK_NONE;
+            }
+            /* pairs near an edge: queue (A slot, chunk entry) for FP64.  The
+             * two conditions are balloted apart (each ballot is its compare's
+             * own mask) and the lane's bit is read back from the mask inside
+             * the rare branch, so no lane value is kept across it */
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(amb) | __builtin_amdgcn_ballot_w64(ambm);
+            idx = (amb | ambm) ? FK_NONE : idx;
             if (m) {
-              if (amb) {
+              if ((m >> lane) & 1ull) {
                 const int pos = qn + __popcll(m & lanemask_lt);
                 wq[pos] = ((unsigned) (tid * R + r) << 8) | (unsigned) j;
-                idx = FK_NONE;
               }
               qn += __popcll(m);
               if (qn > FK_QCAP - 64) { drain(qn); qn = 0; }
@@ -1654,8 +1671,11 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double 
   FkWin &w = *win;
   w.e_s2 = (float) (es2 * 1.25 + 1e-7);
   w.e_pi = (float) ((u * (2 * pmx + E) + tiny) * 1.25 + 1e-7);
-  w.e_s2_hi = 1.0f - w.e_s2;
-  w.e_pi_hi = 1.0f - w.e_pi;
+  w.e_s2x2 = 2.0f * w.e_s2;
+  w.e_pix2 = 2.0f * w.e_pi;
+  const double cz = 0.5 / std::max(b->nmu, 1);
+  w.idle_z = (float) (1e18 * cz);
+  w.idle_x = (float) (1e18 * sqrt(1.0 - cz * cz));
   const double nmu = b->nmu;
   const double D = 1.25 * (nmu * 10 * u + (sqrt(3.0) * u * E * nmu + nmu * (u * E + tiny)) / sqrt(s2f))
                    + 4 * nmu * u;
@@ -1894,6 +1914,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   fb.bintype = b->bintype; fb.geometry = geom;
   fb.ns = b->ns; fb.nmu = b->nmu; fb.np = b->np; fb.ntot = b->ntot;
   fb.nsh = hist_stride(b); fb.ntoh = hist_slots(b);
+  fb.scap32 = b->soff + b->stab_len + 1; fb.pcap32 = b->poff + b->ptab_len + 1;
   fb.tabtype = b->tabtype; fb.soff = b->soff; fb.poff = b->poff;
   fb.stab_len = b->stab_len; fb.ptab_len = b->ptab_len;
   fb.s2min = b->s2bin[0]; fb.s2max = b->s2bin[b->ns];
