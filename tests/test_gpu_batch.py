@@ -86,3 +86,69 @@ def test_plan_graph_replays(eng, prec):
                     np.testing.assert_array_equal(got[k], ref[k])
     finally:
         plan.close()
+
+
+def test_batch_edge_catalogues(eng):
+    """One batched call sorts all its catalogues together (one launch per
+    stage over the concatenated objects): empty and one-object catalogues
+    between others, host and device catalogues, unit weights filled on the
+    device for a weighted count of an unweighted catalogue; every job is
+    checked against the oracle."""
+    import torch
+    from oracle import oracle as O
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=20, bsize=L)
+    f = b.rescale
+    ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu)
+    x, y, z, w = F.gen_clustered(31, 1500, 10, 50, 2.0, L, 0.3)
+    xr, yr, zr = F.gen_uniform(32, 2500, L)
+    D = F.Catalog(b, x, y, z, w)
+    R = F.Catalog(b, xr, yr, zr)
+    E = F.Catalog(b, np.zeros(0), np.zeros(0), np.zeros(0))
+    one = F.Catalog(b, np.array([500.0]), np.array([500.0]), np.array([500.0]))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a * f)).cuda()
+    Rd = F.Catalog(b, dev(xr), dev(yr), dev(zr), rescaled=True)      # device, no weights
+    jobs = [(E, None, True, False), (D, None, True, True), (one, R, False, False),
+            (E, R, False, True), (Rd, D, False, True), (R, None, True, False),
+            (one, None, True, False), (Rd, None, True, True)]
+    print(f"case batch edge catalogues jobs={len(jobs)}", flush=True)
+    got = eng.count_batch(jobs, b)
+    host = {id(Rd): R}
+    for k, (a, c, au, wt) in enumerate(jobs):
+        a = host.get(id(a), a)
+        c = host.get(id(c), c) if c is not None else None
+        # weighted counts give unweighted catalogues unit weights
+        # (build_tree.c:143-156); the oracle takes them explicitly
+        uw = lambda k: (k.w if k.w is not None else np.ones(k.n)) if wt else None
+        ta = (a.x, a.y, a.z, None, uw(a))
+        tc = None if au else (c.x, c.y, c.z, None, uw(c))
+        ref, cen = O.count(0, b.bsize_rescaled, ta, tc, au, ob, variant=1, weighted=bool(wt),
+                           census_mask=0 if wt else 0b11111)
+        if wt:
+            np.testing.assert_allclose(got[k], ref, rtol=1e-12, atol=0, err_msg=f"job {k}")
+        else:
+            diff = np.abs(got[k].astype(np.int64) - ref.astype(np.int64)).sum()
+            assert diff <= 2 * cen, f"job {k}: counts differ by {diff} (census {cen})"
+            if cen == 0:
+                np.testing.assert_array_equal(got[k], ref)
+
+
+def test_batch_survey_norms_on_device(eng):
+    """Survey (s, mu): |r|^2 computed on the GPU inside the batched sort
+    (device catalogues without s, with FCFC's lane rule) gives the counts of
+    the host-prepared catalogues."""
+    import torch
+    b = F.Bins.linear(F.GEOM_SURVEY, F.BIN_SMU, 0, 200, 5, nmu=10)
+    f = b.rescale
+    x, y, z, w = F.gen_survey(41, 3001)
+    xr, yr, zr, wr = F.gen_survey(42, 5003)
+    D = F.Catalog(b, x, y, z, w)
+    R = F.Catalog(b, xr, yr, zr, wr)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a * f)).cuda()
+    Dd = F.Catalog(b, dev(x), dev(y), dev(z), torch.from_numpy(w).cuda(), rescaled=True)
+    Rd = F.Catalog(b, dev(xr), dev(yr), dev(zr), torch.from_numpy(wr).cuda(), rescaled=True)
+    assert Dd.s is None and Rd.s is None
+    print("case batch survey norms", flush=True)
+    host = eng.count_batch([(D, None, True, False), (D, R, False, True)], b)
+    devc = eng.count_batch([(Dd, None, True, False), (Dd, Rd, False, True)], b)
+    np.testing.assert_array_equal(devc[0], host[0])
+    np.testing.assert_allclose(devc[1], host[1], rtol=1e-12, atol=0)
