@@ -176,12 +176,14 @@ def mocks_workload(args):
         log("autotuned on mocks of four sizes")
 
     def xi_all(res):
+        """Landy-Szalay xi(s, mu) and xi_0,2,4 of every mock (the same code
+        for the batched and the per-count results)."""
         out = []
         for k, (D, R) in enumerate(mocks):
             dd, dr, rr = res[3 * k], res[3 * k + 1], res[3 * k + 2]
-            ddn = 2 * dd / (D.sumw * (D.sumw - 1))
-            drn = dr / (D.sumw * R.n)
-            rrn = 2 * rr / (float(R.n) * (R.n - 1))
+            ddn = dd * (2 / (D.sumw * (D.sumw - 1)))
+            drn = dr * (1 / (D.sumw * R.n))
+            rrn = rr * (2 / (float(R.n) * (R.n - 1)))
             xi = (ddn - 2 * drn + rrn) / np.where(rrn > 0, rrn, np.inf)
             out.append(multipoles(xi.reshape(100, 40), 100, (0, 2, 4)))
         return out

@@ -325,7 +325,8 @@ class Engine:
 
     def count_batch(self, jobs, bins, precision="fp64"):
         """jobs: list of (a, b_or_None, isauto, withwt).  Returns one
-        histogram per job (float64 for weighted jobs, int64 otherwise)."""
+        histogram per job (float64 for weighted jobs, int64 otherwise); the
+        histograms are rows of one array allocated for the call."""
         n = len(jobs)
         arr = _job_array(jobs, bins)
         raw = np.zeros((max(n, 1), bins.ntot), dtype=np.int64)
@@ -335,7 +336,7 @@ class Engine:
         self._check(e, "fcfc_gpu_count_batch")
         out = []
         for k, (_, _, _, withwt) in enumerate(jobs):
-            out.append(raw[k].view(np.float64).copy() if withwt else raw[k].copy())
+            out.append(raw[k].view(np.float64) if withwt else raw[k])
         return out
 
     def stats(self):

@@ -9,6 +9,8 @@
   weighted pair count and whose backward is the GPU weight-gradient kernel
   (d cnt / d w, contracted with the incoming gradient).
 """
+import functools
+
 import numpy as np
 
 from . import Catalog
@@ -35,15 +37,20 @@ def legpoly(ell, x):
     raise ValueError("ell must be <= 6")
 
 
+@functools.lru_cache(maxsize=32)
+def _pole_weights(nmu, poles):
+    mu = (np.arange(nmu) + 0.5) / nmu
+    w = np.stack([(2 * ell + 1) / nmu * legpoly(ell, mu) for ell in poles])
+    w.setflags(write=False)
+    return w
+
+
 def multipoles(xi_smu, nmu, poles=(0, 2, 4)):
     """xi_ell(s) = (2 ell + 1)/nmu sum_j xi(s, mu_j) P_ell(mu_j), mu_j the bin
-    midpoints (eval_cf.c:326-354).  xi_smu: (..., nmu, ns)."""
-    mu = (np.arange(nmu) + 0.5) / nmu
-    out = []
-    for ell in poles:
-        w = (2 * ell + 1) / nmu * legpoly(ell, mu)
-        out.append(np.tensordot(xi_smu, w, axes=([-2], [0])))
-    return np.stack(out, axis=-2)
+    midpoints (eval_cf.c:326-354).  xi_smu: (..., nmu, ns); returns
+    (..., len(poles), ns), one matrix product with the cached weights."""
+    with np.errstate(invalid="ignore", over="ignore"):      # NaN bins propagate
+        return np.matmul(_pole_weights(nmu, tuple(poles)), xi_smu)
 
 
 def wp(xi_spi, pbin_raw):
