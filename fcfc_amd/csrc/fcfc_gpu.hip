@@ -702,22 +702,28 @@ k_count(const FkArgs a) {
     };
     /* alw_below > 0: cells whose lower end lies below alw_below + 1 are
      * always re-evaluated (the extra cell covers pairs just below the first
-     * other cell, which the window test reads through that cell) */
-    auto entry = [&](const uint16_t *t, int len, int nb, int c, double off, double alw_below) -> unsigned {
+     * other cell, which the window test reads through that cell).  Hybrid
+     * tables (lut_hybrid, setup_cf.c) give the cell holding a non-integer
+     * lower range edge the first bin and leave values below the edge to the
+     * lookup's explicit range check, so that cell is always re-evaluated as
+     * well (refinement cells, v >= nb, always are). */
+    auto entry = [&](const uint16_t *t, int len, int nb, int c, double off, double alw_below,
+                     double lo_edge) -> unsigned {
       const unsigned v = cell(t, len, c);
       if (!F32) return v;
       unsigned f = 0;
       if (cell(t, len, c - 1) != v) f |= FK_F_EDGE;
-      if (v != FK_OUT && (v >= (unsigned) nb || (alw_below > 0 && off + c < alw_below + 1.0)))
+      if (v != FK_OUT && (v >= (unsigned) nb || (alw_below > 0 && off + c < alw_below + 1.0) ||
+                          (HYB && c == 0 && off != lo_edge)))
         f |= FK_F_ALW | FK_F_EDGE;
       return v | f;
     };
     const double s2f = (F32 && BIN == FCFC_BIN_SMU) ? (double) a.jobs[0].win.s2_floor : 0.0;
     for (int i = tid; i < a.stab_c0 + bn.stab_len + 2; i += FK_NT)
-      stab_raw[i] = entry(bn.stab, bn.stab_len, bn.ns, i - a.stab_c0, bn.soff, s2f);
+      stab_raw[i] = entry(bn.stab, bn.stab_len, bn.ns, i - a.stab_c0, bn.soff, s2f, bn.s2min);
     if (BIN == FCFC_BIN_SPI)
       for (int i = tid; i < a.ptab_c0 + bn.ptab_len + 2; i += FK_NT)
-        ptab_raw[i] = entry(bn.ptab, bn.ptab_len, bn.np, i - a.ptab_c0, bn.poff, 0.0);
+        ptab_raw[i] = entry(bn.ptab, bn.ptab_len, bn.np, i - a.ptab_c0, bn.poff, 0.0, bn.pmin);
   }
   if (HYB) {
     for (int i = tid; i <= bn.ns; i += FK_NT) s2e[i] = bn.s2bin[i];

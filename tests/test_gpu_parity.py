@@ -139,18 +139,41 @@ def test_lattice_mu_edges(eng, nmu):
         np.testing.assert_array_equal(gl, ref)
 
 
-def test_hybrid_bins(eng, cats):
+@pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
+def test_hybrid_bins(eng, cats, bt):
     """Non-integer bin edges take FCFC's hybrid lookup table route."""
     (x, y, z, w), (xr, yr, zr) = cats
-    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0.5, 146.5, 7.3, nmu=7, bsize=L)
+    b = F.Bins.linear(F.GEOM_BOX, bt, 0.5, 146.5, 7.3, nmu=7 if bt == F.BIN_SMU else 1,
+                      pmin=1.5, pmax=96.5, dpi=3.7, bsize=L)
     assert b.tabtype == 1
     d = F.Catalog(b, x, y, z, w)
     r = F.Catalog(b, xr, yr, zr)
+    ref, cen = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None),
+                       (r.x, r.y, r.z, None, None), False, _obins(b), variant=1,
+                       census_mask=0b11111)
     for prec in ("fp64", "fp32", "plain"):
+        print(f"case hybrid bt={bt} prec={prec}", flush=True)
         g = eng.count_pairs(d, r, b, isauto=False, withwt=False, precision=prec)
-        ref, _ = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None),
-                         (r.x, r.y, r.z, None, None), False, _obins(b), variant=1)
-        _cmp(g, ref, False)
+        _cmp(g, ref, False, cen)
+
+
+@pytest.mark.parametrize("nmu", [255, 3])
+def test_mu_bins_many_and_few(eng, cats, nmu):
+    """The finest mu binning the reference allows (MAX_NMU = 255,
+    define_comm.h:148) and a coarse one: the mu windows scale with nmu."""
+    (x, y, z, w), (xr, yr, zr) = cats
+    b = _bins(F.GEOM_BOX, F.BIN_SMU, nmu=nmu)
+    d = F.Catalog(b, x, y, z, w)
+    r = F.Catalog(b, xr, yr, zr)
+    ref, cen = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None),
+                       (r.x, r.y, r.z, None, None), False, _obins(b), variant=1,
+                       census_mask=0b11111)
+    refa, cena = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None), None, True,
+                         _obins(b), variant=1, census_mask=0b11111)
+    for prec in ("fp64", "fp32", "plain"):
+        print(f"case nmu={nmu} prec={prec}", flush=True)
+        _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=False, precision=prec), ref, False, cen)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), refa, False, cena)
 
 
 @pytest.mark.parametrize("bt,smin,smax,ds,pmin,pmax,dpi,soff", [
