@@ -42,6 +42,30 @@ def test_backward_matches_fp64_oracle(eng, setup, isauto):
         np.testing.assert_allclose(gb, rb, rtol=1e-6, atol=1e-9 * np.abs(rb).max())
 
 
+@pytest.mark.parametrize("bt", [F.BIN_SMU, F.BIN_SPI])
+def test_backward_hybrid_and_spi(eng, bt):
+    """The gradient kernel with hybrid (non-integer) tables and with
+    (s_perp, pi) bins, cross and auto, against the FP64 oracle."""
+    b = F.Bins.linear(F.GEOM_BOX, bt, 0.5, 146.5, 7.3, nmu=7 if bt == F.BIN_SMU else 1,
+                      pmin=1.5, pmax=96.5, dpi=3.7, bsize=L)
+    assert b.tabtype == 1
+    x, y, z, w = F.gen_clustered(41, 1200, 15, 30, 2.0, L, 0.3)
+    xr, yr, zr = F.gen_uniform(42, 2000, L)
+    wr = np.linspace(0.7, 1.3, len(xr))
+    D = F.Catalog(b, x, y, z, w)
+    R = F.Catalog(b, xr, yr, zr, wr)
+    G = np.random.default_rng(5).normal(size=b.ntot)
+    ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu, b.pbin if bt == F.BIN_SPI else None)
+    for isauto in (True, False):
+        print(f"case backward hybrid bt={bt} isauto={isauto}", flush=True)
+        ga, gb = eng.count_backward(D, None if isauto else R, b, G, isauto=isauto)
+        ra, rb = O.weight_grad(0, b.bsize_rescaled, (D.x, D.y, D.z, None, D.w),
+                               None if isauto else (R.x, R.y, R.z, None, R.w), isauto, ob, G)
+        np.testing.assert_allclose(ga, ra, rtol=1e-6, atol=1e-9 * np.abs(ra).max())
+        if not isauto:
+            np.testing.assert_allclose(gb, rb, rtol=1e-6, atol=1e-9 * np.abs(rb).max())
+
+
 def test_backward_finite_difference(eng, setup):
     b, D, R, G = setup
     ga, gb = eng.count_backward(D, R, b, G, isauto=False)

@@ -117,6 +117,26 @@ def test_survey(eng, bt, prec):
     _cmp(gx, refx, True)
 
 
+@pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
+def test_survey_hybrid(eng, bt):
+    """Survey geometry with hybrid (non-integer) tables, every form."""
+    x, y, z, w = F.gen_survey(23, 3000)
+    xr, yr, zr, wr = F.gen_survey(24, 4000)
+    b = F.Bins.linear(F.GEOM_SURVEY, bt, 0.5, 146.5, 7.3, nmu=7 if bt == F.BIN_SMU else 1,
+                      pmin=1.5, pmax=96.5, dpi=3.7)
+    assert b.tabtype == 1
+    d = F.Catalog(b, x, y, z, w)
+    r = F.Catalog(b, xr, yr, zr, wr)
+    ob = _obins(b)
+    refa, cen = O.count(1, None, (d.x, d.y, d.z, d.s, None), None, True, ob, variant=0,
+                        census_mask=0b111111)
+    refx, _ = O.count(1, None, _otuple(d), _otuple(r), False, ob, variant=0, weighted=True)
+    for prec in ("fp64", "fp32", "plain"):
+        print(f"case survey hybrid bt={bt} prec={prec}", flush=True)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), refa, False, cen)
+        _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec), refx, True)
+
+
 @pytest.mark.parametrize("nmu", [100, 7, 12])
 def test_lattice_mu_edges(eng, nmu):
     """Objects on an integer lattice (in rescaled units): many pairs sit
