@@ -1267,7 +1267,12 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
     while (cd > 1 && objs_per_col / (cd * cd) < 4.0) cd--;
   }
   if (zd <= 0) zd = 8;
-  double w = Rxy / cd;
+  /* cells of Rxy / cd, but never more cells than about 4 per object
+   * (between 2^18 and 2^25): a sparse catalogue in a large volume gets
+   * larger cells instead of a grid of mostly empty ones */
+  const double cap = std::min(std::max(4.0 * (double) (nA + nB), 262144.0), 33554432.0);
+  const double w_cap = cbrt(ext[0] * ext[1] * ext[2] * zd / cap);
+  double w = std::max(Rxy / cd, w_cap);
   if (b->geometry == FCFC_GEOM_BOX) {
     g.periodic = 1;
     g.Lx = bsize[0]; g.Ly = bsize[1]; g.Lz = bsize[2];

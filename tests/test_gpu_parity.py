@@ -171,10 +171,34 @@ def test_hybrid_bins(eng, cats, bt):
     ref, cen = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None),
                        (r.x, r.y, r.z, None, None), False, _obins(b), variant=1,
                        census_mask=0b11111)
+    refw, _ = O.count(0, b.bsize_rescaled, _otuple(d), (r.x, r.y, r.z, None, None), False,
+                      _obins(b), variant=1, weighted=True)
     for prec in ("fp64", "fp32", "plain"):
         print(f"case hybrid bt={bt} prec={prec}", flush=True)
         g = eng.count_pairs(d, r, b, isauto=False, withwt=False, precision=prec)
         _cmp(g, ref, False, cen)
+        _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec), refw, True)
+
+
+@pytest.mark.parametrize("bt", [F.BIN_SMU, F.BIN_SPI])
+def test_large_box_small_bins(eng, bt):
+    """A box of 20000 Mpc/h with separations up to 20 Mpc/h (coordinates
+    three orders of magnitude above the bins): FP32 windows are wide
+    relative to the bins there, more pairs are re-evaluated, counts stay
+    exact."""
+    Lb = 20000.0
+    x, y, z, w = F.gen_clustered(51, 3000, 300, 10, 1.0, Lb, 0.3)
+    b = F.Bins.linear(F.GEOM_BOX, bt, 0, 20, 0.5, nmu=50 if bt == F.BIN_SMU else 1,
+                      pmin=0, pmax=20 if bt == F.BIN_SPI else 0, dpi=0.5, bsize=Lb)
+    d = F.Catalog(b, x, y, z, w)
+    ref, cen = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None), None, True, _obins(b),
+                       variant=1, census_mask=0b11111)
+    refw, _ = O.count(0, b.bsize_rescaled, _otuple(d), None, True, _obins(b), variant=1,
+                      weighted=True)
+    for prec in ("fp64", "fp32", "plain"):
+        print(f"case large box bt={bt} prec={prec}", flush=True)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), ref, False, cen)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=True, precision=prec), refw, True)
 
 
 @pytest.mark.parametrize("nmu", [255, 3])
