@@ -180,6 +180,26 @@ def test_hybrid_bins(eng, cats, bt):
         _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec), refw, True)
 
 
+@pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
+def test_non_cubic_box(eng, bt):
+    """A periodic box with three different side lengths (per-axis wrap)."""
+    Lb = np.array([1000.0, 700.0, 450.0])
+    rng = np.random.default_rng(61)
+    x, y, z = (rng.random(6000) * Lb[k] for k in range(3))
+    w = rng.lognormal(0, 0.3, 6000)
+    b = F.Bins.linear(F.GEOM_BOX, bt, 0, 150, 5, nmu=20 if bt == F.BIN_SMU else 1,
+                      pmin=0, pmax=150 if bt == F.BIN_SPI else 0, dpi=5, bsize=Lb)
+    d = F.Catalog(b, x, y, z, w)
+    ref, cen = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None), None, True, _obins(b),
+                       variant=1, census_mask=0b11111)
+    refw, _ = O.count(0, b.bsize_rescaled, _otuple(d), None, True, _obins(b), variant=1,
+                      weighted=True)
+    for prec in ("fp64", "fp32", "plain"):
+        print(f"case non-cubic box bt={bt} prec={prec}", flush=True)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), ref, False, cen)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=True, precision=prec), refw, True)
+
+
 @pytest.mark.parametrize("bt", [F.BIN_SMU, F.BIN_SPI])
 def test_large_box_small_bins(eng, bt):
     """A box of 20000 Mpc/h with separations up to 20 Mpc/h (coordinates
