@@ -180,6 +180,35 @@ def test_hybrid_bins(eng, cats, bt):
         _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec), refw, True)
 
 
+@pytest.mark.parametrize("geom", [F.GEOM_BOX, F.GEOM_SURVEY])
+def test_coincident_points(eng, geom):
+    """Duplicated objects (s = 0: the reference puts such pairs in the first
+    mu bin, metric_common.c:184) and near-duplicates 1e-9 apart."""
+    if geom == F.GEOM_BOX:
+        x, y, z, w = F.gen_clustered(71, 1500, 10, 20, 2.0, L, 0.3)
+    else:
+        x, y, z, w = F.gen_survey(72, 2000)
+    n = len(x)
+    k = np.arange(0, n, 7)
+    x = np.concatenate([x, x[k], x[k[::2]] + 1e-9])
+    y = np.concatenate([y, y[k], y[k[::2]]])
+    z = np.concatenate([z, z[k], z[k[::2]]])
+    w = np.concatenate([w, w[k], w[k[::2]]])
+    for bt in (F.BIN_SMU, F.BIN_SPI):
+        b = _bins(geom, bt)
+        d = F.Catalog(b, x, y, z, w)
+        ob = _obins(b)
+        g0 = 0 if geom == F.GEOM_BOX else 1
+        var = 1 if geom == F.GEOM_BOX else 0
+        ref, cen = O.count(g0, b.bsize_rescaled if geom == F.GEOM_BOX else None,
+                           (d.x, d.y, d.z, d.s, None), None, True, ob, variant=var,
+                           census_mask=0b111111)
+        precs = ("fp64", "fp32", "plain") if geom == F.GEOM_BOX else ("fp64", "plain")
+        for prec in precs:
+            print(f"case coincident geom={geom} bt={bt} prec={prec}", flush=True)
+            _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), ref, False, cen)
+
+
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
 def test_non_cubic_box(eng, bt):
     """A periodic box with three different side lengths (per-axis wrap)."""
