@@ -109,6 +109,7 @@ def cpu_baseline(target_s=15.0):
 
 
 PROFILE = os.path.join(ROOT, "profiles", "k_count_smu_box_RR_counters.txt")
+PROFILE32 = os.path.join(ROOT, "profiles", "k_count_smu_box_RR_fp32_counters.txt")
 
 
 def traffic_from_profile():
@@ -124,17 +125,20 @@ def traffic_from_profile():
     return None
 
 
-def issue_from_profile():
+def issue_from_profile(fn=PROFILE):
     """VALU busy and VALU issue-slot fractions of the RR count kernel from the
-    same summary (rocprofv3 SQ counters); None when absent."""
+    same summary (rocprofv3 SQ counters); for the FP32 form's summary the
+    instruction rate against the FP32 vector peak.  None when absent."""
     out = {}
     try:
-        for l in open(PROFILE):
+        for l in open(fn):
             t = l.split()
             if t[:1] == ["VALUBusy"]:
                 out["valu_busy"] = float(t[-2]) / 100
             elif t[:2] == ["issue", "slots"]:
                 out["valu_issue_slots"] = float(t[-2]) / 100
+            elif t[:1] == ["measured"] and "%" in t:
+                out["valu_issue_rate_vs_fp32_peak"] = float(t[t.index("%") - 1]) / 100
     except (OSError, ValueError, IndexError):
         return None
     return out or None
@@ -503,6 +507,12 @@ def main():
                         "RR_identical": eq["RR"], "weighted_max_rel": wrel}
         if other == "fp32":
             forms[other]["rechecked_fraction"] = rco / max(evo, 1)
+            ach32 = ro["RR_evals"] * FLOPS_PAIR["smu"] / (kmo["RR"] * 1e-3)
+            forms[other]["roofline"] = {
+                "bound": "valu", "achieved": ach32 / 1e12, "peak": PEAK_FP32_VALU / 1e12,
+                "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32_VALU,
+                "kernel": "k_count (s,mu) box fp32, RR auto count",
+                "issue_profile": issue_from_profile(PROFILE32)}
         log(f"{other} step done: {evo / dto:.4g} evals/s")
     line["forms"] = forms
     if "plain" in forms:
