@@ -140,18 +140,34 @@ __global__ void k_stage_scatter(int nobj, int nst, const FkStage *__restrict__ s
   }
 }
 
+/* Work items of the tile list: tile ti becomes S items (each a share of the
+ * strip list) for ti < tt and St > S finer items from tt on, so that the
+ * items the persistent workgroups take last are short and the launch ends
+ * evenly; tiles from ncap on are left out (the autotuner's samples). */
+struct FkSplit { int S, tt, St, ncap; };
+
+__host__ __device__ inline long long split_pos(const FkSplit &s, int ti) {
+  return ti < s.tt ? (long long) s.S * ti
+                   : (long long) s.S * s.tt + (long long) s.St * (ti - s.tt);
+}
+
 /* Tiles of one row: consecutive cells, at most T objects, at most zmaxc
- * z-cells; a cell with more than T objects is split into T-sized tiles. */
+ * z-cells; a cell with more than T objects is split into T-sized tiles.
+ * t0: index of the row's first tile in the tile list. */
 __device__ int row_tiles(const int *__restrict__ cs, int ncz, int T, int zmaxc,
-    int row, int job, int nstrips, int S, FkTile *out) {
+    int row, int job, int nstrips, const FkSplit &sp, int t0, FkTile *out) {
   int nt = 0, open = 0, tp0 = 0, tcnt = 0, tz0 = 0, tz1 = 0;
-  /* emit tile (p0, p1, iz0, iz1) as S items over the strip list */
+  /* emit tile (p0, p1, iz0, iz1) as items over the strip list */
   auto emit = [&](int p0, int p1, int iz0, int iz1) {
-    if (out)
+    const int ti = t0 + nt;
+    if (out && ti < sp.ncap) {
+      const int S = ti < sp.tt ? sp.S : sp.St;
+      FkTile *o = out + split_pos(sp, ti);
       for (int q = 0; q < S; q++)
-        out[nt * S + q] = FkTile{row, p0, p1, iz0, iz1, job,
-                                 (int) ((long long) nstrips * q / S),
-                                 (int) ((long long) nstrips * (q + 1) / S)};
+        o[q] = FkTile{row, p0, p1, iz0, iz1, job,
+                      (int) ((long long) nstrips * q / S),
+                      (int) ((long long) nstrips * (q + 1) / S)};
+    }
     nt++;
   };
   for (int iz = 0; iz < ncz; iz++) {
@@ -190,18 +206,20 @@ __global__ void k_tile_count(int nrow, int njob, const FkTileJob *__restrict__ t
   for (int gr = blockIdx.x * blockDim.x + threadIdx.x; gr < nrow; gr += gridDim.x * blockDim.x) {
     const FkTileJob t = tj[fk_tile_job(tj, njob, gr)];
     const int r = gr - t.row0;
-    ntile[gr] = row_tiles(t.cs + (size_t) r * t.ncz, t.ncz, t.T, t.zmaxc, r, 0, 0, 1, nullptr);
+    ntile[gr] = row_tiles(t.cs + (size_t) r * t.ncz, t.ncz, t.T, t.zmaxc, r, 0, 0,
+                          FkSplit{1, 0, 1, 0}, 0, nullptr);
   }
 }
-/* items of row gr start at S * toff[gr] */
+/* the tiles of row gr are numbered from toff[gr] */
 __global__ void k_tile_emit(int nrow, int njob, const FkTileJob *__restrict__ tj,
-    const int *__restrict__ toff, int S, FkTile *__restrict__ tiles) {
+    const int *__restrict__ toff, FkSplit sp, FkTile *__restrict__ tiles) {
   for (int gr = blockIdx.x * blockDim.x + threadIdx.x; gr < nrow; gr += gridDim.x * blockDim.x) {
+    if (toff[gr] >= sp.ncap) continue;
     const int k = fk_tile_job(tj, njob, gr);
     const FkTileJob t = tj[k];
     const int r = gr - t.row0;
-    row_tiles(t.cs + (size_t) r * t.ncz, t.ncz, t.T, t.zmaxc, r, k, t.nstrips, S,
-              tiles + (size_t) S * toff[gr]);
+    row_tiles(t.cs + (size_t) r * t.ncz, t.ncz, t.T, t.zmaxc, r, k, t.nstrips, sp,
+              toff[gr], tiles);
   }
 }
 
@@ -1084,6 +1102,7 @@ struct fcfc_gpu_ctx {
   int t_nh = 0;                /* LDS histogram copies (0: by occupancy) */
   int census = 0;
   int tile_cap = 0;            /* autotuner: count only the first tiles */
+  /* tiles (before their split into work items) of the last launch built */
   int last_ntiles = 0;
   struct Tuned { int geom, bintype, mode, autoc, wt, bucket, cd, zd, T; };
   std::vector<Tuned> tuned;    /* autotuner results per density bucket */
@@ -1859,20 +1878,28 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *) fn, FK_NT, L.total));
   if (occ < 1) occ = 1;
   const int nwg = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
-  int S = 1;
+  /* the autotuner's samples count only the first tile_cap tiles */
+  const int ncap = (c->tile_cap > 0 && c->tile_cap < ntile0) ? c->tile_cap : ntile0;
+  FkSplit sp{1, ncap, 1, ncap};
   if (ntile0 > 0) {
     int minstr = 1 << 30;
     for (auto &j : jobs) minstr = std::min(minstr, std::max(j.P.nstrips, 1));
-    S = (int) std::min<long long>(std::max<long long>(1, (16LL * nwg + ntile0 - 1) / ntile0),
-                                  std::min(minstr, 64));
-    if ((long long) S * ntile0 > (1LL << 30)) S = 1;
+    const int smax = std::min(minstr, 64);
+    sp.S = (int) std::min<long long>(std::max<long long>(1, (16LL * nwg + ntile0 - 1) / ntile0),
+                                     smax);
+    /* the tiles of about two items per workgroup at the end of the list
+     * are split four times finer */
+    sp.St = std::min(4 * sp.S, smax);
+    sp.tt = (int) std::max<long long>(0, ncap - (2LL * nwg + sp.S - 1) / sp.S);
+    if (sp.St <= sp.S) { sp.St = sp.S; sp.tt = ncap; }
+    if (split_pos(sp, ncap) > (1LL << 30)) sp = FkSplit{1, ncap, 1, ncap};
   }
-  const int ntiles = ntile0 * S;
+  const int ntiles = (int) split_pos(sp, ncap);
   FkTile *tiles = (FkTile *) al.get(WS_TILES, sizeof(FkTile) * (size_t) (ntiles > 0 ? ntiles : 1));
   if (al.err) return al.err;
   if (nrow > 0)
     hipLaunchKernelGGL(k_tile_emit, dim3(nblk(nrow, 64)), dim3(64), 0, c->stream, nrow, njob, dtj,
-                       rowscan, S, tiles);
+                       rowscan, sp, tiles);
   /* ---- tables, strips, job descriptors ---- */
   size_t nstr = 0;
   for (auto &j : jobs) nstr += j.P.strips.size();
@@ -1957,10 +1984,9 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   a.off_chunk = L.off_chunk; a.off_queue = L.off_queue;
   a.census = c->census;
   a.G = G;
-  if (c->tile_cap > 0 && c->tile_cap < ntiles) a.ntiles = c->tile_cap;
   int nblock = nwg;
   if (nblock > a.ntiles) nblock = std::max(a.ntiles, 1);
-  c->last_ntiles = ntiles;
+  c->last_ntiles = ntile0;
   c->stats.tile = jobs.empty() ? 0 : jobs[0].P.T;
   c->stats.cell_div = jobs.empty() ? 0 : (int) lround(sqrt(jobs[0].P.g.Rxy2) / jobs[0].P.g.wx);
   out->fn = fn; out->nblock = nblock; out->lds = L.total; out->ntiles = a.ntiles; out->a = a;
