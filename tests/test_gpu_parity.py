@@ -315,6 +315,37 @@ def test_launch_shapes_identical(eng, cats):
         np.testing.assert_array_equal(res[k], res[k % 2])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_work_item_split(eng, cats, prec):
+    """Few persistent workgroups and small tiles, so that the tile list is cut
+    into coarse work items and a finer tail at many different boundaries
+    (with one workgroup the finer tail is the last two tiles): weighted
+    cross, unweighted auto and the weight gradient against the oracle."""
+    (x, y, z, w), (xr, yr, zr) = cats
+    b = _bins(F.GEOM_BOX, F.BIN_SMU)
+    d = F.Catalog(b, x, y, z, w)
+    r = F.Catalog(b, xr, yr, zr)
+    ob = _obins(b)
+    refw, _ = O.count(0, b.bsize_rescaled, _otuple(d), (r.x, r.y, r.z, None, None), False,
+                      ob, variant=1, weighted=True)
+    refu, cen = O.count(0, b.bsize_rescaled, (d.x, d.y, d.z, None, None), None, True, ob,
+                        variant=1, census_mask=0b11111)
+    G = np.random.default_rng(7).normal(size=b.ntot)
+    refg, _ = O.weight_grad(0, b.bsize_rescaled, _otuple(d), None, True, ob, G)
+    try:
+        for nb in (1, 2, 5, 33, 160):
+            eng.set_tuning(3, 64, 4, nb)
+            print(f"case work items prec={prec} nblock={nb}", flush=True)
+            _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec), refw, True)
+            _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec),
+                 refu, False, cen)
+            if prec == "fp64":
+                ga, _ = eng.count_backward(d, None, b, G, isauto=True)
+                np.testing.assert_allclose(ga, refg, rtol=1e-6, atol=1e-9 * np.abs(refg).max())
+    finally:
+        eng.set_tuning(0, 0, 0, 0)
+
+
 def test_empty_and_tiny(eng):
     b = _bins(F.GEOM_BOX, F.BIN_ISO)
     e = F.Catalog(b, np.zeros(0), np.zeros(0), np.zeros(0))
