@@ -424,10 +424,10 @@ int main(int argc, char **argv) {
   for (int i = 0; i < ntype; i++) if (atoi(types[i]) != 0) { P_ERR("only ASCII catalogues (CATALOG_TYPE 0) are supported\n"); FAIL(FCFC_ERR_CFG); }
   if (kv_get("SELECTION")) {
     char **sel; int ns = split_list(kv_get("SELECTION"), &sel);
-    for (int i = 0; i < ns; i++) if (sel[i][0]) { P_ERR("SELECTION is not supported by this build\n"); free_list(sel, ns); FAIL(FCFC_ERR_CFG); }
+    for (int i = 0; i < ns; i++) if (sel[i][0]) { P_ERR("SELECTION is not supported (catalogues are read whole)\n"); free_list(sel, ns); FAIL(FCFC_ERR_CFG); }
     free_list(sel, ns);
   }
-  if (get_bool("COORD_CONVERT", 0)) { P_ERR("COORD_CONVERT is not supported by this build\n"); FAIL(FCFC_ERR_CFG); }
+  if (get_bool("COORD_CONVERT", 0)) { P_ERR("COORD_CONVERT is not supported (give comoving Cartesian coordinates)\n"); FAIL(FCFC_ERR_CFG); }
   if (npc < 1 || npco != npc) { P_ERR("PAIR_COUNT and PAIR_COUNT_FILE must be set with equal lengths\n"); FAIL(FCFC_ERR_CFG); }
   if (ncf != ncfo) { P_ERR("CF_ESTIMATOR and CF_OUTPUT_FILE must have equal lengths\n"); FAIL(FCFC_ERR_CFG); }
   const int bintype = get_int("BINNING_SCHEME", 0);
