@@ -227,6 +227,27 @@ __global__ void k_tile_emit(int nrow, int njob, const FkTileJob *__restrict__ tj
 /*                             count kernel                                 */
 /* ======================================================================== */
 
+/* v of lane (lane with the DPP pattern CTRL applied), both halves moved */
+template <int CTRL>
+__device__ __forceinline__ double fk_dpp(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int) (unsigned) u, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int) (unsigned) (u >> 32), CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((unsigned long long) (unsigned) hi << 32) | (unsigned) lo);
+}
+
+/* Sum of v over each row of 16 lanes, in every lane of the row: quad
+ * butterflies (lane ^ 1, lane ^ 2), then the row's half mirror (lane i of
+ * 8 with 7 - i) and mirror (lane i of 16 with 15 - i).  Every lane of the
+ * wavefront must be active. */
+__device__ __forceinline__ double fk_row_sum(double v) {
+  v += fk_dpp<0xB1>(v);       /* quad_perm [1, 0, 3, 2] */
+  v += fk_dpp<0x4E>(v);       /* quad_perm [2, 3, 0, 1] */
+  v += fk_dpp<0x141>(v);      /* row_half_mirror */
+  v += fk_dpp<0x140>(v);      /* row_mirror */
+  return v;
+}
+
 /* FP32 exactness windows (see fp32_windows on the host). */
 struct FkWin {
   float e_s2, e_s2x2;         /* absolute window on s^2 (or s_perp^2), 2 e_s2 */
@@ -253,9 +274,10 @@ struct FkJob {
   int autoc;                  /* COUNT: A == B, half shell + self tile;
                                  GRAD: A == B, exclude j == i          */
   int wt;                     /* weighted histogram (double)          */
-  int swap_all;               /* GRAD: every pair oriented (a - D) - b  */
   long long hist_off;         /* histogram of this job: gcnt + hist_off */
-  double *gA;                 /* GRAD: output, original object order  */
+  double *gA;                 /* GRAD: output for A objects, original order (or null) */
+  double *gB;                 /* GRAD: output for B objects, original order (or null;
+                                 auto counts: the same array as gA)   */
   FkWin win;                  /* FP32 error model (MODE_C32)          */
 };
 
@@ -288,7 +310,6 @@ struct FkStripLds {
 /* Compute the (up to three) B segments of one strip for tile t. */
 __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
     int dy, int diag, int *b0, int *len, int *code) {
-  const int diag_in = diag;
   const FkGrid &g = J.g;
   b0[0] = b0[1] = b0[2] = 0;
   len[0] = len[1] = len[2] = 0;
@@ -309,9 +330,7 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
   const double zc1 = (double) (t.iz1 + 1) + hz / g.wz + 1e-6;
   int izlo = (int) floor(zc0), izhi = (int) floor(zc1);
   const int *cs = J.B.cell_start + (size_t) (bx * g.ncy + by) * g.ncz;
-  const int swapf = (diag & 2) ? 32 : 0;   /* gradient: count evaluates from the other tile */
-  diag &= 1;
-  const int cxy = (cx + 1) + 3 * (cy + 1) + swapf;
+  const int cxy = (cx + 1) + 3 * (cy + 1);
   if (g.periodic) {
     if (izhi - izlo + 1 > g.ncz) {   /* cannot happen with the tile limits */
       izlo = 0; izhi = g.ncz - 1;
@@ -331,10 +350,7 @@ __device__ void fk_strip_segments(const FkJob &J, const FkTile &t, int dx,
     }
     if (izhi >= g.ncz && !diag) {    /* image above: B shifted by +Lz */
       int c0 = 0, c1 = izhi - g.ncz;
-      /* gradient of auto counts, same row: the count takes this image from
-       * the lower tile (the (0,0) strip keeps the lower wrap only) */
-      const int sw = (dx == 0 && dy == 0 && (diag_in & 4)) ? 32 : 0;
-      b0[2] = cs[c0]; len[2] = cs[c1 + 1] - cs[c0]; code[2] = (cxy | sw) + 9 * 2;
+      b0[2] = cs[c0]; len[2] = cs[c1 + 1] - cs[c0]; code[2] = cxy + 9 * 2;
     }
   }
   else {
@@ -668,7 +684,7 @@ k_count(const FkArgs a) {
   constexpr bool GRAD = (MODE == MODE_GRAD);
   constexpr bool NEED_S = (GEOM == FCFC_GEOM_SURVEY && BIN != FCFC_BIN_ISO);
   constexpr bool NEED_W = WT || GRAD;
-  constexpr bool NEED_C = F32 || GRAD;
+  constexpr bool NEED_C = F32;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const FkBins &bn = a.bins;
@@ -698,6 +714,8 @@ k_count(const FkArgs a) {
   if (NEED_S) cp += 8 * FK_CH;
   double *cw = (double *) cp;                  /* weights */
   if (NEED_W) cp += 8 * FK_CH;
+  double *cb = (double *) cp;                  /* GRAD: gradient share of each B object */
+  if (GRAD) cp += 8 * FK_CH;
   int *cj = (int *) cp;                        /* sorted index */
   cp += 4 * FK_CH;
   int *cc = (int *) cp;                        /* periodic image code */
@@ -833,7 +851,6 @@ k_count(const FkArgs a) {
     }
     const int nva = min(t.p1 - t.p0, R * FK_NT);
     const bool jwt = WT && J.wt;
-    const bool grad_self = GRAD && J.autoc;
 
     /* FP32: FP64 re-evaluation of the n queued pairs of this wavefront
      * (entries: A slot << 8 | chunk index), while the chunk is staged */
@@ -871,15 +888,12 @@ k_count(const FkArgs a) {
         else { bx = cx[j]; by = cy[j]; bz = cz[j]; }
         if (NEED_S) bs = cs_[j];
         const double wj = NEED_W ? cw[j] : 1.0;
-        const int jj = (SELF || grad_self || F32) ? cj[j] : 0;
-        const int code = NEED_C ? cc[j] : 13;
-        if (GEOM == FCFC_GEOM_BOX && !PLAIN && !(GRAD && (code & 32))) {
+        const int jj = (SELF || F32) ? cj[j] : 0;
+        if (GEOM == FCFC_GEOM_BOX && !PLAIN) {
           /* the wavefront's A objects are one run of consecutive objects
            * of one cell column: when this B object is out of range of all
            * of them, skip the binning (same expressions as the pair
-           * functions, so the compiler shares them; gradient pairs taken in
-           * the other orientation are staged unshifted and are not tested;
-           * the plain form, like the reference's leaf loops, bins every
+           * functions, so the compiler shares them; the plain form, like the reference's leaf loops, bins every
            * candidate pair) */
           bool any = false;
 #pragma unroll
@@ -918,6 +932,7 @@ k_count(const FkArgs a) {
           }
           if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
         }
+        double bsum = 0.0;          /* GRAD: sum of G[bin] w_a over this lane's pairs */
 #pragma unroll
         for (int r = 0; r < R; r++) {
           unsigned idx;
@@ -944,33 +959,41 @@ k_count(const FkArgs a) {
               if (qn > FK_QCAP - 64) { drain(qn); qn = 0; }
             }
           }
-          else if (GEOM == FCFC_GEOM_BOX) {
-            if (GRAD && (code & 32)) {
-              /* pair that the count evaluates from the other side:
-               * dx = b - (a - D) with D the image of this B segment */
-              const int cd = code & 31;
-              const double ux = ax[r] - (double) (cd % 3 - 1) * J.g.Lx;
-              const double uy = ay[r] - (double) ((cd / 3) % 3 - 1) * J.g.Ly;
-              const double uz = az[r] - (double) (cd / 9 - 1) * J.g.Lz;
-              idx = fk_box64<BIN, HYB>(bx - ux, by - uy, bz - uz, bn, stab, ptab, s2e, pe);
-            }
-            else
-              idx = fk_box64<BIN, HYB, false, PLAIN>(bx - ax[r], by - ay[r], bz - az[r], bn, stab,
-                                                      ptab, s2e, pe);
-          }
+          else if (GEOM == FCFC_GEOM_BOX)
+            idx = fk_box64<BIN, HYB, false, PLAIN>(bx - ax[r], by - ay[r], bz - az[r], bn, stab,
+                                                    ptab, s2e, pe);
           else
             idx = fk_sv64<BIN, HYB, PLAIN>(ax[r], ay[r], az[r], as[r], bx, by, bz, bs, bn,
                                            stab, ptab, s2e, pe);
           if (!F32 && SELF && !(jj > ai[r])) idx = FK_NONE;
-          if (GRAD && grad_self && jj == ai[r]) idx = FK_NONE;
           if (idx < ntot) {
-            if (GRAD) gacc[r] += hist_d[idx] * wj;
+            if (GRAD) {
+              const double gk = hist_d[idx];
+              gacc[r] += gk * wj;
+              bsum += gk * aw[r];
+            }
             else if (WT && jwt) atomicAdd(myhist_d + idx, aw[r] * wj);
             else atomicAdd(myhist_u + idx, 1u);
           }
         }
+        if (GRAD && J.gB) {
+          /* the B object's share over the wavefront's pairs: one sum per
+           * row of 16 lanes, added into the chunk's LDS accumulator */
+          const double v = fk_row_sum(bsum);
+          if ((lane & 15) == 0 && v != 0.0) atomicAdd(cb + j, v);
+        }
       }
       if (F32 && qn) drain(qn);
+    };
+
+    /* GRAD: after a chunk, the B objects' shares go to the output */
+    auto grad_flush = [&](int cnt) {
+      if (!GRAD || !J.gB) return;
+      __syncthreads();
+      for (int q = tid; q < cnt; q += FK_NT) {
+        const double v = cb[q];
+        if (v != 0.0) unsafeAtomicAdd(J.gB + J.B.idx[cj[q]], v);
+      }
     };
 
     auto guard = [&](int cnt) {
@@ -980,7 +1003,7 @@ k_count(const FkArgs a) {
     };
 
     /* ---- auto counts: pairs inside the tile (j > i), first item only ---- */
-    if (!GRAD && J.autoc && t.s0 == 0) {
+    if (J.autoc && t.s0 == 0) {
       for (int c0 = t.p0; c0 < t.p1; c0 += FK_CH) {
         const int cnt = min(FK_CH, t.p1 - c0);
         __syncthreads();
@@ -991,10 +1014,12 @@ k_count(const FkArgs a) {
           else { cx[q] = x; cy[q] = y; cz[q] = z; }
           if (NEED_S) cs_[q] = J.A.s[jg];
           if (NEED_W) cw[q] = jw ? J.A.w[jg] : 1.0;
+          if (GRAD) cb[q] = 0.0;
           cj[q] = jg;
         }
         __syncthreads();
         run_chunk(std::integral_constant<bool, true>(), cnt);
+        grad_flush(cnt);
         evals += (double) cnt * nva * 0.5;
         guard(cnt);
       }
@@ -1037,33 +1062,30 @@ k_count(const FkArgs a) {
           const int jg = st->b0[sgi] + rem;
           const int code = st->code[sgi];
           double x = J.B.x[jg], y = J.B.y[jg], z = J.B.z[jg];
-          /* gradient: pairs that the count evaluates from the other object's
-           * tile keep that orientation, (a - D) - b (see fk_strip_segments) */
-          const bool swp = GRAD && (J.swap_all || (code & 32));
-          if (GEOM == FCFC_GEOM_BOX && (code & 31) != 13 && !swp) {
-            const int cd = code & 31;
-            x = x + (double) (cd % 3 - 1) * J.g.Lx;
-            y = y + (double) ((cd / 3) % 3 - 1) * J.g.Ly;
-            z = z + (double) (cd / 9 - 1) * J.g.Lz;
+          if (GEOM == FCFC_GEOM_BOX && code != 13) {
+            x = x + (double) (code % 3 - 1) * J.g.Lx;
+            y = y + (double) ((code / 3) % 3 - 1) * J.g.Ly;
+            z = z + (double) (code / 9 - 1) * J.g.Lz;
           }
           if (F32) { fx[q] = (float) (x - tcx); fy[q] = (float) (y - tcy); fz[q] = (float) (z - tcz); cc[q] = code; }
           else { cx[q] = x; cy[q] = y; cz[q] = z; }
-          if (GRAD) cc[q] = (code & 31) | (swp ? 32 : 0);
           if (NEED_S) cs_[q] = J.B.s[jg];
           if (NEED_W) cw[q] = jw ? J.B.w[jg] : 1.0;
+          if (GRAD) cb[q] = 0.0;
           cj[q] = jg;
         }
         __syncthreads();
         run_chunk(std::integral_constant<bool, false>(), cnt);
+        grad_flush(cnt);
         evals += (double) cnt * nva;
         guard(cnt);
       }
     }
 
-    if (GRAD) {
+    if (GRAD && J.gA) {
 #pragma unroll
       for (int r = 0; r < R; r++)
-        if (ai[r] != 0x7fffffff) unsafeAtomicAdd(J.gA + J.A.idx[ai[r]], gacc[r]);
+        if (ai[r] != 0x7fffffff && gacc[r] != 0.0) unsafeAtomicAdd(J.gA + J.A.idx[ai[r]], gacc[r]);
     }
   }
 
@@ -1355,16 +1377,11 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
       double gx = std::max(abs(dx) - 1, 0) * g.wx, gy = std::max(abs(dy) - 1, 0) * g.wy;
       if (gx * gx + gy * gy >= rxy2) continue;
       int diag = 0;
-      if (autoc && mode != MODE_GRAD) {
+      if (autoc) {
+        /* auto counts and their gradient: half plane, and on the own row
+         * only the objects after the tile */
         if (dx < 0 || (dx == 0 && dy < 0)) continue;
         if (dx == 0 && dy == 0) diag = 1;
-      }
-      else if (autoc) {
-        /* gradient of auto counts: full shell; flag the pairs that the count
-         * evaluates from the other object's tile (negative half plane: 2;
-         * the own-row strip, whose upper z image is taken from below: 4) */
-        if (dx < 0 || (dx == 0 && dy < 0)) diag = 2;
-        if (dx == 0 && dy == 0) diag = 4;
       }
       P->strips.push_back(dx); P->strips.push_back(dy); P->strips.push_back(diag);
     }
@@ -1607,8 +1624,9 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   int per = (mode == MODE_C32) ? 12 : 24;
   if (need_s) per += 8;
   if (wt || mode == MODE_GRAD) per += 8;
+  if (mode == MODE_GRAD) per += 8;             /* gradient share of each B object */
   per += 4;
-  if (mode == MODE_C32 || mode == MODE_GRAD) per += 4;
+  if (mode == MODE_C32) per += 4;
   o = align16(o + FK_CH * per);
   L.off_queue = o;
   if (mode == MODE_C32) o += (FK_NT / 64) * FK_QCAP * 4;
@@ -1727,8 +1745,8 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double 
 struct PJob {
   JobPlan P;
   DevCat A, B;
-  int autoc = 0, wt = 0, swap_all = 0;
-  double *gA = nullptr;
+  int autoc = 0, wt = 0;
+  double *gA = nullptr, *gB = nullptr;
   long long hist_off = 0;
 };
 
@@ -1927,9 +1945,9 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     f.nstrips = J.P.nstrips;
     f.autoc = J.autoc;
     f.wt = J.wt;
-    f.swap_all = J.swap_all;
     f.hist_off = J.hist_off;
     f.gA = J.gA;
+    f.gB = J.gB;
     if (m == MODE_C32) fp32_windows(b, J.P.g, J.P.zmaxc, s2f, &f.win);
     hstr.insert(hstr.end(), J.P.strips.begin(), J.P.strips.end());
   }
@@ -2166,23 +2184,28 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
   memset(&c->stats, 0, sizeof c->stats);
   c->census = 0;
   const int dev = (flags & FCFC_GPU_CNT_DEVICE) ? 1 : 0;
-  const int npass = (autoc || !gb) ? 1 : 2;
-  Alloc al(c, npass > 1);
+  /* one pass over the pairs exactly as the count enumerates them (auto: each
+   * unordered pair once; cross: the larger catalogue in the lanes), so every
+   * pair is binned as in the count; each pair adds G[bin] w_b to its A object
+   * and G[bin] w_a to its B object */
+  const int sw = (!autoc && b->n > a->n) ? 1 : 0;
+  const fcfc_cat *pa = sw ? b : a, *pb = autoc ? a : (sw ? a : b);
+  Alloc al(c, false);
   std::vector<CopyKey> copies;
   std::vector<FkStage> stages;
-  std::vector<PJob> jobs(npass);
+  std::vector<PJob> jobs(1);
   int err = 0;
-  for (int p = 0; p < npass; p++) {
-    const fcfc_cat *pa = p ? b : a, *pb = p ? a : b;
-    int e = setup_job(c, al, copies, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &jobs[p], stages);
-    if (e) return e;
-    /* second pass of a cross gradient: the count evaluated every pair with
-     * the other catalogue as A */
-    jobs[p].swap_all = p;
-    jobs[p].gA = (double *) al.get(p ? WS_GB : WS_GA, sizeof(double) * (size_t) (pa->n > 0 ? pa->n : 1));
-    if (al.err) return al.err;
-    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, jobs[p].gA, (int64_t) pa->n, 0.0);
-  }
+  if (int e = setup_job(c, al, copies, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &jobs[0], stages))
+    return e;
+  double *dga = (double *) al.get(WS_GA, sizeof(double) * (size_t) (a->n > 0 ? a->n : 1));
+  double *dgb = (!autoc && gb) ? (double *) al.get(WS_GB, sizeof(double) * (size_t) (b->n > 0 ? b->n : 1))
+                               : nullptr;
+  if (al.err) return al.err;
+  hipLaunchKernelGGL(k_fill_f64, dim3(nblk(a->n, 256)), dim3(256), 0, c->stream, dga, (int64_t) a->n, 0.0);
+  if (dgb)
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(b->n, 256)), dim3(256), 0, c->stream, dgb, (int64_t) b->n, 0.0);
+  jobs[0].gA = sw ? dgb : dga;
+  jobs[0].gB = autoc ? dga : (sw ? dga : dgb);
   if (int e = stage_run(c, stages)) return e;
   double *dG = (double *) ws(c, WS_G, sizeof(double) * bins->ntot, &err);
   unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
@@ -2193,18 +2216,19 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
   float ms = 0;
   int e = run_jobs(c, al, bins, jobs, MODE_GRAD, dG, nullptr, misc + 8, (unsigned int *) misc, &ms);
   if (e) return e;
-  for (int p = 0; p < npass; p++) {
-    const fcfc_cat *pa = p ? b : a;
-    double *gout = p ? gb : ga;
+  for (int p = 0; p < 2; p++) {
+    const fcfc_cat *k = p ? b : a;
+    double *gout = p ? gb : ga, *gdev = p ? dgb : dga;
+    if (!gdev) continue;
     if (dev) {
-      hipLaunchKernelGGL(k_add_f64, dim3(nblk(pa->n, 256)), dim3(256), 0, c->stream, gout, jobs[p].gA, (int) pa->n);
+      hipLaunchKernelGGL(k_add_f64, dim3(nblk(k->n, 256)), dim3(256), 0, c->stream, gout, gdev, (int) k->n);
       HIPCHK(c, hipGetLastError());
     }
     else {
-      std::vector<double> h((size_t) pa->n);
-      HIPCHK(c, hipMemcpyAsync(h.data(), jobs[p].gA, sizeof(double) * (size_t) pa->n, hipMemcpyDeviceToHost, c->stream));
+      std::vector<double> h((size_t) k->n);
+      HIPCHK(c, hipMemcpyAsync(h.data(), gdev, sizeof(double) * (size_t) k->n, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
-      for (int64_t i = 0; i < pa->n; i++) gout[i] += h[(size_t) i];
+      for (int64_t i = 0; i < k->n; i++) gout[i] += h[(size_t) i];
     }
   }
   if ((e = stats_from(c, misc + 8))) return e;
