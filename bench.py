@@ -321,6 +321,13 @@ def grad_workload(args):
     t0 = time.time()
     rr = WeightedPairCount.apply(wr, None, eng, R, None, b, True).detach()
     log(f"RR once: {time.time() - t0:.2f} s")
+    if not args.no_autotune:
+        Dw = F.Catalog(b, D.x, D.y, D.z, wd.detach().to(torch.float64).contiguous())
+        Rw = F.Catalog(b, R.x, R.y, R.z, wr)
+        for c2, au in ((None, True), (Rw, False)):
+            for g in (False, True):
+                log(f"autotune {'gradient' if g else 'count'} {'DD' if au else 'DR'}: "
+                    f"{eng.autotune(Dw, c2, b, isauto=au, withwt=True, gradient=g)}")
     fit = WeightFitCounts(eng, D, R, b)
     rng = np.random.default_rng(20273)
     target = torch.tensor(rng.normal(0, 0.01, (2, b.ns)), dtype=torch.float64, device="cuda")

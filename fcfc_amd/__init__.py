@@ -265,11 +265,16 @@ class Engine:
         self._check(lib().fcfc_gpu_set_tuning(self._h, cell_div, tile, zdiv, nblock),
                     "fcfc_gpu_set_tuning")
 
-    def autotune(self, a, b, bins, isauto=False, withwt=False, precision="fp64"):
+    def autotune(self, a, b, bins, isauto=False, withwt=False, precision="fp64",
+                 gradient=False):
         """Pick grid/tile shapes for counts like this one (cached in the
-        engine per density bucket).  Returns (cell_div, zdiv, tile, est_ms)."""
+        engine per density bucket), or with gradient=True for the weight
+        gradient of such counts (count_backward).  Returns (cell_div, zdiv,
+        tile, est_ms)."""
         flags = (L.FCFC_GPU_AUTO if isauto else 0) | (L.FCFC_GPU_WEIGHTED if withwt else 0)
         flags |= _prec_flag(precision)
+        if gradient:
+            flags |= L.FCFC_GPU_GRADIENT
         best = (C.c_int * 4)()
         bs = bins.bsize_rescaled
         ca = a._c()

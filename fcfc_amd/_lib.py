@@ -26,6 +26,7 @@ FCFC_GPU_PREC_FP64 = 0x00
 FCFC_GPU_PREC_FP32 = 0x10
 FCFC_GPU_CNT_DEVICE = 0x20
 FCFC_GPU_CENSUS = 0x40
+FCFC_GPU_GRADIENT = 0x100
 FCFC_GPU_PREC_PLAIN = 0x80
 
 _dp = C.POINTER(C.c_double)

@@ -2354,10 +2354,25 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
     uint32_t flags, int *best) {
   if (!c || !bins_ok(bins)) return FCFC_ERR_ARG;
-  const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 :
+  const int autoc = (flags & FCFC_GPU_AUTO) ? 1 : 0;
+  if (!cat_ok(a) || (!autoc && !cat_ok(b))) { set_err(c, "invalid argument"); return FCFC_ERR_ARG; }
+  const int grad = (flags & FCFC_GPU_GRADIENT) ? 1 : 0;
+  const int mode = grad ? MODE_GRAD : (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 :
                    ((flags & FCFC_GPU_PREC_PLAIN) ? MODE_P64 : MODE_C64);
   const int save_cd = c->t_cell_div, save_zd = c->t_zdiv, save_t = c->t_tile;
   std::vector<unsigned char> scratch(8 * (size_t) bins->ntot);
+  /* gradient timing: dL/dcnt = 1 in every bin (every pair contributes),
+   * gradient of a only, as weight fitting asks for it */
+  std::vector<double> gone(grad ? (size_t) bins->ntot : 0, 1.0);
+  std::vector<double> gout(grad ? (size_t) std::max<int64_t>(a->n, 1) : 0);
+  const uint32_t cflags = flags & ~(FCFC_GPU_CNT_DEVICE | FCFC_GPU_GRADIENT);
+  auto run = [&]() -> int {
+    if (grad)
+      return fcfc_gpu_count_backward(c, a, b, bins, bsize, flags & FCFC_GPU_AUTO, gone.data(),
+                                     gout.data(), nullptr);
+    memset(scratch.data(), 0, scratch.size());
+    return fcfc_gpu_count(c, a, b, bins, bsize, cflags, scratch.data());
+  };
   double bestms = 1e300;
   int bcd = 3, bzd = 8, bT = 0;
   int last_T = 0;
@@ -2365,12 +2380,11 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     c->t_cell_div = cd; c->t_zdiv = zd; c->t_tile = T;
     /* first pass: learn the number of tiles of this shape with a tiny cap */
     c->tile_cap = 1;
-    memset(scratch.data(), 0, scratch.size());
-    int e = fcfc_gpu_count(c, a, b, bins, bsize, flags & ~FCFC_GPU_CNT_DEVICE, scratch.data());
+    int e = run();
     if (e) return e;
     const int nt = c->last_ntiles;
     c->tile_cap = std::max(1, nt / 8);
-    e = fcfc_gpu_count(c, a, b, bins, bsize, flags & ~FCFC_GPU_CNT_DEVICE, scratch.data());
+    e = run();
     if (e) return e;
     *est = c->stats.kernel_ms * (double) nt / (double) c->tile_cap;
     last_T = c->stats.tile;
@@ -2398,18 +2412,20 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
   c->tile_cap = 0;
   c->t_cell_div = save_cd; c->t_zdiv = save_zd; c->t_tile = save_t;
   if (e) return e;
-  /* cache for the density bucket of this job */
+  /* cache for the density bucket of this job, keyed as the count (or the
+   * gradient) plans it: the larger catalogue of a cross count is A */
   {
     JobPlan P;
     double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
-    const int autoc = (flags & FCFC_GPU_AUTO) ? 1 : 0;
+    const fcfc_cat *ka = a, *kb = autoc ? a : b;
+    if (!autoc && kb->n > ka->n) std::swap(ka, kb);
     if (bins->geometry == FCFC_GEOM_SURVEY) {
       for (int k = 0; k < 3; k++) { lo[k] = 1e300; hi[k] = -1e300; }
-      bbox(c, a, lo, hi);
-      if (!autoc) bbox(c, b, lo, hi);
+      bbox(c, ka, lo, hi);
+      if (!autoc) bbox(c, kb, lo, hi);
     }
-    const int wt = (flags & FCFC_GPU_WEIGHTED) ? 1 : 0;
-    make_plan(c, bins, bsize, lo, hi, autoc, wt, mode, a->n, autoc ? a->n : b->n, &P);
+    const int wt = (grad || (flags & FCFC_GPU_WEIGHTED)) ? 1 : 0;
+    make_plan(c, bins, bsize, lo, hi, autoc, wt, mode, ka->n, kb->n, &P);
     bool found = false;
     for (auto &t : c->tuned)
       if (t.geom == bins->geometry && t.bintype == bins->bintype && t.mode == mode &&

@@ -146,6 +146,8 @@ const char *fcfc_gpu_last_error(const fcfc_gpu_ctx *ctx);
 #define FCFC_GPU_CNT_DEVICE   0x20u  /* cnt is a device pointer             */
 #define FCFC_GPU_CENSUS       0x40u  /* count pairs whose bin is rounding-path
                                         dependent in the reference (FP32 mode) */
+#define FCFC_GPU_GRADIENT     0x100u /* fcfc_gpu_autotune: tune the weight-
+                                        gradient kernel (fcfc_gpu_count_backward) */
 
 /*
  * Count pairs.  Replaces count_pairs(tree1, tree2, cf, cnt, isauto, withwt).
@@ -225,8 +227,10 @@ int fcfc_gpu_set_tuning(fcfc_gpu_ctx *ctx, int cell_div, int tile,
  * shapes for counts like (a, b, bins, flags) on the first eighth of their tile
  * list (a slab of the box at the true density), and keeps the fastest in the
  * context for this geometry, binning, precision and density bucket; later
- * counts whose shape is automatic use it.  best (may be NULL) receives
- * {cell_div, zcell_div, tile, estimated full-count microseconds}. */
+ * counts whose shape is automatic use it.  With FCFC_GPU_GRADIENT in flags
+ * it tunes fcfc_gpu_count_backward for such counts instead.  best (may be
+ * NULL) receives {cell_div, zcell_div, tile, estimated full-count
+ * microseconds}. */
 int fcfc_gpu_autotune(fcfc_gpu_ctx *ctx, const fcfc_cat *a, const fcfc_cat *b,
     const fcfc_bins *bins, const double *bsize, uint32_t flags, int *best);
 

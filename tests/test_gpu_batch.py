@@ -53,6 +53,32 @@ def test_autotune_keeps_counts(eng):
     np.testing.assert_array_equal(eng.count_pairs(D, None, b, isauto=True), ref)
 
 
+def test_autotune_cross_and_gradient(eng):
+    """Tuned shapes of a cross count whose first catalogue is the smaller one
+    (the count swaps them) and of the weight gradient leave the results as
+    they were: counts identical, gradients to rounding."""
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=10, bsize=L)
+    x, y, z, w = F.gen_clustered(21, 3000, 20, 50, 2.0, L, 0.3)
+    xr, yr, zr = F.gen_uniform(22, 12000, L)
+    D = F.Catalog(b, x, y, z, w)
+    R = F.Catalog(b, xr, yr, zr, np.linspace(0.8, 1.2, len(xr)))
+    G = np.random.default_rng(3).normal(size=b.ntot)
+    ref_c = eng.count_pairs(D, R, b, withwt=True)
+    ref_g = [eng.count_backward(D, R, b, G), eng.count_backward(D, None, b, G, isauto=True)[0]]
+    for g in (False, True):
+        print(f"case autotune cross gradient={g}", flush=True)
+        cd, zd, tile, est = eng.autotune(D, R, b, withwt=True, gradient=g)
+        assert cd in (2, 3, 4, 6) and zd in (4, 8, 16) and tile in (512, 1024) and est > 0
+        cd, zd, tile, est = eng.autotune(D, None, b, isauto=True, withwt=True, gradient=g)
+        assert est > 0
+    np.testing.assert_allclose(eng.count_pairs(D, R, b, withwt=True), ref_c, rtol=1e-12, atol=0)
+    ga, gb = eng.count_backward(D, R, b, G)
+    np.testing.assert_allclose(ga, ref_g[0][0], rtol=1e-10, atol=1e-12 * np.abs(ref_g[0][0]).max())
+    np.testing.assert_allclose(gb, ref_g[0][1], rtol=1e-10, atol=1e-12 * np.abs(ref_g[0][1]).max())
+    gd, _ = eng.count_backward(D, None, b, G, isauto=True)
+    np.testing.assert_allclose(gd, ref_g[1], rtol=1e-10, atol=1e-12 * np.abs(ref_g[1]).max())
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_plan_graph_replays(eng, prec):
     """A prepared batch replays its HIP graph: every replay must equal an
