@@ -248,6 +248,11 @@ def survey_workload(args):
     log(f"survey workload ready in {time.time() - t0:.1f} s: D={D.n} R={R.n}")
     eng = F.Engine(0)
     prec = "fp64" if args.precision == "fp32" else args.precision   # survey: FP64 forms
+    if not args.no_autotune:
+        forms = [prec] + ([] if "plain" in args.skip or prec == "plain" else ["plain"])
+        for p in forms:
+            for name, a, c, au in (("DD", D, None, True), ("DR", D, R, False), ("RR", R, None, True)):
+                log(f"autotune {p} {name}: {eng.autotune(a, c, b, isauto=au, withwt=True, precision=p)}")
 
     def step(p):
         out, ev, km = {}, 0.0, {}
