@@ -66,6 +66,41 @@ def test_backward_hybrid_and_spi(eng, bt):
             np.testing.assert_allclose(gb, rb, rtol=1e-6, atol=1e-9 * np.abs(rb).max())
 
 
+@pytest.mark.parametrize("bt", [F.BIN_SMU, F.BIN_SPI])
+def test_backward_survey(eng, bt):
+    """Survey geometry: auto gradient, and cross gradients with the smaller
+    catalogue first (the pass puts the other one in the lanes and reduces
+    the first one's terms over wavefronts) and with the larger first and
+    only its own gradient asked for, against the FP64 oracle."""
+    b = F.Bins.linear(F.GEOM_SURVEY, bt, 0, 200, 5, nmu=10 if bt == F.BIN_SMU else 1,
+                      pmin=0, pmax=200 if bt == F.BIN_SPI else 0, dpi=5)
+    x, y, z, w = F.gen_survey(51, 3000)
+    xr, yr, zr, wr = F.gen_survey(52, 5000)
+    D = F.Catalog(b, x, y, z, w)
+    R = F.Catalog(b, xr, yr, zr, wr)
+    G = np.random.default_rng(9).normal(size=b.ntot)
+    ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu, b.pbin if bt == F.BIN_SPI else None)
+    tD = (D.x, D.y, D.z, D.s, D.w)
+    tR = (R.x, R.y, R.z, R.s, R.w)
+
+    def close(g, r):
+        np.testing.assert_allclose(g, r, rtol=1e-6, atol=1e-9 * np.abs(r).max())
+
+    print(f"case backward survey bt={bt} auto", flush=True)
+    ga, _ = eng.count_backward(D, None, b, G, isauto=True)
+    ra, _ = O.weight_grad(1, None, tD, None, True, ob, G)
+    close(ga, ra)
+    print(f"case backward survey bt={bt} cross", flush=True)
+    ga, gb = eng.count_backward(D, R, b, G)
+    ra, rb = O.weight_grad(1, None, tD, tR, False, ob, G)
+    close(ga, ra)
+    close(gb, rb)
+    print(f"case backward survey bt={bt} cross, larger first, one side", flush=True)
+    gr, none = eng.count_backward(R, D, b, G, need_b=False)
+    assert none is None
+    close(gr, rb)
+
+
 def test_backward_finite_difference(eng, setup):
     b, D, R, G = setup
     ga, gb = eng.count_backward(D, R, b, G, isauto=False)
