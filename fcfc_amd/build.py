@@ -25,9 +25,11 @@ CC = os.environ.get("CC", "gcc")
 
 # Packed FP32 instructions are disabled for all device code; the FP64 pair
 # arithmetic must not be contracted into FMAs other than the explicit ones.
+# The pair loop asks for a 2x unroll, which the compiler declines for the
+# variants with re-evaluation queues or cross-lane sums (a note, not an error).
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-    "-ffp-contract=off",
+    "-ffp-contract=off", "-Wno-pass-failed",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 C_FLAGS = ["-O2", "-std=c99", "-D_POSIX_C_SOURCE=200809L", "-fPIC",
