@@ -307,7 +307,8 @@ class Engine:
 
     def count_backward(self, a, b, bins, grad_cnt, isauto=False, need_b=True):
         """d(sum_k G_k cnt_k)/dw for the objects of a (and of b for cross
-        counts unless need_b is False, which skips the pass over b)."""
+        counts unless need_b is False, which skips accumulating b's terms).
+        One pass over the pairs as the count enumerates them."""
         g = _f64(grad_cnt)
         ga = np.zeros(a.n)
         gb = np.zeros((b if b is not None else a).n)

@@ -210,7 +210,9 @@ int fcfc_gpu_plan_destroy(fcfc_gpu_plan *plan);
  * histogram as returned, i.e. before the caller's auto doubling), adds
  *   ga[i] += sum_j G[bin(i,j)] * w_b[j]     (cross: also gb[j] += ...)
  * For auto counts every unordered pair contributes to both members and gb
- * is unused.  ga/gb are double[n] in the ORIGINAL object order.
+ * is unused; for cross counts gb may be NULL when only a's gradient is
+ * wanted.  ga/gb are double[n] in the ORIGINAL object order.  The pairs are
+ * enumerated and binned exactly as fcfc_gpu_count enumerates them.
  */
 int fcfc_gpu_count_backward(fcfc_gpu_ctx *ctx, const fcfc_cat *a,
     const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
