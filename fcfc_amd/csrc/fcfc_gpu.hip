@@ -388,12 +388,24 @@ __device__ __forceinline__ int fk_hslot(int i, int ns, int nsh) {
  * zero-based: entry k belongs to (int) s2 = k, the entries below (int) s2min
  * and the one at (int) s2min + len are FK_OUT, so with s2 >= 0 the range
  * check is one min.  Hybrid tables (pointer at cell 0) refine downwards
- * against the edges (FCFC_LOOKUP_HYBRID, metric_common.c:56-64). */
+ * against the edges (FCFC_LOOKUP_HYBRID, metric_common.c:56-64): the result
+ * is the last edge at or below s2 (bin 0 below the first edge).  Tables too
+ * long for LDS are replaced by a binary search for that same edge. */
+__device__ __forceinline__ unsigned fk_edge_search(const double *e, int n, double v) {
+  int lo = 0, hi = n - 1;                 /* last i in [0, n) with e[i] <= v, else 0 */
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (e[mid] <= v) lo = mid; else hi = mid - 1;
+  }
+  return (unsigned) lo;
+}
+
 template <bool HYB, bool MK = false>
 __device__ __forceinline__ unsigned fk_sbin(double s2, const FkBins &bn,
     const unsigned *stab, const double *s2e) {
   if (HYB) {
     if (!(s2 < bn.s2max) || (bn.s2min != 0 && s2 < bn.s2min)) return FK_OUT;
+    if (bn.ssearch) return fk_edge_search(s2e, bn.ns, s2);
     int k = (int) s2 - bn.soff;
     k = k < 0 ? 0 : (k >= bn.stab_len ? bn.stab_len - 1 : k);
     unsigned sb = MK ? (stab[k] & 0xFFFFu) : stab[k];
@@ -412,6 +424,7 @@ __device__ __forceinline__ unsigned fk_pbin(double p, const FkBins &bn,
     const unsigned *ptab, const double *pe) {
   if (HYB) {
     if (!(p < bn.pmax) || p < bn.pmin) return FK_OUT;
+    if (bn.psearch) return fk_edge_search(pe, bn.np, p);
     int k = (int) p - bn.poff;
     k = k < 0 ? 0 : (k >= bn.ptab_len ? bn.ptab_len - 1 : k);
     unsigned pb = MK ? (ptab[k] & 0xFFFFu) : ptab[k];
@@ -1608,7 +1621,8 @@ static bool zb_fits(const fcfc_bins *b) {
   return b->soff <= FK_ZB_MAXOFF && (b->bintype != FCFC_BIN_SPI || b->poff <= FK_ZB_MAXOFF);
 }
 
-static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh, int hyb) {
+static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh, int hyb,
+    int ssearch = 0, int psearch = 0) {
   LdsPlan L;
   int o = 0;
   L.nh = nh;
@@ -1634,9 +1648,10 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   const int zero_based = (!hyb || mode == MODE_C32);
   L.stab_c0 = zero_based ? b->soff : 1;
   L.ptab_c0 = zero_based ? b->poff : 1;
-  L.off_stab = o; o = align16(o + (L.stab_c0 + b->stab_len + 2) * 4);
+  L.off_stab = o; o = align16(o + (L.stab_c0 + (ssearch ? 0 : b->stab_len) + 2) * 4);
   L.off_ptab = o;
-  if (b->bintype == FCFC_BIN_SPI) o = align16(o + (L.ptab_c0 + b->ptab_len + 2) * 4);
+  if (b->bintype == FCFC_BIN_SPI)
+    o = align16(o + (L.ptab_c0 + (psearch ? 0 : b->ptab_len) + 2) * 4);
   L.total = o;
   return L;
 }
@@ -1828,11 +1843,21 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   if (m == MODE_C32 && geom != FCFC_GEOM_BOX) m = MODE_C64;    /* survey FP32: FP64 kernel */
   const bool zb = zb_fits(b);
   if (m == MODE_C32 && !zb) m = MODE_C64;                      /* FP32 tables are zero-based */
-  const int hyb = (b->tabtype || !zb) ? 1 : 0;
+  int hyb = (b->tabtype || !zb) ? 1 : 0;
   const int need_s = (geom == FCFC_GEOM_SURVEY && b->bintype != FCFC_BIN_ISO);
   int anywt = 0;
   for (auto &j : jobs) anywt |= j.wt;
   if (m == MODE_GRAD) anywt = 1;
+  const int lds_max = 160 * 1024 - 2048;       /* room for the static LDS */
+  /* tables too long for LDS (hybrid binnings rescaled to large integers):
+   * hybrid lookups that binary-search the edges, in the FP64 forms */
+  int ssearch = 0, psearch = 0;
+  if (lds_plan(b, m, anywt, need_s, 1, hyb).total > lds_max) {
+    if (m == MODE_C32) m = MODE_C64;
+    hyb = 1;
+    ssearch = 1;
+    if (lds_plan(b, m, anywt, need_s, 1, hyb, 1, 0).total > lds_max) psearch = 1;
+  }
   /* one kernel serves every job of the launch: tiles of the longest kind */
   {
     int R = 1;
@@ -1844,8 +1869,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   const int R = jobs.empty() ? 1 : jobs[0].P.R;
   kfn_t fn = pick_kernel(b->bintype, geom, anywt, (m == MODE_C32 && c->census) ? MODE_C32C : m, hyb, R);
   /* histogram copies: as many as keep the occupancy of a single copy */
-  const int lds_max = 160 * 1024 - 2048;       /* room for the static LDS */
-  LdsPlan L = lds_plan(b, m, anywt, need_s, 1, hyb);
+  LdsPlan L = lds_plan(b, m, anywt, need_s, 1, hyb, ssearch, psearch);
   if (L.total > lds_max) {
     set_err(c, "binning tables need %d bytes of LDS (limit %d)", L.total, lds_max);
     return FCFC_ERR_CFG;
@@ -1860,7 +1884,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     if (int e = occ_of(L, &occ1)) return e;
     const int nhmax = c->t_nh > 0 ? c->t_nh : 8;
     for (int nh = 2; nh <= nhmax; nh *= 2) {
-      LdsPlan L2 = lds_plan(b, m, anywt, need_s, nh, hyb);
+      LdsPlan L2 = lds_plan(b, m, anywt, need_s, nh, hyb, ssearch, psearch);
       int occ = 0;
       if (L2.total > lds_max) break;
       if (int e = occ_of(L2, &occ)) return e;
@@ -1972,7 +1996,9 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   fb.nsh = hist_stride(b); fb.ntoh = hist_slots(b);
   fb.scap32 = b->soff + b->stab_len + 1; fb.pcap32 = b->poff + b->ptab_len + 1;
   fb.tabtype = b->tabtype; fb.soff = b->soff; fb.poff = b->poff;
-  fb.stab_len = b->stab_len; fb.ptab_len = b->ptab_len;
+  /* searched tables are not staged to LDS (length 0: sentinels only) */
+  fb.stab_len = ssearch ? 0 : b->stab_len; fb.ptab_len = psearch ? 0 : b->ptab_len;
+  fb.ssearch = ssearch; fb.psearch = psearch;
   fb.s2min = b->s2bin[0]; fb.s2max = b->s2bin[b->ns];
   fb.pmin = (b->bintype == FCFC_BIN_SPI) ? b->pbin[0] : 0;
   fb.pmax = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;

@@ -45,6 +45,8 @@ struct FkBins {
   int tabtype;               /* 0 integer table, 1 hybrid table             */
   int soff, poff;            /* (int) s2min, (int) pmin (or pi^2 min)       */
   int stab_len, ptab_len;    /* table lengths without sentinels             */
+  int ssearch, psearch;      /* hybrid lookups: tables too long for LDS, bin
+                                by binary search over the edges instead    */
   int scap32, pcap32;        /* last entry of the zero-based FP32 tables    */
   double s2min, s2max;       /* squared s (or s_perp) range                 */
   double pmin, pmax;         /* pi range (box) or pi^2 range (survey)       */
