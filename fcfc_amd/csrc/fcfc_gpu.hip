@@ -762,6 +762,10 @@ k_count(const FkArgs a) {
       if (!F32) return v;
       unsigned f = 0;
       if (cell(t, len, c - 1) != v) f |= FK_F_EDGE;
+      /* the cell after one holding a non-integer lower edge: a value just
+       * below that edge may land at its lower end (same bin, no edge of the
+       * table there) */
+      if (HYB && c == 1 && off != lo_edge) f |= FK_F_EDGE;
       if (v != FK_OUT && (v >= (unsigned) nb || (alw_below > 0 && off + c < alw_below + 1.0) ||
                           (HYB && c == 0 && off != lo_edge)))
         f |= FK_F_ALW | FK_F_EDGE;
@@ -1849,6 +1853,18 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   for (auto &j : jobs) anywt |= j.wt;
   if (m == MODE_GRAD) anywt = 1;
   const int lds_max = 160 * 1024 - 2048;       /* room for the static LDS */
+  /* the FP32 window test reads one table cell and the edge at its lower end:
+   * windows must stay well inside a cell (wide only for a few objects in a
+   * huge volume, whose grid cells, and so tiles, are large) */
+  if (m == MODE_C32) {
+    double s2f = 0;
+    for (const PJob &J : jobs) s2f = std::max(s2f, fp32_s2_floor(b, J.P.g, J.P.zmaxc));
+    for (const PJob &J : jobs) {
+      FkWin w;
+      fp32_windows(b, J.P.g, J.P.zmaxc, s2f, &w);
+      if (!(w.e_s2x2 < 0.5f) || !(w.e_pix2 < 0.5f)) { m = MODE_C64; break; }
+    }
+  }
   /* tables too long for LDS (hybrid binnings rescaled to large integers):
    * hybrid lookups that binary-search the edges, in the FP64 forms */
   int ssearch = 0, psearch = 0;

@@ -71,7 +71,9 @@ def _config(seed):
     return geom, bt, b, A, B
 
 
-@pytest.mark.parametrize("seed", range(24))
+# seed 8: hybrid edges whose table does not fit in LDS; seed 127: a hybrid
+# lower edge just below an integer, inside the FP32 window of the next cell
+@pytest.mark.parametrize("seed", list(range(48)) + [127])
 def test_random_config(eng, seed):
     geom, bt, b, A, B = _config(seed)
     ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu, b.pbin if bt == F.BIN_SPI else None)
