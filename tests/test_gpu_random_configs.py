@@ -1,8 +1,9 @@
 """GPU parity over seeded random configurations: geometry, binning type,
-integer or hybrid bin edges, number of mu bins, box shape, catalogue sizes
-and clustering, auto or cross (in both catalogue orders), every form --
-against the oracle (counts identical when the ambiguity census is zero,
-weighted counts within 1e-12 relative)."""
+integer or hybrid bin edges, number of mu bins, lower pi edge, box shape,
+catalogue sizes and clustering, auto or cross (in both catalogue orders),
+every form, single and batched counts, and the weight gradient -- against
+the oracle (counts identical when the ambiguity census is zero, weighted
+counts within 1e-12 relative, gradients within 1e-6)."""
 import numpy as np
 import pytest
 
@@ -41,12 +42,13 @@ def _config(seed):
     nmu = int(rng.choice([2, 3, 10, 37, 100])) if bt == F.BIN_SMU else 1
     if bt == F.BIN_SPI:
         dpi = float(rng.choice([2.0, 5.0, 7.5]))
-        pmax = dpi * int(rng.integers(2, 30))
+        pmin = float(rng.choice([0.0, 0.0, dpi, 3 * dpi]))
+        pmax = pmin + dpi * int(rng.integers(2, 30))
         pmax = min(pmax, lmax)
-        pmax = dpi * max(1, int(pmax // dpi))
+        pmax = pmin + dpi * max(1, int((pmax - pmin) // dpi))
     else:
-        dpi, pmax = 1.0, 0.0
-    b = F.Bins.linear(geom, bt, smin, smax, ds, nmu=nmu, pmin=0.0, pmax=pmax, dpi=dpi,
+        dpi, pmin, pmax = 1.0, 0.0, 0.0
+    b = F.Bins.linear(geom, bt, smin, smax, ds, nmu=nmu, pmin=pmin, pmax=pmax, dpi=dpi,
                       bsize=Lb)
     n1 = int(rng.integers(1, 4000))
     n2 = int(rng.integers(1, 6000))
@@ -108,3 +110,24 @@ def test_random_config(eng, seed):
         same(eng.count_pairs(B, A, b, precision=prec), ref_x, cen_x)
         close(eng.count_pairs(A, None, b, isauto=True, withwt=True, precision=prec), ref_aw)
         close(eng.count_pairs(A, B, b, withwt=True, precision=prec), ref_xw)
+    # one batched launch of mixed jobs
+    ba, bx, bxw = eng.count_batch([(A, None, True, False), (A, B, False, False),
+                                   (A, B, False, True)], b, precision="fp64")
+    same(ba, ref_a, cen_a)
+    same(bx, ref_x, cen_x)
+    close(bxw, ref_xw)
+    # weight gradients (pairs binned as the count bins them: equal bins
+    # when the census is zero)
+    if cen_a == 0 and cen_x == 0:
+        G = np.random.default_rng(seed).normal(size=b.ntot)
+
+        def grad_close(g, r):
+            np.testing.assert_allclose(g, r, rtol=1e-6, atol=1e-9 * max(np.abs(r).max(), 1e-300))
+
+        ga, _ = eng.count_backward(A, None, b, G, isauto=True)
+        ra, _ = O.weight_grad(og, bs, wA, None, True, ob, G)
+        grad_close(ga, ra)
+        ga, gb = eng.count_backward(A, B, b, G)
+        ra, rb = O.weight_grad(og, bs, wA, wB, False, ob, G)
+        grad_close(ga, ra)
+        grad_close(gb, rb)
