@@ -131,3 +131,62 @@ def test_random_config(eng, seed):
         ra, rb = O.weight_grad(og, bs, wA, wB, False, ob, G)
         grad_close(ga, ra)
         grad_close(gb, rb)
+
+
+def _large_config(seed):
+    rng = np.random.default_rng(5000 + seed)
+    bt = (F.BIN_ISO, F.BIN_SMU, F.BIN_SPI)[seed % 3]
+    L = 1000.0
+    if seed % 2 == 0:                 # integer-friendly edges; odd seeds: hybrid tables
+        ds = float(rng.choice([2.0, 4.0, 5.0]))
+        smin = float(rng.choice([0.0, ds]))
+    else:
+        ds = float(np.round(rng.uniform(1.3, 6.0), 2))
+        smin = float(np.round(rng.uniform(0.0, 2.0 * ds), 2))
+    ns = max(1, int((min(smin + int(rng.integers(5, 41)) * ds, 120.0) - smin) // ds))
+    nmu = int(rng.choice([5, 40, 100])) if bt == F.BIN_SMU else 1
+    dpi, pmax = (5.0, 5.0 * int(rng.integers(4, 21))) if bt == F.BIN_SPI else (1.0, 0.0)
+    b = F.Bins.linear(F.GEOM_BOX, bt, smin, smin + ns * ds, ds, nmu=nmu, pmin=0.0, pmax=pmax,
+                      dpi=dpi, bsize=L)
+    na = int(rng.integers(30000, 80000))
+    x, y, z, w = F.gen_clustered(int(rng.integers(1 << 30)), na // 2, na // 400, 200, 2.0, L, 0.3)
+    A = F.Catalog(b, x, y, z, w)
+    xr, yr, zr = F.gen_uniform(int(rng.integers(1 << 30)), int(rng.integers(30000, 100000)), L)
+    B = F.Catalog(b, xr, yr, zr)
+    shape = (int(rng.choice([1, 2, 3, 4, 6])), int(rng.choice([64, 256, 512, 1024])),
+             int(rng.choice([1, 2, 4, 8, 16])), int(rng.choice([0, 7, 64, 300])))
+    return bt, b, A, B, shape
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_large(eng, seed):
+    """Larger catalogues (tiles of two objects per lane, many work items per
+    tile, both regions of the end-of-launch split) under random grid, tile
+    and workgroup shapes, every form, against the oracle."""
+    bt, b, A, B, (cd, tile, zd, nb) = _large_config(seed)
+    ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu, b.pbin if bt == F.BIN_SPI else None)
+    bs = b.bsize_rescaled
+    print(f"case random large seed={seed} bt={bt} tab={b.tabtype} ns={b.ns} nA={A.n} nB={B.n} "
+          f"shape={(cd, tile, zd, nb)}", flush=True)
+    ref_a, cen_a = O.count(0, bs, (A.x, A.y, A.z, None, None), None, True, ob, variant=1,
+                           census_mask=0b11111)
+    ref_x, cen_x = O.count(0, bs, (A.x, A.y, A.z, None, None), (B.x, B.y, B.z, None, None),
+                           False, ob, variant=1, census_mask=0b11111)
+    ref_xw, _ = O.count(0, bs, (A.x, A.y, A.z, None, A.w), (B.x, B.y, B.z, None, None), False,
+                        ob, variant=1, weighted=True)
+    try:
+        eng.set_tuning(cd, tile, zd, nb)
+        for prec in ("fp64", "fp32", "plain"):
+            g = eng.count_pairs(A, None, b, isauto=True, precision=prec)
+            assert np.abs(g - ref_a).sum() <= 2 * cen_a
+            if cen_a == 0:
+                np.testing.assert_array_equal(g, ref_a)
+            g = eng.count_pairs(A, B, b, precision=prec)
+            assert np.abs(g - ref_x).sum() <= 2 * cen_x
+            if cen_x == 0:
+                np.testing.assert_array_equal(g, ref_x)
+            g = eng.count_pairs(A, B, b, withwt=True, precision=prec)
+            scale = np.maximum(np.abs(ref_xw), 1e-300)
+            assert np.max(np.abs(g - ref_xw) / scale) <= WT_RTOL
+    finally:
+        eng.set_tuning(0, 0, 0, 0)
