@@ -1,7 +1,8 @@
 """GPU parity over seeded random configurations: geometry, binning type,
 integer or hybrid bin edges, number of mu bins, lower pi edge, box shape,
 catalogue sizes and clustering, auto or cross (in both catalogue orders),
-every form, single and batched counts, and the weight gradient -- against
+every form, single and batched counts, prepared plans (HIP graph replays)
+and the weight gradient -- against
 the oracle (counts identical when the ambiguity census is zero, weighted
 counts within 1e-12 relative, gradients within 1e-6)."""
 import numpy as np
@@ -116,6 +117,19 @@ def test_random_config(eng, seed):
     same(ba, ref_a, cen_a)
     same(bx, ref_x, cen_x)
     close(bxw, ref_xw)
+    # a prepared plan on device copies of the catalogues, replayed twice
+    import torch
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    dA = F.Catalog(b, dev(A.x), dev(A.y), dev(A.z), dev(A.w))
+    dB = F.Catalog(b, dev(B.x), dev(B.y), dev(B.z), dev(B.w))
+    plan = eng.plan([(dA, None, True, False), (dA, dB, False, True)], b)
+    try:
+        for _ in range(2):
+            pa, pxw = plan.run()
+            same(pa, ref_a, cen_a)
+            close(pxw, ref_xw)
+    finally:
+        plan.close()
     # weight gradients (pairs binned as the count bins them: equal bins
     # when the census is zero)
     if cen_a == 0 and cen_x == 0:
