@@ -21,13 +21,15 @@ def eng():
         yield e
 
 
-def _config(seed):
+def _config(seed, lfrac=0.4):
+    """Configuration of one seed; separations up to lfrac of the shortest
+    box side (the reference requires less than half)."""
     rng = np.random.default_rng(1000 + seed)
     geom = F.GEOM_BOX if seed % 3 else F.GEOM_SURVEY
     bt = (F.BIN_ISO, F.BIN_SMU, F.BIN_SPI)[(seed // 3) % 3]
     if geom == F.GEOM_BOX:
         Lb = rng.choice([1000.0, 600.0, 850.0], 3) if rng.random() < 0.3 else np.full(3, 1000.0)
-        lmax = 0.4 * Lb.min()
+        lmax = lfrac * Lb.min()
     else:
         Lb, lmax = None, 300.0
     if rng.random() < 0.5:            # integer-friendly edges
@@ -77,8 +79,8 @@ def _config(seed):
 # seed 8: hybrid edges whose table does not fit in LDS; seed 127: a hybrid
 # lower edge just below an integer, inside the FP32 window of the next cell
 @pytest.mark.parametrize("seed", list(range(48)) + [127])
-def test_random_config(eng, seed):
-    geom, bt, b, A, B = _config(seed)
+def test_random_config(eng, seed, lfrac=0.4):
+    geom, bt, b, A, B = _config(seed, lfrac)
     ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu, b.pbin if bt == F.BIN_SPI else None)
     og, bs, var, cmask = ((0, b.bsize_rescaled, 1, 0b11111) if geom == F.GEOM_BOX
                           else (1, None, 0, 0b111111))
@@ -145,6 +147,13 @@ def test_random_config(eng, seed):
         ra, rb = O.weight_grad(og, bs, wA, wB, False, ob, G)
         grad_close(ga, ra)
         grad_close(gb, rb)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_config_near_half_box(eng, seed):
+    """Separations up to 0.49 of the shortest box side: periodic images of
+    the strips as close to aliasing as the reference allows."""
+    test_random_config(eng, seed, lfrac=0.49)
 
 
 def _large_config(seed):
