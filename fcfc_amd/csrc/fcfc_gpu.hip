@@ -2388,7 +2388,8 @@ extern "C" int fcfc_gpu_plan_destroy(fcfc_gpu_plan *P) {
 /* Project-written autotuner: for jobs like (a, b, bins, flags) try grid and
  * tile shapes (cells per maximum separation, z-cells per cell, A objects
  * per tile), timing each on the first eighth of the tile list (a slab of the
- * box at the true density) and extrapolating to all tiles.  Coordinate
+ * box at the true density) and extrapolating to all tiles (counts of at most
+ * 4096 tiles are timed whole).  Coordinate
  * descent over cell_div, then zdiv, then tile.  The winner is cached in the
  * context for this geometry / binning / precision and density bucket and
  * used by later counts whose shape is left automatic. */
@@ -2425,10 +2426,12 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     int e = run();
     if (e) return e;
     const int nt = c->last_ntiles;
-    c->tile_cap = std::max(1, nt / 8);
+    /* small counts (a few thousand tiles, milliseconds) are timed whole: an
+     * eighth of them is too few tiles to rank shapes */
+    c->tile_cap = (nt <= 4096) ? 0 : std::max(1, nt / 8);
     e = run();
     if (e) return e;
-    *est = c->stats.kernel_ms * (double) nt / (double) c->tile_cap;
+    *est = c->stats.kernel_ms * (c->tile_cap ? (double) nt / (double) c->tile_cap : 1.0);
     last_T = c->stats.tile;
     return 0;
   };
