@@ -185,7 +185,8 @@ def _large_config(seed):
 def test_random_large(eng, seed):
     """Larger catalogues (tiles of two objects per lane, many work items per
     tile, both regions of the end-of-launch split) under random grid, tile
-    and workgroup shapes, every form, against the oracle."""
+    and workgroup shapes, every form and the weight gradient, against the
+    oracle."""
     bt, b, A, B, (cd, tile, zd, nb) = _large_config(seed)
     ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu, b.pbin if bt == F.BIN_SPI else None)
     bs = b.bsize_rescaled
@@ -211,5 +212,16 @@ def test_random_large(eng, seed):
             g = eng.count_pairs(A, B, b, withwt=True, precision=prec)
             scale = np.maximum(np.abs(ref_xw), 1e-300)
             assert np.max(np.abs(g - ref_xw) / scale) <= WT_RTOL
+        if cen_a == 0 and cen_x == 0:
+            # weight gradients of A (auto and cross, A's catalogue first
+            # and second) under the same shapes
+            G = np.random.default_rng(100 + seed).normal(size=b.ntot)
+            ra, _ = O.weight_grad(0, bs, (A.x, A.y, A.z, None, A.w), None, True, ob, G)
+            rx, _ = O.weight_grad(0, bs, (A.x, A.y, A.z, None, A.w), (B.x, B.y, B.z, None, None),
+                                  False, ob, G)
+            for g, r in ((eng.count_backward(A, None, b, G, isauto=True)[0], ra),
+                         (eng.count_backward(A, B, b, G, need_b=False)[0], rx),
+                         (eng.count_backward(B, A, b, G)[1], rx)):
+                np.testing.assert_allclose(g, r, rtol=1e-6, atol=1e-9 * np.abs(r).max())
     finally:
         eng.set_tuning(0, 0, 0, 0)
