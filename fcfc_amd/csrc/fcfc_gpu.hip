@@ -2442,7 +2442,7 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     if ((e = measure(cd, 8, 0, &est))) break;
     if (est < bestms) { bestms = est; bcd = cd; bT = last_T; }
   }
-  const int zds[] = {4, 8, 16};
+  const int zds[] = {1, 2, 4, 8, 16};
   for (int zd : zds) {
     double est;
     if (e || (e = measure(bcd, zd, 0, &est))) break;
