@@ -73,11 +73,17 @@ def run_step(eng, b, D, R, prec):
     return out, ev, kms, rc
 
 
-def cpu_baseline(target_s=15.0):
+def cpu_baseline(target_s=6.0, runs=3):
     """The upstream FCFC_2PT_BOX (OpenMP + SIMD build from the unmodified
-    sources, oracle/_ref) on a bounded sample of the same workload: RR of a
-    uniform subsample, (s,mu) 40x100; count-phase time from its own progress
-    lines.  Returns in-range pairs per second."""
+    sources, oracle/_ref) on the host cores, OMP_PROC_BIND=spread
+    OMP_PLACES=cores, count-phase time from its own progress lines:
+      * a bounded sample of the headline workload: RR of a uniform subsample
+        in the L = 1000 box, (s,mu) 40x100, sized for ~target_s per run,
+        median of `runs` runs;
+      * the C1 configuration as specified (N = 1e5 uniform, DD, iso 40 bins).
+    Returns the baseline record plus the catalogues and the reference's
+    counts, which the caller compares with the GPU's (the reference is the
+    checker there, never the thing timed for `value`)."""
     from oracle import oracle as O
     import fcfc_amd as F
     flav = O.host_flavour()
@@ -85,27 +91,69 @@ def cpu_baseline(target_s=15.0):
     if not os.path.exists(exe):
         return None
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env = {"OMP_PROC_BIND": "spread", "OMP_PLACES": "cores"}
     L = 1000.0
 
-    def one(n):
+    def rr(n, wd=None):
         x, y, z = F.gen_uniform(777, n, L)
         res, phase, wall, _ = O.run_reference("FCFC_2PT_BOX", flav, [(x, y, z)], ["R"], ["RR"],
-                                              1, 0, 200, 5, nmu=100, bsize=L, threads=threads)
-        # FCFC doubles auto counts: unordered in-range pairs = sum / 2
-        return float(res["RR"]["cnt"].sum()) / 2, phase["RR"], wall
+                                              1, 0, 200, 5, nmu=100, bsize=L, threads=threads,
+                                              extra_env=env, workdir=wd, reuse_files=wd is not None)
+        return (x, y, z), res["RR"]["cnt"].astype(np.int64), phase["RR"], wall
 
-    n = 400000
-    pairs, t, _ = one(n)
-    rate = pairs / max(t, 1e-6)
-    n2 = int(min(8e6, max(n, n * np.sqrt(target_s / max(t, 1e-3)))))
-    if n2 > 1.3 * n:
-        pairs, t, _ = one(n2)
-        n = n2
-        rate = pairs / max(t, 1e-6)
-    return {"value": rate, "unit": "unordered in-range pairs/s", "cores": threads, "kind": "reference",
-            "sample": f"upstream FCFC_2PT_BOX ({flav} build, OpenMP {threads} threads), RR auto "
-                      f"count of {n} uniform points in the L=1000 box, (s,mu) 40x100; "
-                      f"count phase {t:.2f} s, {pairs:.4g} unordered in-range pairs"}
+    _, c0, t0, _ = rr(400000)
+    n = int(min(8e6, max(400000, 400000 * np.sqrt(target_s / max(t0, 1e-3)))))
+    times, cnts = [], []
+    import shutil
+    import tempfile
+    wd = tempfile.mkdtemp(prefix="fcfc_base_")
+    try:
+        for _ in range(runs):
+            cat, cnt, t, _ = rr(n, wd)
+            times.append(t)
+            cnts.append(cnt)
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+    tmed = float(np.median(times))
+    pairs = float(cnts[0].sum()) / 2          # the file holds doubled auto counts
+    xc, yc, zc = F.gen_uniform(1001, 100000, L)
+    res, phase, wall, _ = O.run_reference("FCFC_2PT_BOX", flav, [(xc, yc, zc)], ["D"], ["DD"],
+                                          0, 0, 200, 5, bsize=L, threads=threads, extra_env=env)
+    c1 = res["DD"]["cnt"].astype(np.int64)
+    rec = {"value": pairs / tmed, "unit": "unordered in-range pairs/s", "cores": threads,
+           "kind": "reference",
+           "sample": f"upstream FCFC_2PT_BOX ({flav} build, OpenMP {threads} threads, "
+                     f"OMP_PROC_BIND=spread OMP_PLACES=cores), RR auto count of {n} uniform "
+                     f"points in the L=1000 box, (s,mu) 40x100; count phase median of {runs} "
+                     f"runs {tmed:.2f} s ({', '.join(f'{t:.2f}' for t in times)}), "
+                     f"{pairs:.4g} unordered in-range pairs",
+           "runs_identical": all(np.array_equal(c, cnts[0]) for c in cnts),
+           "c1": {"value": float(c1.sum()) / 2 / phase["DD"], "unit": "unordered in-range pairs/s",
+                  "count_phase_s": phase["DD"], "wall_s": wall,
+                  "sample": "C1 as specified: N=1e5 uniform, DD auto, iso 40 bins in [0,200)"}}
+    return {"record": rec, "rr": (cat, cnts[0]), "c1": ((xc, yc, zc), c1)}
+
+
+def parity_vs_reference(eng, base, forms=("fp64", "fp32", "plain")):
+    """The GPU counts of the baseline's catalogues in every form against the
+    reference's own counts (auto counts doubled as in its files)."""
+    import fcfc_amd as F
+    L = 1000.0
+    out = {}
+    for key, bt in (("rr_sample", F.BIN_SMU), ("c1", F.BIN_ISO)):
+        (x, y, z), ref = base[key]
+        b = F.Bins.linear(F.GEOM_BOX, bt, 0, 200, 5, nmu=100 if bt == F.BIN_SMU else 1, bsize=L)
+        c = F.Catalog(b, x, y, z)
+        rec = {"n": int(c.n), "reference_pairs": int(ref.sum()) // 2}
+        for prec in forms:
+            got = 2 * eng.count_pairs(c, None, b, isauto=True, withwt=False, precision=prec)
+            st = eng.stats()
+            rec[prec] = {"identical": bool(np.array_equal(got, ref)),
+                         "abs_diff": int(np.abs(got - ref).sum()),
+                         "kernel_ms": st["kernel_ms"],
+                         "inrange_pairs_per_s": float(ref.sum()) / 2 / (st["kernel_ms"] * 1e-3)}
+        out[key] = rec
+    return out
 
 
 PROFILE = os.path.join(ROOT, "profiles", "k_count_smu_box_RR_counters.txt")
@@ -144,40 +192,61 @@ def issue_from_profile(fn=PROFILE):
     return out or None
 
 
-def mocks_workload(args):
-    """256 periodic-box mocks (L = 1000): sizes log-uniform in [5e4, 5e5] from
-    the seed (times --mock-scale), each half uniform / half in halos
-    (sigma = 2) with lognormal weights, 5x uniform randoms; DD/DR/RR in
-    (s,mu) 40x100 and xi_0,2,4.  All 768 counts in one batched call (one
-    persistent launch per kernel variant) vs one call per count; shapes from
-    the autotuner for both (run on a few mocks across the size range)."""
+def gpu_mocks(b, nm, mock_scale, seed=4040):
+    """256 periodic-box mocks (L = 1000) generated on the GPU from a seeded
+    torch generator: sizes log-uniform in [5e4, 5e5] (times mock_scale),
+    half uniform and half in halos of 500 members (sigma = 2, wrapped), with
+    lognormal(0, 0.3) weights rounded to float32; 5x uniform randoms.
+    Returns (sizes, [(D, R) device catalogues, rescaled])."""
     import torch
     import fcfc_amd as F
-    from fcfc_amd.xi import multipoles
     L = 1000.0
-    nm = max(1, int(round(256 * args.scale)))
-    sizes = F.gen_sizes(4040, nm, 5e4 * args.mock_scale, 5e5 * args.mock_scale)
-    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=100, bsize=L)
+    f = b.rescale
+    sizes = F.gen_sizes(seed, nm, 5e4 * mock_scale, 5e5 * mock_scale)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    kw = dict(generator=g, device="cuda", dtype=torch.float64)
+
+    def wrap(p):
+        p = torch.remainder(p, L)
+        return torch.where(p >= L, p - L, p)
+
+    def cat(p, w=None):
+        p = p * f
+        return F.Catalog(b, p[:, 0].contiguous(), p[:, 1].contiguous(), p[:, 2].contiguous(), w)
+
     mocks = []
-    for k, n in enumerate(sizes):
+    for n in sizes:
         n = int(n)
         nh = max(1, n // 1000)
-        x, y, z, w = F.gen_clustered(9000 + k, max(n - nh * 500, 1), nh, 500, 2.0, L, 0.3)
-        xr, yr, zr = F.gen_uniform(19000 + k, 5 * n, L)
-        mocks.append((to_device(b, (x, y, z, w), F, torch), to_device(b, (xr, yr, zr), F, torch)))
+        nu = max(n - nh * 500, 1)
+        cen = torch.rand((nh, 1, 3), **kw) * L
+        halo = (cen + 2.0 * torch.randn((nh, 500, 3), **kw)).reshape(-1, 3)
+        p = wrap(torch.cat([torch.rand((nu, 3), **kw) * L, halo]))
+        w = torch.exp(0.3 * torch.randn(p.shape[0], **kw)).to(torch.float32).to(torch.float64)
+        r = torch.rand((5 * n, 3), **kw) * L
+        mocks.append((cat(p, w), cat(torch.where(r >= L, r - L, r))))
+    torch.cuda.synchronize()
+    return sizes, mocks
+
+
+def mocks_run(eng, b, sizes, mocks, prec, steps=1, warmup=0, tune=True):
+    """All 768 counts (DD, DR, RR of every mock) in one batched call (one
+    persistent launch per kernel variant) against one call per count, with
+    xi_0,2,4 of every mock; shapes from the autotuner for both (run on mocks
+    of four sizes)."""
+    import torch
+    from fcfc_amd.xi import multipoles
+    nm = len(mocks)
     jobs = []
     for D, R in mocks:
         jobs += [(D, None, True, True), (D, R, False, True), (R, None, True, False)]
-    torch.cuda.synchronize()
-    log(f"{nm} mocks ready, {sum(int(v) for v in sizes)} data objects")
-    eng = F.Engine(0)
-    prec = args.precision
-    if not args.no_autotune:
+    if tune:
         order = np.argsort(sizes)
         for k in sorted(set(int(order[int(q * (nm - 1))]) for q in (0.0, 0.33, 0.67, 1.0))):
             for a, c, au, wt in jobs[3 * k:3 * k + 3]:
                 eng.autotune(a, c, b, isauto=au, withwt=wt, precision=prec)
-        log("autotuned on mocks of four sizes")
+        log("mocks: autotuned on mocks of four sizes")
 
     def xi_all(res):
         """Landy-Szalay xi(s, mu) and xi_0,2,4 of every mock (the same code
@@ -189,42 +258,61 @@ def mocks_workload(args):
             drn = dr * (1 / (D.sumw * R.n))
             rrn = rr * (2 / (float(R.n) * (R.n - 1)))
             xi = (ddn - 2 * drn + rrn) / np.where(rrn > 0, rrn, np.inf)
-            out.append(multipoles(xi.reshape(100, 40), 100, (0, 2, 4)))
+            out.append(multipoles(xi.reshape(b.nmu, b.ns), b.nmu, (0, 2, 4)))
         return out
 
-    for i in range(args.warmup):
+    for i in range(warmup):
         eng.count_batch(jobs, b, precision=prec)
     torch.cuda.synchronize()
     t0 = time.time()
-    for i in range(args.steps):
+    for i in range(steps):
         res = eng.count_batch(jobs, b, precision=prec)
         xi_b = xi_all(res)
-        log(f"batched step {i} done")
+        log(f"mocks: batched pass {i} done")
     torch.cuda.synchronize()
-    tb = (time.time() - t0) / args.steps
+    tb = (time.time() - t0) / steps
     stb = eng.stats()
-    # the same counts, one call per catalogue pair
     t1 = time.time()
     res1 = [eng.count_pairs(a, c, b, isauto=au, withwt=wt, precision=prec) for a, c, au, wt in jobs]
     xi_s = xi_all(res1)
     torch.cuda.synchronize()
     ts = time.time() - t1
+    log("mocks: per-count pass done")
     exact = all(np.array_equal(res[3 * k + 2], res1[3 * k + 2]) for k in range(nm))
     wrel = max(float(np.max(np.abs(res[j] - res1[j]) / np.maximum(np.abs(res1[j]), 1e-300)))
                for j in range(len(jobs)) if jobs[j][3])
+    xrel = max(float(np.nanmax(np.abs(a - c) / np.maximum(np.abs(c), 1e-300)))
+               for a, c in zip(xi_b, xi_s))
+    return {"mocks_per_s": nm / tb, "per_catalogue_mocks_per_s": nm / ts,
+            "speedup_vs_per_catalogue": ts / tb, "batched_ms": tb * 1e3, "per_count_ms": ts * 1e3,
+            "evals_per_s": stb["evals"] / tb, "kernel_ms": stb["kernel_ms"],
+            "n_mocks": nm, "n_data_total": int(sum(int(v) for v in sizes)),
+            "counts_identical_batched_vs_single": {"RR": exact, "weighted_max_rel": wrel,
+                                                   "xi_max_rel": xrel}}
+
+
+def mocks_workload(args):
+    """--workload mocks: the batched-mocks configuration on its own line."""
+    import fcfc_amd as F
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=100, bsize=1000.0)
+    nm = max(1, int(round(256 * args.scale)))
+    sizes, mocks = gpu_mocks(b, nm, args.mock_scale)
+    log(f"{nm} mocks ready, {sum(int(v) for v in sizes)} data objects")
+    eng = F.Engine(0)
+    prec = args.precision
+    m = mocks_run(eng, b, sizes, mocks, prec, steps=args.steps, warmup=args.warmup,
+                  tune=not args.no_autotune)
     line = {
-        "metric": METRIC, "value": nm / tb, "unit": "mocks/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": tb * 1e3,
+        "metric": METRIC, "value": m["mocks_per_s"], "unit": "mocks/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["batched_ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": f"{nm} box mocks L=1000, N log-uniform in "
                                f"[{5e4 * args.mock_scale:.3g},{5e5 * args.mock_scale:.3g}], 5x randoms, "
                                "DD+DR+RR (s,mu) 40x100 -> xi_0,2,4, batched",
-                   "n_data_total": int(sum(int(v) for v in sizes)), "precision": prec},
-        "per_catalogue_mocks_per_s": nm / ts, "speedup_vs_per_catalogue": ts / tb,
-        "evals_per_s": stb["evals"] / tb, "kernel_ms": stb["kernel_ms"],
-        "counts_identical_batched_vs_single": {"RR": exact, "weighted_max_rel": wrel},
+                   "n_data_total": m["n_data_total"], "precision": prec},
     }
+    line.update({k: v for k, v in m.items() if k not in ("mocks_per_s",)})
     eng.close()
     print(json.dumps(line), flush=True)
 
@@ -390,8 +478,10 @@ def main():
                     help="fraction of the workload sizes (1 = the headline configuration)")
     ap.add_argument("--precision", choices=["fp64", "fp32", "plain"], default="fp64",
                     help="form of the timed steps (the others run one step each)")
-    ap.add_argument("--skip", nargs="*", default=[], choices=["fp64", "fp32", "plain"],
-                    help="forms not to run for comparison")
+    ap.add_argument("--skip", nargs="*", default=[],
+                    choices=["fp64", "fp32", "plain", "census", "mocks"],
+                    help="forms not to run for comparison; census: no full-size census "
+                         "count; mocks: no batched-mocks leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mock-scale", type=float, default=1.0,
                     help="mocks workload: factor on the catalogue sizes")
@@ -527,11 +617,55 @@ def main():
                 "issue_profile": issue_from_profile(PROFILE32)}
         log(f"{other} step done: {evo / dto:.4g} evals/s")
     line["forms"] = forms
-    if "plain" in forms:
-        line["speedup_vs_plain_fp64"] = value / forms["plain"]["evals_per_s"]
+    mine = {"evals_per_s": value, "ms_per_step": dt / args.steps * 1e3, "kernel_ms": line["kernel_ms"]}
+    allf = dict(forms, **{prec: mine})
+    if "plain" in allf and prec != "plain":
+        line["speedup_vs_plain_fp64"] = value / allf["plain"]["evals_per_s"]
+    if "fp32" in allf and "fp64" in allf:
+        f32, f64 = allf["fp32"], allf["fp64"]
+        line["speedup_fp32_vs_fp64_form"] = {
+            "step": f64["ms_per_step"] / f32["ms_per_step"],
+            "RR_kernel": f64["kernel_ms"]["RR"] / f32["kernel_ms"]["RR"],
+            "DD_kernel": f64["kernel_ms"]["DD"] / f32["kernel_ms"]["DD"],
+            "DR_kernel": f64["kernel_ms"]["DR"] / f32["kernel_ms"]["DR"]}
+    if "plain" in allf and "fp32" in allf:
+        line["speedup_fp32_vs_plain_fp64"] = {
+            "step": allf["plain"]["ms_per_step"] / allf["fp32"]["ms_per_step"],
+            "RR_kernel": allf["plain"]["kernel_ms"]["RR"] / allf["fp32"]["kernel_ms"]["RR"]}
     line["counts_bit_exact"] = all(f["RR_identical"] and f["weighted_max_rel"] <= 1e-12
                                    for f in forms.values())
-    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
+    # the rounding census of the full-size RR count: the FP32 kernel evaluates
+    # the reference's other rounding paths for every pair it re-evaluates in
+    # FP64; census 0 means every path of the reference bins every pair as the
+    # GPU does
+    if "census" not in args.skip:
+        rc = eng.count_pairs(R, None, b, isauto=True, withwt=False, precision="fp32", census=True)
+        st = eng.stats()
+        line["census_RR_full_size"] = {"census": st["census"], "rechecked": st["rechecked"],
+                                       "evals": st["evals"], "kernel_ms": st["kernel_ms"],
+                                       "counts_identical": bool(np.array_equal(rc, res["RR"]))}
+        log(f"census RR: {st['census']} of {st['rechecked']} re-evaluated pairs")
+    # batched mocks at the configured sizes (256 mocks, N in [5e4, 5e5])
+    if "mocks" not in args.skip:
+        nm = max(1, int(round(256 * args.scale)))
+        del D, R
+        torch.cuda.empty_cache()
+        sizes, mocks = gpu_mocks(b, nm, args.mock_scale)
+        log(f"mocks: {nm} ready, {int(sum(int(v) for v in sizes))} data objects")
+        line["mocks"] = mocks_run(eng, b, sizes, mocks, prec, tune=not args.no_autotune)
+        line["mocks"]["config"] = (f"{nm} box mocks L=1000, N log-uniform in [{5e4 * args.mock_scale:.3g},"
+                                   f"{5e5 * args.mock_scale:.3g}], 5x randoms, DD+DR+RR (s,mu) "
+                                   "40x100 -> xi_0,2,4, one batched call vs one call per count")
+        del mocks
+        torch.cuda.empty_cache()
+    base = None if args.no_cpu else cpu_baseline()
+    line["cpu_baseline"] = base["record"] if base else None
+    if base:
+        line["parity_vs_reference"] = parity_vs_reference(
+            eng, base, forms=tuple(f for f in ("fp64", "fp32", "plain") if f not in args.skip))
+        log("parity vs reference: " + json.dumps(
+            {k: {f: v[f]["identical"] for f in v if isinstance(v[f], dict)}
+             for k, v in line["parity_vs_reference"].items()}))
     eng.close()
     print(json.dumps(line), flush=True)
 

@@ -259,27 +259,146 @@ static int read_ascii(const char *fname, long skip, char cmt, const char *fmt,
 }
 
 /* ---------------------------------------------------------------------- */
-/* pair-count files: reader for the resume path (read_res.c:256-...)       */
-static int read_pair_file(const char *fname, const fcfc_bins *b, int weighted, void *cnt, double *norm) {
+/* pair-count files: reader for the resume path (OVERWRITE = 1).           */
+/* As read_pair_count (src/fcfc/2pt_box/read_res.c:256-560, survey twin    */
+/* src/fcfc/2pt/read_res.c) it takes the NORMALISED counts from the file   */
+/* and checks the settings against the current configuration: the header   */
+/* line "# Created by <code> <version> (<vernum>) ; format = <0|1>", then  */
+/*  binary: blocks [u64 size][payload][u64 size] read by position --       */
+/*    spec int32[7] (iso) or [8]; precision spec[0] (0: FP32 edges/box);    */
+/*    spec[2] periodic (box) or not (survey); spec[5] binning; spec[6] ns;  */
+/*    spec[7] nmu or np -- the box size (rescaled, box only), the s edges,  */
+/*    the pi edges (s_perp,pi), the object numbers (1 or 2, u64 or f64),    */
+/*    the normalisation, the normalised counts, the raw counts; edges and   */
+/*    box within 1e-10 (1e-5 for FP32 files);                              */
+/*  ASCII: comment lines with the column header on the 4th line, then ntot  */
+/*    rows whose last column is the normalised count.                      */
+/* Unlike the reference, a file that ends before its count blocks, or a    */
+/* pi-bin number that differs (reported there without stopping), fails.   */
+static int read_pair_file(const char *fname, const fcfc_bins *b, const double *bres,
+    double *ncnt) {
   FILE *fp = fopen(fname, "rb");
-  if (!fp) return FCFC_ERR_FILE;
-  int ch;
-  while ((ch = fgetc(fp)) != EOF && ch != '\n') {}
-  if (ch == EOF) { fclose(fp); return FCFC_ERR_FILE; }
-  uint64_t sz, sz2;
-  int blk = 0, e = 0;
-  char *buf = NULL;
-  while (fread(&sz, sizeof sz, 1, fp) == 1) {
-    buf = realloc(buf, sz ? sz : 1);
-    if ((sz && fread(buf, sz, 1, fp) != 1) || fread(&sz2, sizeof sz2, 1, fp) != 1 || sz2 != sz) { e = FCFC_ERR_FILE; break; }
-    if (blk == 0) {
-      const int32_t *spec = (const int32_t *) buf;
-      if (spec[5] != b->bintype || spec[6] != b->ns || spec[4] != weighted) { e = FCFC_ERR_FILE; break; }
-    }
-    blk++;
-    if (sz == sizeof(double) && norm) memcpy(norm, buf, sizeof(double));
-    if (sz == (uint64_t) b->ntot * 8) memcpy(cnt, buf, sz);   /* last ntot-block: raw counts */
+  if (!fp) { P_ERR("cannot open the pair count file `%s'\n", fname); return FCFC_ERR_FILE; }
+  char head[1024];
+  if (!fgets(head, sizeof head, fp)) { fclose(fp); P_ERR("empty pair count file `%s'\n", fname); return FCFC_ERR_FILE; }
+  char code[64];
+  int ver = 0, fmt = -1;
+  if (sscanf(head, "# Created by %63s %*s (%d) ; format = %d", code, &ver, &fmt) != 3 ||
+      strcmp(code, CODE_NAME)) {
+    fclose(fp); P_ERR("unrecognized pair count file: `%s'\n", fname); return FCFC_ERR_FILE;
   }
+  const int spi = (b->bintype == FCFC_BIN_SPI), iso = (b->bintype == FCFC_BIN_ISO);
+  int e = 0;
+  if (fmt == 1) {                                   /* ASCII (save_res.c:45-164) */
+    const char *hdr = iso ? "Columns: s_min(1) s_max(2) normalized_pair_count(3)"
+        : (spi ? "Columns: s_perp_min(1) s_perp_max(2) pi_min(3) pi_max(4) normalized_pair_count(5)"
+               : "Columns: s_min(1) s_max(2) mu_min(3) mu_max(4) normalized_pair_count(5)");
+    char *line = NULL; size_t cap = 0;
+    int lno = 1, n = 0;
+    while (getline(&line, &cap, fp) >= 0) {
+      lno++;
+      if (line[0] == '#') {
+        if (lno == 4 && strncmp(line + 2, hdr, strlen(hdr))) {
+          P_ERR("unexpected separation bins in pair count file: `%s'\n", fname);
+          e = FCFC_ERR_FILE; break;
+        }
+        continue;
+      }
+      if (n == 0 && lno != 5) {
+        P_ERR("invalid number of header lines in pair count file: `%s'\n", fname);
+        e = FCFC_ERR_FILE; break;
+      }
+      char *p = trim(line);
+      if (!*p) continue;
+      if (n == b->ntot) { P_ERR("too many rows in pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; break; }
+      double v;
+      if (sscanf(p, iso ? "%*f %*f %lf" : "%*f %*f %*f %*f %lf", &v) != 1) {
+        P_ERR("failed to read line %d of file: `%s'\n", lno, fname); e = FCFC_ERR_FILE; break;
+      }
+      ncnt[n++] = v;
+    }
+    free(line);
+    fclose(fp);
+    if (!e && n != b->ntot) {
+      P_ERR("unexpected number of bins (%d) in pair count file: `%s'\n", n, fname);
+      e = FCFC_ERR_FILE;
+    }
+    return e;
+  }
+  if (fmt != 0) { fclose(fp); P_ERR("unrecognized pair count file format: %d\n", fmt); return FCFC_ERR_FILE; }
+  /* binary: block k of the layout, by position */
+  const int nblk = 7 + (FCFC_CLI_GEOM == 0 ? 1 : 0) + (spi ? 1 : 0) - 1;
+  int fp32 = 0, isauto = 0, wt = 0, k = 0;
+  char *buf = NULL;
+  uint64_t sz, sz2;
+  for (; k < nblk; k++) {
+    if (fread(&sz, sizeof sz, 1, fp) != 1 || sz > ((uint64_t) 1 << 32)) break;
+    char *nb = realloc(buf, sz ? sz : 1);
+    if (!nb) { e = FCFC_ERR_MEMORY; break; }
+    buf = nb;
+    if ((sz && fread(buf, sz, 1, fp) != 1) || fread(&sz2, sizeof sz2, 1, fp) != 1 || sz2 != sz) {
+      P_ERR("invalid format of pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; break;
+    }
+    /* the role of block k */
+    int role = k;                          /* 0 spec, 1 box, 2 s, 3 pi, 4 n, 5 norm, 6 ncnt, 7 cnt */
+    if (FCFC_CLI_GEOM != 0 && role >= 1) role++;
+    if (!spi && role >= 3) role++;
+    const size_t fsz = fp32 ? sizeof(float) : sizeof(double);
+    if (role == 0) {
+      int32_t spec[8] = {0};
+      if (sz != (uint64_t) (iso ? 7 : 8) * sizeof(int32_t)) { e = FCFC_ERR_FILE; P_ERR("invalid format of pair count file: `%s'\n", fname); break; }
+      memcpy(spec, buf, sz);
+      fp32 = (spec[0] == 0);
+      if (fp32) fprintf(stderr, "Warning: pair count file was generated with single precision: `%s'\n", fname);
+      if (spec[1] == 1 && b->bintype == FCFC_BIN_SMU)
+        fprintf(stderr, "Warning: pair count file was generated with `-DWITH_MU_ONE`: `%s'\n", fname);
+      if (spec[2] != (FCFC_CLI_GEOM == 0 ? 1 : 0)) {
+        P_ERR("%s pair counts in file: `%s'\n", FCFC_CLI_GEOM == 0 ? "non-periodic" : "periodic", fname);
+        e = FCFC_ERR_FILE; break;
+      }
+      isauto = (spec[3] == 0);
+      wt = (spec[4] == 1);
+      if (spec[5] != b->bintype) { P_ERR("unexpected binning scheme in pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; break; }
+      if (spec[6] != b->ns) { P_ERR("unexpected number of separation bins (%d) in pair count file: `%s'\n", spec[6], fname); e = FCFC_ERR_FILE; break; }
+      if (!iso && spec[7] != (spi ? b->np : b->nmu)) {
+        P_ERR("unexpected number of %s bins (%d) in pair count file: `%s'\n", spi ? "pi" : "mu", spec[7], fname);
+        e = FCFC_ERR_FILE; break;
+      }
+    }
+    else if (role == 1 || role == 2 || role == 3) {
+      const int len = role == 1 ? 3 : (role == 2 ? b->ns + 1 : b->np + 1);
+      const double *want = role == 1 ? bres : (role == 2 ? b->sbin_raw : b->pbin_raw);
+      const double tol = fp32 ? 1e-5 : 1e-10;
+      if (sz != (uint64_t) len * fsz) { P_ERR("invalid format of pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; break; }
+      for (int i = 0; i < len; i++) {
+        double v;
+        if (fp32) { float f; memcpy(&f, buf + i * sizeof f, sizeof f); v = f; }
+        else memcpy(&v, buf + i * sizeof v, sizeof v);
+        if (fabs(v - want[i]) > tol) {
+          P_ERR("unexpected %s in pair count file: `%s'\n",
+                role == 1 ? "box size" : (role == 2 ? "separation bin" : "pi bin"), fname);
+          e = FCFC_ERR_FILE; break;
+        }
+      }
+      if (e) break;
+    }
+    else if (role == 4) {
+      if (sz != (uint64_t) 8 * (isauto ? 1 : 2)) { P_ERR("invalid format of pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; break; }
+    }
+    else if (role == 5) {
+      if (sz != 8) { P_ERR("invalid format of pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; break; }
+    }
+    else if (role == 6) {
+      if (sz != (uint64_t) b->ntot * 8) { P_ERR("invalid format of pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; break; }
+      memcpy(ncnt, buf, sz);
+    }
+    else {
+      if (sz != (uint64_t) b->ntot * 8) { P_ERR("invalid format of pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; break; }
+    }
+  }
+  (void) wt;
+  if (!e && k < nblk) { P_ERR("pair count file ends before its counts: `%s'\n", fname); e = FCFC_ERR_FILE; }
+  if (!e && fread(&sz, 1, 1, fp) == 1) { P_ERR("invalid format of pair count file: `%s'\n", fname); e = FCFC_ERR_FILE; }
   free(buf);
   fclose(fp);
   return e;
@@ -401,7 +520,7 @@ int main(int argc, char **argv) {
   memset(&bins, 0, sizeof bins);
   cat_t *cat = NULL;
   double **cnt = NULL, **ncnt = NULL, *norm = NULL;
-  int *weighted = NULL;
+  int *weighted = NULL, *comp_pc = NULL;
 
   ncat = split_list(kv_get("CATALOG"), &cats);
   if (ncat < 1) { P_ERR("CATALOG is not set\n"); return FCFC_ERR_CFG; }
@@ -518,8 +637,52 @@ int main(int argc, char **argv) {
     for (int s = 0; s < 2; s++) {
       pci[s][i] = -1;
       for (int k = 0; k < ncat; k++) if (lab[k] == pcs[i][s]) pci[s][i] = k;
-      if (pci[s][i] < 0) { P_ERR("catalog not found for pair count: %s\n", pcs[i]); FAIL(FCFC_ERR_CFG); }
     }
+  }
+  /* existing outputs (load_conf.c check_output): OVERWRITE <= 0 aborts,
+   * 1 reads existing pair-count files and rewrites the correlation function
+   * files, >= 2 rewrites everything.  (The reference asks on the terminal
+   * for negative values; here they abort like 0.) */
+  comp_pc = calloc(npc, sizeof(int));
+  for (int i = 0; i < npc; i++) {
+    comp_pc[i] = 1;
+    if (!file_exists(pcout[i])) continue;
+    if (overwrite <= 0) { P_ERR("abort to avoid overwriting PAIR_COUNT_FILE: `%s'\n", pcout[i]); FAIL(FCFC_ERR_FILE); }
+    if (overwrite == 1) comp_pc[i] = 0;
+  }
+  {
+    char **outs[3] = {cfout, mpout, wpout};
+    const int nout[3] = {ncfo, (bintype == FCFC_BIN_SMU && npole) ? nmpo : 0,
+                         (bintype == FCFC_BIN_SPI && want_wp) ? nwpo : 0};
+    const char *keys[3] = {"CF_OUTPUT_FILE", "MULTIPOLE_FILE", "PROJECTED_FILE"};
+    for (int q = 0; q < 3; q++)
+      for (int i = 0; i < nout[q]; i++)
+        if (file_exists(outs[q][i]) && overwrite <= 0) {
+          P_ERR("abort to avoid overwriting %s: `%s'\n", keys[q], outs[q][i]);
+          FAIL(FCFC_ERR_FILE);
+        }
+  }
+  /* MULTIPOLE: sorted, duplicates removed, 0 <= ell <= 6 (load_conf.c) */
+  if (npole > 1) {
+    for (int i = 1; i < npole; i++)
+      for (int j = i; j > 0 && atoi(poles[j - 1]) > atoi(poles[j]); j--) {
+        char *t = poles[j]; poles[j] = poles[j - 1]; poles[j - 1] = t;
+      }
+    int m = 0;
+    for (int i = 1; i < npole; i++)
+      if (atoi(poles[i]) != atoi(poles[m])) { m++; char *t = poles[m]; poles[m] = poles[i]; poles[i] = t; }
+    for (int i = m + 1; i < npole; i++) { free(poles[i]); poles[i] = NULL; }
+    npole = m + 1;
+  }
+  for (int l = 0; l < npole; l++)
+    if (atoi(poles[l]) < 0 || atoi(poles[l]) > 6) { P_ERR("MULTIPOLE must be between 0 and 6\n"); FAIL(FCFC_ERR_CFG); }
+  /* catalogue labels are needed for the pair counts that are computed */
+  for (int i = 0; i < npc; i++)
+    if (comp_pc[i] && (pci[0][i] < 0 || pci[1][i] < 0)) {
+      P_ERR("catalog label not found for PAIR_COUNT: %s\n", pcs[i]); FAIL(FCFC_ERR_CFG);
+    }
+  if (bintype == FCFC_BIN_SMU && npole && ncf && nmpo != ncf) {
+    P_ERR("MULTIPOLE_FILE must have as many entries as CF_ESTIMATOR\n"); FAIL(FCFC_ERR_CFG);
   }
   int *has_wt = calloc(ncat, sizeof(int));
   for (int k = 0; k < ncat; k++) has_wt[k] = (k < nwt && wts[k][0] && strcmp(wts[k], "1"));
@@ -527,20 +690,18 @@ int main(int argc, char **argv) {
 
   for (int i = 0; i < npc; i++) {
     const int a = pci[0][i], b = pci[1][i], isauto = (a == b);
-    weighted[i] = has_wt[a] || has_wt[b];
     cnt[i] = calloc(bins.ntot, sizeof(double));
     ncnt[i] = calloc(bins.ntot, sizeof(double));
-    /* reuse an existing pair-count file (OVERWRITE 1, eval_cf.c:64-85) */
-    if (file_exists(pcout[i]) && overwrite < 2) {
-      if (overwrite == 1 && read_pair_file(pcout[i], &bins, weighted[i], cnt[i], &norm[i]) == 0) {
-        printf("Reading %s pairs ...\n", pcs[i]);
-        for (int k = 0; k < bins.ntot; k++)
-          ncnt[i][k] = (weighted[i] ? cnt[i][k] : (double) ((int64_t *) cnt[i])[k]) / norm[i];
-        continue;
-      }
-      P_ERR("output file exists: `%s' (set OVERWRITE)\n", pcout[i]);
-      FAIL(FCFC_ERR_FILE);
+    /* an existing pair-count file with OVERWRITE = 1: its normalised counts
+     * are read instead of counted (eval_cf.c:64-85) */
+    if (!comp_pc[i]) {
+      printf("Reading %s pairs ...\n", pcs[i]);
+      if (verbose) printf("  Filename: %s\n", pcout[i]);
+      if ((ret = read_pair_file(pcout[i], &bins, bres, ncnt[i]))) goto done;
+      printf("[DONE]\n");
+      continue;
     }
+    weighted[i] = has_wt[a] || has_wt[b];
     for (int s = 0; s < 2; s++) {
       int k = pci[s][i];
       if (cat[k].loaded) continue;
@@ -701,7 +862,7 @@ done:
   if (cat) for (int k = 0; k < ncat; k++) { free(cat[k].x); free(cat[k].y); free(cat[k].z); free(cat[k].w); }
   free(cat);
   if (cnt) for (int i = 0; i < npc; i++) { free(cnt[i]); free(ncnt[i]); }
-  free(cnt); free(ncnt); free(norm); free(weighted);
+  free(cnt); free(ncnt); free(norm); free(weighted); free(comp_pc);
   free_list(cats, ncat); free_list(labels, nlab); free_list(pos, npos); free_list(wts, nwt);
   free_list(fmts, nfmt); free_list(pcs, npc); free_list(pcout, npco); free_list(cfs, ncf);
   free_list(cfout, ncfo); free_list(mpout, nmpo); free_list(wpout, nwpo); free_list(poles, npole);

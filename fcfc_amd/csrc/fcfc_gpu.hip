@@ -906,12 +906,13 @@ k_count(const FkArgs a) {
         if (NEED_S) bs = cs_[j];
         const double wj = NEED_W ? cw[j] : 1.0;
         const int jj = (SELF || F32) ? cj[j] : 0;
-        if (GEOM == FCFC_GEOM_BOX && !PLAIN) {
+        if (GEOM == FCFC_GEOM_BOX) {
           /* the wavefront's A objects are one run of consecutive objects
            * of one cell column: when this B object is out of range of all
            * of them, skip the binning (same expressions as the pair
-           * functions, so the compiler shares them; the plain form, like the reference's leaf loops, bins every
-           * candidate pair) */
+           * functions, so the compiler shares them).  Every form skips, so
+           * the plain form differs from the FP64 one in the per-pair mu
+           * arithmetic only */
           bool any = false;
 #pragma unroll
           for (int r = 0; r < R; r++) {
@@ -932,7 +933,7 @@ k_count(const FkArgs a) {
           }
           if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
         }
-        if (GEOM == FCFC_GEOM_SURVEY && !PLAIN) {
+        if (GEOM == FCFC_GEOM_SURVEY) {
           /* the same skip for survey pairs: |a - b|^2 (iso) or s - t
            * (= s^2 = s_perp^2 + pi^2) beyond the largest counted value */
           bool any = false;
@@ -1127,6 +1128,7 @@ __global__ void k_add_f64(double *dst, const double *src, int n) {
 /* ======================================================================== */
 /*                                host side                                 */
 /* ======================================================================== */
+
 
 struct DevBuf { void *p = nullptr; size_t cap = 0; };
 
@@ -1431,10 +1433,35 @@ struct Alloc {
   char *cur = nullptr;         /* own: buffers are carved from 64 MiB chunks */
   size_t left = 0;             /* (a batch of hundreds of small catalogues   */
   int err = 0;                 /* would otherwise pay a hipMalloc per array) */
+  char *arena = nullptr;       /* no-allocation calls: every buffer, the     */
+  size_t arena_cap = 0;        /* temporaries included, is carved from the   */
+  size_t arena_used = 0;       /* device scratch array; nothing is allocated */
   Alloc(fcfc_gpu_ctx *c_, bool own_) : c(c_), own(own_) {}
   ~Alloc() { for (void *p : blocks) (void) hipFree(p); }
+  void use_arena(void *p, size_t cap) { arena = (char *) p; arena_cap = cap; arena_used = 0; own = false; }
+  void *carve(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t) 255;
+    if (bytes > arena_cap - arena_used) {
+      if (!err) set_err(c, "no-allocation count: its %zu-byte device scratch is exhausted "
+                        "(%zu bytes in use, %zu more requested); count these catalogues "
+                        "with fcfc_gpu_count", arena_cap, arena_used, bytes);
+      err = FCFC_ERR_MEMORY;
+      return nullptr;
+    }
+    void *r = arena + arena_used;
+    arena_used += bytes;
+    return r;
+  }
+  /* temporaries of one call (sort, tiling, histograms): the context's
+   * reusable slots, or the arena */
+  void *tmp(int slot, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (arena) return carve(bytes);
+    return ws(c, slot, bytes, &err);
+  }
   void *get(int slot, size_t bytes) {
     if (bytes == 0) bytes = 8;
+    if (arena) return carve(bytes);
     if (!own) return ws(c, slot, bytes, &err);
     bytes = (bytes + 255) & ~(size_t) 255;
     if (bytes > left) {
@@ -1488,7 +1515,7 @@ __global__ void k_minmax(int n, const double *__restrict__ x, const double *__re
 
 
 
-static int bbox(fcfc_gpu_ctx *c, const fcfc_cat *k, double lo[3], double hi[3]) {
+static int bbox(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, double lo[3], double hi[3]) {
   if (k->n <= 0) return 0;
   if (!k->on_device) {
     for (int64_t i = 0; i < k->n; i++) {
@@ -1497,10 +1524,9 @@ static int bbox(fcfc_gpu_ctx *c, const fcfc_cat *k, double lo[3], double hi[3]) 
     }
     return 0;
   }
-  int err = 0;
   const int nbk = 64;
-  double *d = (double *) ws(c, WS_MISC, sizeof(double) * (8 + 6 * nbk), &err);
-  if (err) return err;
+  double *d = (double *) al.tmp(WS_MISC, sizeof(double) * (8 + 6 * nbk));
+  if (al.err) return al.err;
   hipLaunchKernelGGL(k_minmax, dim3(nbk), dim3(256), 0, c->stream, (int) k->n, k->x, k->y, k->z, d);
   std::vector<double> h(8 + 6 * nbk);
   HIPCHK(c, hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
@@ -1565,7 +1591,7 @@ static int stage_prep(fcfc_gpu_ctx *c, Alloc &al, const fcfc_cat *k, int need_s,
 }
 
 /* Run the queued sorts of a call: one launch per stage for all of them. */
-static int stage_run(fcfc_gpu_ctx *c, std::vector<FkStage> &st) {
+static int stage_run(fcfc_gpu_ctx *c, Alloc &al, std::vector<FkStage> &st) {
   if (st.empty()) return 0;
   long long nobj = 0, ncell = 0, ncs = 0;
   for (FkStage &q : st) {
@@ -1574,14 +1600,13 @@ static int stage_run(fcfc_gpu_ctx *c, std::vector<FkStage> &st) {
     nobj += q.n; ncell += nc; ncs += nc + 1;
     if (nobj > INT_MAX / 2 || ncs > INT_MAX / 2) { set_err(c, "catalogues of one call too large"); return FCFC_ERR_CFG; }
   }
-  int err = 0;
   const int nst = (int) st.size();
-  FkStage *dst = (FkStage *) ws(c, WS_STAGE, sizeof(FkStage) * st.size(), &err);
-  int *cid = (int *) ws(c, WS_CID, sizeof(int) * (size_t) (nobj > 0 ? nobj : 1), &err);
-  int *rank = (int *) ws(c, WS_RANK, sizeof(int) * (size_t) (nobj > 0 ? nobj : 1), &err);
-  int *count = (int *) ws(c, WS_COUNT, sizeof(int) * (size_t) ncell, &err);
-  int *gstart = (int *) ws(c, WS_GSTART, sizeof(int) * ((size_t) ncell + 1), &err);
-  if (err) return err;
+  FkStage *dst = (FkStage *) al.tmp(WS_STAGE, sizeof(FkStage) * st.size());
+  int *cid = (int *) al.tmp(WS_CID, sizeof(int) * (size_t) (nobj > 0 ? nobj : 1));
+  int *rank = (int *) al.tmp(WS_RANK, sizeof(int) * (size_t) (nobj > 0 ? nobj : 1));
+  int *count = (int *) al.tmp(WS_COUNT, sizeof(int) * (size_t) ncell);
+  int *gstart = (int *) al.tmp(WS_GSTART, sizeof(int) * ((size_t) ncell + 1));
+  if (al.err) return al.err;
   HIPCHK(c, hipMemcpyAsync(dst, st.data(), sizeof(FkStage) * st.size(), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_fill_int, dim3(nblk(ncell, 256)), dim3(256), 0, c->stream, count, (int) ncell, 0);
   if (nobj > 0)
@@ -1779,13 +1804,23 @@ static int bins_ok(const fcfc_bins *b) {
 }
 
 /* Plan a job and stage its catalogues.  Sorted copies are shared between
- * jobs of one call when the catalogue (same coordinate array) and the grid
- * coincide. */
-struct CopyKey { const double *x; int64_t n; int wt; FkGrid g; DevCat d; };
+ * jobs of one call when every array a copy is built from (coordinates,
+ * |r|^2, weights of weighted jobs) and the grid coincide: two catalogues on
+ * the same positions with different weights (jackknife or bootstrap
+ * weights) get copies of their own. */
+struct CopyKey {
+  const double *x, *y, *z, *s, *w; int64_t n; int wt; FkGrid g; DevCat d;
+  bool same(const fcfc_cat *k, int wt_, int need_s, const FkGrid &g_) const;
+};
 
 static bool same_grid(const FkGrid &a, const FkGrid &b) {
   return a.ncx == b.ncx && a.ncy == b.ncy && a.ncz == b.ncz && a.ox == b.ox && a.oy == b.oy &&
       a.oz == b.oz && a.wx == b.wx && a.wy == b.wy && a.wz == b.wz;
+}
+
+bool CopyKey::same(const fcfc_cat *k, int wt_, int need_s, const FkGrid &g_) const {
+  return x == k->x && y == k->y && z == k->z && n == k->n && wt == wt_ &&
+      (!need_s || s == k->s) && (!wt_ || w == k->w) && same_grid(g, g_);
 }
 
 /* weights of a sorted catalogue re-read from the caller's device array */
@@ -1804,8 +1839,8 @@ static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   int e;
   if (bins->geometry == FCFC_GEOM_SURVEY) {
-    if ((e = bbox(c, a, lo, hi))) return e;
-    if (!autoc && (e = bbox(c, b, lo, hi))) return e;
+    if ((e = bbox(c, al, a, lo, hi))) return e;
+    if (!autoc && (e = bbox(c, al, b, lo, hi))) return e;
     if (!(lo[0] <= hi[0])) { for (int k = 0; k < 3; k++) { lo[k] = 0; hi[k] = 1; } }
   }
   if ((e = make_plan(c, bins, bsize, lo, hi, autoc, wt, mode, a->n, autoc ? a->n : b->n, &J->P))) return e;
@@ -1817,12 +1852,12 @@ static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
     const fcfc_cat *k = cs[q];
     bool found = false;
     for (auto &cp : copies)
-      if (cp.x == k->x && cp.n == k->n && cp.wt == wt && same_grid(cp.g, J->P.g)) { *dst[q] = cp.d; found = true; break; }
+      if (cp.same(k, wt, need_s, J->P.g)) { *dst[q] = cp.d; found = true; break; }
     if (found) continue;
     const int slot = (q == 0) ? WS_A_X : WS_B_X;
     const int upslot = (q == 0) ? WS_IN_X : WS_IN2_X;
     if ((e = stage_prep(c, al, k, need_s, wt, J->P.g, slot, upslot, dst[q], stages))) return e;
-    if (al.own) copies.push_back(CopyKey{k->x, k->n, wt, J->P.g, *dst[q]});
+    if (al.own) copies.push_back(CopyKey{k->x, k->y, k->z, k->s, k->w, k->n, wt, J->P.g, *dst[q]});
     if (gathers && wt && k->w && k->on_device && k->n > 0)
       gathers->push_back(Gather{k->w, dst[q]->idx, dst[q]->w, (int) k->n});
   }
@@ -1874,12 +1909,18 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     ssearch = 1;
     if (lds_plan(b, m, anywt, need_s, 1, hyb, 1, 0).total > lds_max) psearch = 1;
   }
-  /* one kernel serves every job of the launch: tiles of the longest kind */
+  /* one kernel serves every job of the launch: tiles of the longest kind.
+   * A lane holds R objects, so no tile may exceed R FK_NT objects: the
+   * hybrid gradient kernel exists for R = 1 only, and a tile size set by
+   * the caller or the autotuner is clamped to what the kernel covers */
   {
     int R = 1;
     for (auto &j : jobs) R = std::max(R, (j.P.T + FK_NT - 1) / FK_NT);
     if (m == MODE_GRAD && hyb) R = 1;
-    for (auto &j : jobs) { j.P.R = R; if (j.P.T_auto) j.P.T = R * FK_NT; }
+    for (auto &j : jobs) {
+      j.P.R = R;
+      if (j.P.T_auto || j.P.T > R * FK_NT) j.P.T = R * FK_NT;
+    }
   }
   /* ---- kernel and LDS plan ---- */
   const int R = jobs.empty() ? 1 : jobs[0].P.R;
@@ -2132,7 +2173,8 @@ static int deliver(fcfc_gpu_ctx *c, const std::vector<PJob> &jobs, int ntot,
 /* counts of njob jobs; job k's histogram lands in cnt + k * ntot */
 static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
     const fcfc_cat *const *bs, const double *const *bsz, const uint32_t *fl,
-    const fcfc_bins *bins, uint32_t flags, void *cnt) {
+    const fcfc_bins *bins, uint32_t flags, void *cnt, void *arena = nullptr,
+    size_t arena_cap = 0) {
   auto t0 = std::chrono::steady_clock::now();
   HIPCHK(c, hipSetDevice(c->device));
   memset(&c->stats, 0, sizeof c->stats);
@@ -2140,6 +2182,7 @@ static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
   const int mode = (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 :
                    ((flags & FCFC_GPU_PREC_PLAIN) ? MODE_P64 : MODE_C64);
   Alloc al(c, njob > 1);
+  if (arena) al.use_arena(arena, arena_cap);
   std::vector<CopyKey> copies;
   std::vector<FkStage> stages;
   std::vector<PJob> jobs(njob);
@@ -2152,12 +2195,11 @@ static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
     if (e) return e;
     jobs[k].hist_off = (long long) k * bins->ntot;
   }
-  if (int e = stage_run(c, stages)) return e;
-  int err = 0;
+  if (int e = stage_run(c, al, stages)) return e;
   const size_t hbytes = 8 * (size_t) bins->ntot * njob;
-  void *dh = ws(c, WS_HIST, hbytes, &err);
-  unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
-  if (err) return err;
+  void *dh = al.tmp(WS_HIST, hbytes);
+  unsigned long long *misc = (unsigned long long *) al.tmp(WS_MISC, 64 * 8);
+  if (al.err) return al.err;
   hipLaunchKernelGGL(k_zero_u64, dim3(nblk((int64_t) bins->ntot * njob, 256)), dim3(256), 0, c->stream,
       (unsigned long long *) dh, bins->ntot * njob);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
@@ -2236,7 +2278,6 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
   std::vector<CopyKey> copies;
   std::vector<FkStage> stages;
   std::vector<PJob> jobs(1);
-  int err = 0;
   if (int e = setup_job(c, al, copies, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &jobs[0], stages))
     return e;
   double *dga = (double *) al.get(WS_GA, sizeof(double) * (size_t) (a->n > 0 ? a->n : 1));
@@ -2248,10 +2289,10 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(b->n, 256)), dim3(256), 0, c->stream, dgb, (int64_t) b->n, 0.0);
   jobs[0].gA = sw ? dgb : dga;
   jobs[0].gB = autoc ? dga : (sw ? dga : dgb);
-  if (int e = stage_run(c, stages)) return e;
-  double *dG = (double *) ws(c, WS_G, sizeof(double) * bins->ntot, &err);
-  unsigned long long *misc = (unsigned long long *) ws(c, WS_MISC, 64 * 8, &err);
-  if (err) return err;
+  if (int e = stage_run(c, al, stages)) return e;
+  double *dG = (double *) al.tmp(WS_G, sizeof(double) * bins->ntot);
+  unsigned long long *misc = (unsigned long long *) al.tmp(WS_MISC, 64 * 8);
+  if (al.err) return al.err;
   HIPCHK(c, hipMemcpyAsync(dG, grad_cnt, sizeof(double) * bins->ntot,
       dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, misc, 64);
@@ -2331,7 +2372,7 @@ extern "C" int fcfc_gpu_plan_create(fcfc_gpu_ctx *c, int njob, const fcfc_batch_
     if (e) return e;
     P->jobs[k].hist_off = (long long) k * bins->ntot;
   }
-  if (int e = stage_run(c, stages)) return e;
+  if (int e = stage_run(c, P->al, stages)) return e;
   P->dh = P->al.get(WS_HIST, 8 * (size_t) bins->ntot * njob);
   P->misc = (unsigned long long *) P->al.get(WS_MISC, 64 * 8);
   if (P->al.err) return P->al.err;
@@ -2403,7 +2444,7 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
   const int mode = grad ? MODE_GRAD : (flags & FCFC_GPU_PREC_FP32) ? MODE_C32 :
                    ((flags & FCFC_GPU_PREC_PLAIN) ? MODE_P64 : MODE_C64);
   const int save_cd = c->t_cell_div, save_zd = c->t_zdiv, save_t = c->t_tile;
-  std::vector<unsigned char> scratch(8 * (size_t) bins->ntot);
+  std::vector<unsigned char> hcnt(8 * (size_t) bins->ntot);
   /* gradient timing: dL/dcnt = 1 in every bin (every pair contributes),
    * gradient of a only, as weight fitting asks for it */
   std::vector<double> gone(grad ? (size_t) bins->ntot : 0, 1.0);
@@ -2413,8 +2454,8 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     if (grad)
       return fcfc_gpu_count_backward(c, a, b, bins, bsize, flags & FCFC_GPU_AUTO, gone.data(),
                                      gout.data(), nullptr);
-    memset(scratch.data(), 0, scratch.size());
-    return fcfc_gpu_count(c, a, b, bins, bsize, cflags, scratch.data());
+    memset(hcnt.data(), 0, hcnt.size());
+    return fcfc_gpu_count(c, a, b, bins, bsize, cflags, hcnt.data());
   };
   double bestms = 1e300;
   int bcd = 3, bzd = 8, bT = 0;
@@ -2448,9 +2489,12 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     if (e || (e = measure(bcd, zd, 0, &est))) break;
     if (est < bestms) { bestms = est; bzd = zd; bT = last_T; }
   }
+  /* the hybrid gradient kernel holds one object per lane: tiles of 2 FK_NT
+   * objects would run as FK_NT, so only that size is timed */
   const int Ts[] = {FK_NT, 2 * FK_NT};
   for (int T : Ts) {
     double est;
+    if (T > FK_NT && grad && (bins->tabtype || !zb_fits(bins))) break;
     if (e || (e = measure(bcd, bzd, T, &est))) break;
     if (est < bestms) { bestms = est; bT = T; }
   }
@@ -2466,8 +2510,9 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     if (!autoc && kb->n > ka->n) std::swap(ka, kb);
     if (bins->geometry == FCFC_GEOM_SURVEY) {
       for (int k = 0; k < 3; k++) { lo[k] = 1e300; hi[k] = -1e300; }
-      bbox(c, ka, lo, hi);
-      if (!autoc) bbox(c, kb, lo, hi);
+      Alloc tal(c, false);
+      bbox(c, tal, ka, lo, hi);
+      if (!autoc) bbox(c, tal, kb, lo, hi);
     }
     const int wt = (grad || (flags & FCFC_GPU_WEIGHTED)) ? 1 : 0;
     make_plan(c, bins, bsize, lo, hi, autoc, wt, mode, ka->n, kb->n, &P);

@@ -297,8 +297,14 @@ static int run_pairs(int geom, const double *bsize, const orc_cat *A,
     }
     for (int k = 0; k < 3; k++) { hi[k] += 1e-9 * (1 + fabs(hi[k])); if (hi[k] <= lo[k]) hi[k] = lo[k] + 1; }
   }
+  /* cells of half the neighbour radius (5^3 cells searched instead of 3^3
+   * of the full radius: 2.5 times fewer candidates) */
   cgrid g;
-  if (grid_build(&g, B, lo, hi, rmax)) return -1;
+  if (grid_build(&g, B, lo, hi, 0.5 * rmax)) return -1;
+  /* candidates beyond the padded radius are out of range on every rounding
+   * path (the paths differ by a few ulps of s^2; the survey formula by
+   * ~1e-10 absolute): skipped before any variant is evaluated */
+  const double r2pad = r2max * (1 + 1e-6) + 1e-6;
   double L[3] = {0, 0, 0}, h[3] = {0, 0, 0};
   if (geom == 0) for (int k = 0; k < 3; k++) { L[k] = bsize[k]; h[k] = bsize[k] * 0.5; }
   const int withwt = (A->w != NULL) || (B->w != NULL);
@@ -346,6 +352,15 @@ static int run_pairs(int geom, const double *bsize, const orc_cat *A,
               int64_t j = g.idx[t];
               if (isauto && j <= i) continue;
               double b[4] = {B->x[j], B->y[j], B->z[j], B->s ? B->s[j] : 0};
+              {
+                double q = 0;
+                for (int k = 0; k < 3; k++) {
+                  double d = a[k] - b[k];
+                  if (geom == 0) { if (d > h[k]) d -= L[k]; else if (d < -h[k]) d += L[k]; }
+                  q += d * d;
+                }
+                if (q > r2pad) continue;
+              }
               int r = (geom == 0) ? box_pair(a, b, L, h, bn, var)
                                   : survey_pair(a, b, bn, var);
               if (census_mask) {

@@ -176,17 +176,21 @@ def write_ascii(fname, cols):
 
 def run_reference(name, flavour, catalogs, labels, pairs, bintype, smin, smax, ds,
                   nmu=None, pmin=None, pmax=None, dpi=None, bsize=None, weights=None,
-                  threads=None, workdir=None, timeout=3600):
+                  threads=None, workdir=None, timeout=3600, extra_env=None,
+                  reuse_files=False):
     """Run the upstream executable on raw (unrescaled) catalogues and return
     ({pair: parsed pair-count file}, {pair: count-phase seconds}, wall).
 
-    catalogs: list of (x, y, z[, w]) arrays; weights: list of bools."""
+    catalogs: list of (x, y, z[, w]) arrays; weights: list of bools.
+    reuse_files: keep catalogue files already written into workdir (repeated
+    timing runs of the same catalogues)."""
     exe = ref_binary(name, flavour)
     tmp = workdir or tempfile.mkdtemp(prefix="fcfc_ref_")
     files = []
     for k, cat in enumerate(catalogs):
         fn = os.path.join(tmp, f"cat{k}.txt")
-        write_ascii(fn, cat)
+        if not (reuse_files and os.path.exists(fn)):
+            write_ascii(fn, cat)
         files.append(fn)
     outs = [os.path.join(tmp, f"pc_{p}.bin") for p in pairs]
     for o in outs:
@@ -210,6 +214,8 @@ def run_reference(name, flavour, catalogs, labels, pairs, bintype, smin, smax, d
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
+    if extra_env:
+        env.update(extra_env)
     # the reference reads fcfc_2pt_box.conf from the cwd when present: use the
     # empty temporary directory
     t0 = time.time()

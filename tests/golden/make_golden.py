@@ -10,7 +10,9 @@ regenerate them from the seeds and check the checksum first.
 Each PAIR_COUNT runs in its own invocation: requesting DD, DR and RR in one
 run with a weighted D and an unweighted R makes the upstream binary crash.
 
-Usage:  python tests/golden/make_golden.py      (needs oracle/_ref built)
+Usage:  python tests/golden/make_golden.py          (needs oracle/_ref built)
+        python tests/golden/make_golden.py --sizes  (the cfg_* cases at the
+                                                     configurations' sizes)
 """
 import hashlib
 import json
@@ -47,6 +49,15 @@ def catalogs(kind):
         return [F.gen_uniform(31, 20000, L)], [False]
     if kind == "box_tiny300":
         return [F.gen_uniform(41, 300, L)], [False]
+    if kind == "c1_full":
+        # the C1 configuration as specified: N = 1e5 uniform in the L = 1000 box
+        return [F.gen_uniform(1001, 100000, L)], [False]
+    if kind == "c2_twentieth":
+        # the C2 recipe at 1/20 of its size: D = 5e4 uniform + 100 halos of
+        # 500 members (sigma = 2), lognormal(0, 0.3) weights; R = 1e6 uniform
+        x, y, z, w = F.gen_clustered(2001, 50000, 100, 500, 2.0, L, 0.3)
+        xr, yr, zr = F.gen_uniform(2002, 1000000, L)
+        return [(x, y, z, w), (xr, yr, zr)], [True, False]
     if kind == "survey":
         x, y, z, w = F.gen_survey(21, 4000)
         xr, yr, zr, wr = F.gen_survey(22, 6000)
@@ -81,6 +92,37 @@ CASES = [
 ]
 
 
+# Cases at the configurations' own sizes (kept apart from CASES: the oracle
+# census over their 1e8 - 1e10 pairs is computed once, here, and stored)
+SIZE_CASES = [
+    ("cfg_c1_full", "FCFC_2PT_BOX", "c1_full", ["DD"],
+     dict(bintype=0, smin=0, smax=200, ds=5), ["avx512", "scalar"]),
+    ("cfg_c2_twentieth", "FCFC_2PT_BOX", "c2_twentieth", ["DD", "DR", "RR"],
+     dict(bintype=1, smin=0, smax=200, ds=5, nmu=100), ["avx512"]),
+]
+
+
+def oracle_census(kind, pairs, bk):
+    """Unweighted oracle counts (variant 1, the GPU's rounding path) and the
+    census of each pair: in-range pairs whose bin differs between the
+    reference's paths (variants 0-4)."""
+    cats, wts = catalogs(kind)
+    b = F.Bins.linear(F.GEOM_BOX, bk["bintype"], bk["smin"], bk["smax"], bk["ds"],
+                      nmu=bk.get("nmu", 1), bsize=L)
+    ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu)
+    prepped = [F.Catalog(b, c[0], c[1], c[2]) for c in cats]
+    out = {}
+    for p in pairs:
+        A, B = prepped["DR".index(p[0])], prepped["DR".index(p[1])]
+        isauto = p[0] == p[1]
+        _, cen = O.count(0, b.bsize_rescaled, (A.x, A.y, A.z, None, None),
+                         None if isauto else (B.x, B.y, B.z, None, None), isauto, ob,
+                         variant=1, census_mask=0b11111)
+        out[p] = int(cen)
+        print(f"  census {p}: {cen}", flush=True)
+    return out
+
+
 def run_case(name, code, kind, pairs, bk, flavours):
     cats, wts = catalogs(kind)
     out = {"name": name, "code": code, "catalogs": kind, "binning": bk,
@@ -109,6 +151,14 @@ def run_case(name, code, kind, pairs, bk, flavours):
 
 
 def main():
+    if "--sizes" in sys.argv:
+        for f in SIZE_CASES:
+            print("case", f[0], flush=True)
+            data = run_case(*f)
+            data["census"] = oracle_census(f[2], f[3], f[4])
+            with open(os.path.join(HERE, f[0] + ".json"), "w") as fp:
+                json.dump(data, fp, separators=(",", ":"))
+        return
     for f in [c for c in CASES]:
         name = f[0]
         print("case", name, flush=True)

@@ -185,3 +185,41 @@ def test_weight_fit_plan_matches_eager():
                 wd = wd * (1.0 + 0.05 * torch.sin(torch.arange(len(w), device="cuda") * 0.1))
         finally:
             fit.close()
+
+
+def test_backward_hybrid_tiles_beyond_one_per_lane(eng):
+    """Hybrid binning holds one A object per lane in the gradient kernel.
+    Dense halos put more than 1024 objects in a cell, so tiles are as large
+    as the tile size allows: with 1024-object tiles set by the caller, and
+    with the autotuner's shape, every object's gradient must still match
+    the FP64 oracle (auto and cross)."""
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0.5, 146.5, 7.3, nmu=7, bsize=L)
+    assert b.tabtype == 1
+    x, y, z, w = F.gen_clustered(71, 1000, 4, 3000, 1.0, L, 0.3)
+    xr, yr, zr = F.gen_uniform(72, 3000, L)
+    wr = np.linspace(0.7, 1.3, len(xr))
+    D = F.Catalog(b, x, y, z, w)
+    R = F.Catalog(b, xr, yr, zr, wr)
+    G = np.random.default_rng(7).normal(size=b.ntot)
+    ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu)
+    refs = {True: O.weight_grad(0, b.bsize_rescaled, (D.x, D.y, D.z, None, D.w), None, True, ob, G),
+            False: O.weight_grad(0, b.bsize_rescaled, (D.x, D.y, D.z, None, D.w),
+                                 (R.x, R.y, R.z, None, R.w), False, ob, G)}
+
+    def check(tag):
+        for isauto in (True, False):
+            print(f"case backward hybrid dense {tag} isauto={isauto}", flush=True)
+            ga, gb = eng.count_backward(D, None if isauto else R, b, G, isauto=isauto)
+            ra, rb = refs[isauto]
+            np.testing.assert_allclose(ga, ra, rtol=1e-6, atol=1e-9 * np.abs(ra).max())
+            if not isauto:
+                np.testing.assert_allclose(gb, rb, rtol=1e-6, atol=1e-9 * np.abs(rb).max())
+
+    try:
+        eng.set_tuning(tile=1024)
+        check("tile=1024")
+    finally:
+        eng.set_tuning()
+    for isauto in (True, False):
+        eng.autotune(D, None if isauto else R, b, isauto=isauto, withwt=True, gradient=True)
+    check("autotuned")

@@ -178,3 +178,44 @@ def test_batch_survey_norms_on_device(eng):
     devc = eng.count_batch([(Dd, None, True, False), (Dd, Rd, False, True)], b)
     np.testing.assert_array_equal(devc[0], host[0])
     np.testing.assert_allclose(devc[1], host[1], rtol=1e-12, atol=0)
+
+
+def test_batch_shared_positions_different_weights(eng):
+    """Jobs on the same device position arrays with different weight arrays
+    (jackknife or bootstrap weights) get sorted copies of their own: every
+    job must carry its own weights, in a batch and in a prepared plan."""
+    import torch
+    from oracle import oracle as O
+    b = F.Bins.linear(F.GEOM_BOX, F.BIN_SMU, 0, 200, 5, nmu=20, bsize=L)
+    f = b.rescale
+    ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu)
+    x, y, z, w = F.gen_clustered(61, 2000, 15, 50, 2.0, L, 0.3)
+    xr, yr, zr = F.gen_uniform(62, 4000, L)
+    w2 = np.ascontiguousarray(w[::-1])
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    X, Y, Z = dev(x * f), dev(y * f), dev(z * f)
+    D1 = F.Catalog(b, X, Y, Z, dev(w), rescaled=True)
+    D2 = F.Catalog(b, X, Y, Z, dev(w2), rescaled=True)
+    R = F.Catalog(b, xr, yr, zr)
+    jobs = [(D1, None, True, True), (D2, None, True, True), (D1, R, False, True),
+            (D2, R, False, True)]
+    Dh = F.Catalog(b, x, y, z)
+    refs = []
+    for ww in (w, w2):
+        ta = (Dh.x, Dh.y, Dh.z, None, ww)
+        refs.append(O.count(0, b.bsize_rescaled, ta, None, True, ob, variant=1, weighted=True)[0])
+    for ww in (w, w2):
+        ta = (Dh.x, Dh.y, Dh.z, None, ww)
+        refs.append(O.count(0, b.bsize_rescaled, ta, (R.x, R.y, R.z, None, np.ones(R.n)), False,
+                            ob, variant=1, weighted=True)[0])
+    print("case batch shared positions, different weights", flush=True)
+    got = eng.count_batch(jobs, b)
+    for k in range(4):
+        np.testing.assert_allclose(got[k], refs[k], rtol=1e-12, atol=0, err_msg=f"job {k}")
+    plan = eng.plan(jobs, b)
+    try:
+        got = plan.run()
+        for k in range(4):
+            np.testing.assert_allclose(got[k], refs[k], rtol=1e-12, atol=0, err_msg=f"plan job {k}")
+    finally:
+        plan.close()

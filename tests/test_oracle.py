@@ -20,7 +20,10 @@ from oracle import oracle as O
 from tests.golden.make_golden import catalogs, checksum
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = sorted(glob.glob(os.path.join(GOLD, "*.json")))
+# cfg_*: the cases at the configurations' own sizes (tests/test_gpu_sizes.py
+# and the size tests below)
+CASES = sorted(c for c in glob.glob(os.path.join(GOLD, "*.json"))
+               if not os.path.basename(c).startswith("cfg_"))
 
 
 def _bins(code, bk, L=1000.0):
@@ -33,6 +36,22 @@ def _bins(code, bk, L=1000.0):
 
 @pytest.mark.parametrize("path", CASES, ids=[os.path.basename(c)[:-5] for c in CASES])
 def test_oracle_matches_reference(path):
+    _check_case(path)
+
+
+@pytest.mark.parametrize("name", ["cfg_c1_full", "cfg_c2_twentieth"])
+def test_oracle_at_config_sizes(name):
+    """The oracle against the reference at the configurations' sizes: C1 as
+    specified (AVX-512 and serial scalar builds) and the DD count of C2 at
+    1/20 size (its DR and RR are left to the GPU tests, which compare the
+    GPU with the reference's files directly; the oracle would need minutes
+    for them here)."""
+    g = json.load(open(os.path.join(GOLD, name + ".json")))
+    assert g["census"]["DD"] == 0
+    _check_case(os.path.join(GOLD, name + ".json"), only=("DD",))
+
+
+def _check_case(path, only=None):
     g = json.load(open(path))
     cats, wts = catalogs(g["catalogs"])
     assert [checksum(*c) for c in cats] == g["checksums"], "generator output changed"
@@ -45,6 +64,8 @@ def test_oracle_matches_reference(path):
         prepped.append(k)
     for key, ref in g["results"].items():
         flav, pair = key.split(":")
+        if only and pair not in only:
+            continue
         A = prepped["DR".index(pair[0])]
         B = prepped["DR".index(pair[1])]
         isauto = pair[0] == pair[1]
