@@ -85,6 +85,8 @@ SIGNATURES = {
     "fcfc_gpu_last_error": (C.c_char_p, [C.c_void_p]),
     "fcfc_gpu_count": (C.c_int, [C.c_void_p, C.POINTER(fcfc_cat), C.POINTER(fcfc_cat),
                                  C.POINTER(fcfc_bins), _dp, C.c_uint32, C.c_void_p]),
+    "fcfc_gpu_count_noalloc": (C.c_int, [C.c_void_p, C.POINTER(fcfc_cat), C.POINTER(fcfc_cat),
+                                         C.POINTER(fcfc_bins), _dp, C.c_uint32, C.c_void_p]),
     "fcfc_gpu_last_stats": (C.c_int, [C.c_void_p, C.POINTER(fcfc_gpu_stats)]),
     "fcfc_gpu_count_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(fcfc_batch_job),
                                        C.POINTER(fcfc_bins), C.c_uint32, C.c_void_p]),

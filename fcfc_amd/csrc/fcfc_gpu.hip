@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1129,6 +1130,18 @@ __global__ void k_add_f64(double *dst, const double *src, int n) {
 /*                                host side                                 */
 /* ======================================================================== */
 
+/* Device scratch of the no-allocation count (fcfc_gpu_count_noalloc): every
+ * device buffer of such a call -- uploads of host catalogues, sorted copies,
+ * cell and tiling arrays, histograms, queue and counters -- is carved from
+ * this array.  Each GPU the library runs on holds its own copy until the
+ * process ends.  In host code the name is only a stand-in: fcfc_gpu_init
+ * resolves the address of the current GPU's copy with hipGetSymbolAddress
+ * (once per device) and every host use goes through that address. */
+#define FK_SCRATCH_BYTES ((size_t) 128 << 20)
+#define FK_MAXDEV 64
+static __device__ __align__(256) unsigned char scratch[FK_SCRATCH_BYTES];
+static void *g_scratch_ptr[FK_MAXDEV];
+static std::mutex g_scratch_mu;
 
 struct DevBuf { void *p = nullptr; size_t cap = 0; };
 
@@ -1148,6 +1161,8 @@ struct fcfc_gpu_ctx {
   struct Tuned { int geom, bintype, mode, autoc, wt, bucket, cd, zd, T; };
   std::vector<Tuned> tuned;    /* autotuner results per density bucket */
   std::vector<DevBuf> bufs;
+  void *scratch = nullptr;     /* device address of this GPU's scratch array */
+  char scratch_err[128] = {0};
 };
 
 static void set_err(fcfc_gpu_ctx *c, const char *fmt, ...) {
@@ -1206,6 +1221,21 @@ extern "C" int fcfc_gpu_init(fcfc_gpu_ctx **out, int device) {
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c; return FCFC_ERR_GPU;
   }
+  /* address of this GPU's scratch array (no-allocation counts), resolved
+   * once per device with the device current; a failure leaves the context
+   * usable and makes fcfc_gpu_count_noalloc report it */
+  if (device < FK_MAXDEV) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void *p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e == hipSuccess && p) g_scratch_ptr[device] = p;
+      else snprintf(c->scratch_err, sizeof c->scratch_err, "hipGetSymbolAddress: %s",
+                    hipGetErrorString(e));
+    }
+    c->scratch = g_scratch_ptr[device];
+  }
+  else snprintf(c->scratch_err, sizeof c->scratch_err, "device index beyond %d", FK_MAXDEV - 1);
   *out = c;
   return 0;
 }
@@ -2235,6 +2265,41 @@ extern "C" int fcfc_gpu_count(fcfc_gpu_ctx *c, const fcfc_cat *a,
   const double *bz[1] = {bsize};
   uint32_t fl[1] = {flags & (FCFC_GPU_AUTO | FCFC_GPU_WEIGHTED)};
   return count_jobs(c, 1, as, bs, bz, fl, bins, flags, cnt);
+}
+
+/* Device bytes a no-allocation count needs at least (uploads, sorted copies,
+ * per-object sort scratch); cells, tiles and histograms come on top. */
+static size_t noalloc_floor(const fcfc_cat *k, const fcfc_bins *bins, int wt) {
+  if (!k || k->n <= 0) return 0;
+  const int need_s = (bins->geometry == FCFC_GEOM_SURVEY && bins->bintype != FCFC_BIN_ISO);
+  size_t per = 24 + 4 + 8 + (wt ? 8 : 0) + (need_s ? 8 : 0);     /* sorted copy + sort */
+  if (!k->on_device) per += 24 + (wt && k->w ? 8 : 0) + (need_s && k->s ? 8 : 0);
+  return per * (size_t) k->n;
+}
+
+extern "C" int fcfc_gpu_count_noalloc(fcfc_gpu_ctx *c, const fcfc_cat *a,
+    const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
+    uint32_t flags, void *cnt) {
+  if (!c) return FCFC_ERR_ARG;
+  if (!bins_ok(bins) || !cnt) { set_err(c, "invalid argument"); return FCFC_ERR_ARG; }
+  if (!c->scratch) {
+    set_err(c, "no-allocation count: the device scratch of GPU %d is unavailable (%s)",
+            c->device, c->scratch_err);
+    return FCFC_ERR_GPU;
+  }
+  const int autoc = (flags & FCFC_GPU_AUTO) ? 1 : 0;
+  const int wt = (flags & FCFC_GPU_WEIGHTED) ? 1 : 0;
+  if (!cat_ok(a) || (!autoc && !cat_ok(b))) { set_err(c, "invalid catalogue"); return FCFC_ERR_ARG; }
+  const size_t need = noalloc_floor(a, bins, wt) + (autoc ? 0 : noalloc_floor(b, bins, wt));
+  if (need > FK_SCRATCH_BYTES) {
+    set_err(c, "no-allocation count: at most %zu bytes of device scratch, these catalogues "
+            "need more than %zu; count them with fcfc_gpu_count", FK_SCRATCH_BYTES, need);
+    return FCFC_ERR_MEMORY;
+  }
+  const fcfc_cat *as[1] = {a}, *bs[1] = {b};
+  const double *bz[1] = {bsize};
+  uint32_t fl[1] = {flags & (FCFC_GPU_AUTO | FCFC_GPU_WEIGHTED)};
+  return count_jobs(c, 1, as, bs, bz, fl, bins, flags, cnt, c->scratch, FK_SCRATCH_BYTES);
 }
 
 extern "C" int fcfc_gpu_count_batch(fcfc_gpu_ctx *c, int njob,

@@ -158,6 +158,25 @@ const char *fcfc_gpu_last_error(const fcfc_gpu_ctx *ctx);
 int fcfc_gpu_count(fcfc_gpu_ctx *ctx, const fcfc_cat *a, const fcfc_cat *b,
     const fcfc_bins *bins, const double *bsize, uint32_t flags, void *cnt);
 
+/*
+ * No-allocation count.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * fcfc_gpu_count, but no device memory is allocated or freed: every device
+ * buffer of the call comes from a fixed scratch array of 128 MiB per GPU
+ * (held by the library from its first use on that GPU until the process
+ * ends).  Catalogues whose sorted copies and sort scratch cannot fit (about
+ * 2 - 3e6 objects in total; fewer with host catalogues, weights or survey
+ * |r|^2) are rejected with FCFC_ERR_MEMORY and a message, before any work is
+ * queued or, for a grid or tile list that overflows the rest, before the
+ * count launch.  All calls on one GPU share its scratch array: calls that
+ * overlap on one GPU (several threads or contexts) race and must not be
+ * made.  Host memory (the call's bookkeeping) is still used.
+ */
+int fcfc_gpu_count_noalloc(fcfc_gpu_ctx *ctx, const fcfc_cat *a,
+    const fcfc_cat *b, const fcfc_bins *bins, const double *bsize,
+    uint32_t flags, void *cnt);
+
 /* Per-call statistics of the last fcfc_gpu_count / _batch / _backward. */
 typedef struct {
   double evals;          /* candidate pair distances evaluated              */
