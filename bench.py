@@ -140,8 +140,8 @@ def parity_vs_reference(eng, base, forms=("fp64", "fp32", "plain")):
     import fcfc_amd as F
     L = 1000.0
     out = {}
-    for key, bt in (("rr_sample", F.BIN_SMU), ("c1", F.BIN_ISO)):
-        (x, y, z), ref = base[key]
+    for key, src, bt in (("rr_sample", "rr", F.BIN_SMU), ("c1", "c1", F.BIN_ISO)):
+        (x, y, z), ref = base[src]
         b = F.Bins.linear(F.GEOM_BOX, bt, 0, 200, 5, nmu=100 if bt == F.BIN_SMU else 1, bsize=L)
         c = F.Catalog(b, x, y, z)
         rec = {"n": int(c.n), "reference_pairs": int(ref.sum()) // 2}

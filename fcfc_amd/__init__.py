@@ -28,7 +28,7 @@ from ._lib import (FCFC_BIN_ISO as BIN_ISO, FCFC_BIN_SMU as BIN_SMU,
 __all__ = ["Bins", "Catalog", "Engine", "FcfcError", "BIN_ISO", "BIN_SMU",
            "BIN_SPI", "GEOM_BOX", "GEOM_SURVEY", "pair_counts", "gen_uniform",
            "gen_clustered", "gen_survey", "gen_sizes", "save_pair_count",
-           "lib"]
+           "convert_coords", "convert_coords_table", "lib"]
 
 
 def lib():
@@ -414,6 +414,38 @@ def save_pair_count(fname, bins, cnt, a, b, isauto, withwt, ofmt=0):
                                    a.sumw, (b if b is not None else a).sumw, c.ctypes.data)
     if e:
         raise FcfcError(e, "fcfc_save_pair_count")
+
+
+# -- survey coordinate conversion (src/fcfc/2pt/cnvt_coord.c) -----------------
+def convert_coords(ra, dec, z, omega_m, omega_l=None, w=None, err=None):
+    """(RA, Dec) in degrees and redshift -> comoving Cartesian coordinates
+    (Mpc/h), integrating c/H(z) by Gauss-Legendre quadrature of the order
+    FCFC's convergence test picks; returns (x, y, z, order)."""
+    nan = float("nan")
+    cm = L.fcfc_cosmology(omega_m, nan if omega_l is None else omega_l, 0.0,
+                          nan if w is None else w, nan if err is None else err)
+    e = lib().fcfc_cosmology_check(C.byref(cm))
+    if e:
+        raise FcfcError(e, "fcfc_cosmology_check: invalid cosmology")
+    x, y, zz = (np.array(a, dtype=np.float64, copy=True) for a in (ra, dec, z))
+    order = C.c_int(0)
+    e = lib().fcfc_convert_coords(C.byref(cm), len(x), _dptr(x), _dptr(y), _dptr(zz),
+                                  C.byref(order))
+    if e:
+        raise FcfcError(e, "fcfc_convert_coords")
+    return x, y, zz, order.value
+
+
+def convert_coords_table(zs, ds, ra, dec, z):
+    """The same with distances from a natural cubic spline through the
+    (redshift, distance) samples (Z_CMVDST_FILE); returns (x, y, z)."""
+    zs, ds = _f64(zs), _f64(ds)
+    x, y, zz = (np.array(a, dtype=np.float64, copy=True) for a in (ra, dec, z))
+    e = lib().fcfc_convert_coords_table(_dptr(zs), _dptr(ds), len(zs), len(x), _dptr(x),
+                                        _dptr(y), _dptr(zz))
+    if e:
+        raise FcfcError(e, "fcfc_convert_coords_table")
+    return x, y, zz
 
 
 # -- deterministic generator ---------------------------------------------------

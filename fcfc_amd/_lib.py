@@ -70,6 +70,11 @@ class fcfc_batch_job(C.Structure):
     ]
 
 
+class fcfc_cosmology(C.Structure):
+    _fields_ = [("omega_m", C.c_double), ("omega_l", C.c_double), ("omega_k", C.c_double),
+                ("w", C.c_double), ("err", C.c_double)]
+
+
 # every symbol the header declares, with its ctypes signature
 SIGNATURES = {
     "fcfc_bins_linear": (C.c_int, [C.POINTER(fcfc_bins), C.c_int, C.c_int, C.c_double,
@@ -104,6 +109,12 @@ SIGNATURES = {
     "fcfc_save_pair_count": (C.c_int, [C.c_char_p, C.POINTER(fcfc_bins), C.c_int, _dp,
                                        C.c_int, C.c_int, C.c_uint64, C.c_uint64,
                                        C.c_double, C.c_double, C.c_void_p]),
+    "fcfc_cosmology_check": (C.c_int, [C.POINTER(fcfc_cosmology)]),
+    "fcfc_cmvdst_order": (C.c_int, [C.POINTER(fcfc_cosmology), C.c_int64, _dp,
+                                    C.POINTER(C.c_int)]),
+    "fcfc_convert_coords": (C.c_int, [C.POINTER(fcfc_cosmology), C.c_int64, _dp, _dp, _dp,
+                                      C.POINTER(C.c_int)]),
+    "fcfc_convert_coords_table": (C.c_int, [_dp, _dp, C.c_int64, C.c_int64, _dp, _dp, _dp]),
     "fcfc_gen_uniform": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, _dp, _dp, _dp]),
     "fcfc_gen_clustered": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
                                      C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,

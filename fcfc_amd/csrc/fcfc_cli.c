@@ -10,9 +10,11 @@
  * write the pair-count file (same binary/ASCII layout), then evaluate the
  * CF_ESTIMATOR expressions, multipoles and projected CFs.
  *
- * Supported subset: ASCII catalogues (CATALOG_TYPE 0) whose POSITION /
- * WEIGHT entries are plain column references ${n}; SELECTION and
- * COORD_CONVERT are not supported (an error is reported).  Extra key with
+ * Supported subset: ASCII catalogues (CATALOG_TYPE 0); POSITION, WEIGHT
+ * and SELECTION as the numeric and logical subset of FCFC's expressions
+ * (columns read as double; bitwise operators and strings are refused);
+ * COORD_CONVERT with the fiducial cosmology or a (z, distance) table
+ * (FCFC_2PT).  FITS and HDF5 catalogues are refused.  Extra key with
  * no counterpart in the reference: GPU_PRECISION = 0 (FP64 kernel, default)
  * or 1 (FP32 distances with FP64 re-evaluation near bin edges); CLI --gpu-prec.
  * Compiled twice with -DFCFC_CLI_GEOM=0 (box) or 1 (survey).
@@ -88,7 +90,9 @@ static const opt_t OPTS[] = {
   {0, "s-step", "SEP_BIN_SIZE"}, {0, "mu-num", "MU_BIN_NUM"}, {0, "pi-file", "PI_BIN_FILE"},
   {0, "pi-min", "PI_BIN_MIN"}, {0, "pi-max", "PI_BIN_MAX"}, {0, "pi-step", "PI_BIN_SIZE"},
   {'F', "out-format", "OUTPUT_FORMAT"}, {'O', "overwrite", "OVERWRITE"}, {'v', "verbose", "VERBOSE"},
-  {0, "convert", "COORD_CONVERT"}, {0, "gpu-prec", "GPU_PRECISION"},
+  {0, "convert", "COORD_CONVERT"}, {'d', "omega-m", "OMEGA_M"}, {0, "omega-l", "OMEGA_LAMBDA"},
+  {0, "eos-w", "DE_EOS_W"}, {0, "cmvdst-err", "CMVDST_ERR"}, {0, "cmvdst-file", "Z_CMVDST_FILE"},
+  {0, "gpu-prec", "GPU_PRECISION"},
   {0, NULL, NULL}
 };
 
@@ -168,24 +172,166 @@ static int get_int(const char *key, int def) {
 /* ASCII catalogues                                                        */
 typedef struct { int64_t n; double *x, *y, *z, *w; double sumw; int loaded; } cat_t;
 
-/* column number from "${n}" or "$n" */
-static int colref(const char *s) {
-  const char *p = s;
-  while (isspace((unsigned char) *p)) p++;
-  if (*p != '$') return -1;
-  p++;
-  int brace = (*p == '{');
-  if (brace) p++;
+/* ---------------------------------------------------------------------- */
+/* Catalogue expressions for POSITION, WEIGHT and SELECTION: the numeric   */
+/* and logical subset of libast (src/lib/libast.c, ast_build/ast_eval):   */
+/* numbers, columns $n / ${n}, ( ), abs sqrt ln log isfinite, unary - !,   */
+/* then binary ** ; * / % ; + - ; < <= > >= ; == != ; && ; || from the     */
+/* tightest, all binary operators left-associative and unary ones binding */
+/* tighter than any binary one (libast's precedence table).  Evaluated in */
+/* double (% is fmod, ** is pow, a comparison gives 1 or 0); a row is     */
+/* selected when its SELECTION is non-zero.  Columns are read as double;  */
+/* bitwise and shift operators and string columns are refused.            */
+enum { X_NUM, X_COL, X_NEG, X_NOT, X_ABS, X_SQRT, X_LN, X_LOG, X_ISFIN, X_POW,
+       X_MUL, X_DIV, X_REM, X_ADD, X_SUB, X_LT, X_LE, X_GT, X_GE, X_EQ, X_NE,
+       X_AND, X_OR };
+typedef struct { int op; int col; double v; } xop_t;
+typedef struct { xop_t *code; int n, cap, depth, maxdepth, maxcol; const char *p, *err; } xprog_t;
+
+static void x_emit(xprog_t *x, int op, int col, double v) {
+  if (x->err) return;
+  if (x->n == x->cap) {
+    x->cap = x->cap ? 2 * x->cap : 16;
+    x->code = realloc(x->code, sizeof(xop_t) * x->cap);
+  }
+  x->code[x->n++] = (xop_t) {op, col, v};
+  if (op == X_NUM || op == X_COL) { if (++x->depth > x->maxdepth) x->maxdepth = x->depth; }
+  else if (op >= X_POW) x->depth--;
+}
+static void x_ws(xprog_t *x) { while (isspace((unsigned char) *x->p)) x->p++; }
+static void x_expr(xprog_t *x, int minprec);
+
+static void x_unary(xprog_t *x) {
+  x_ws(x);
+  const char *p = x->p;
+  if (*p == '-') { x->p++; x_unary(x); x_emit(x, X_NEG, 0, 0); return; }
+  if (*p == '!' && p[1] != '=') { x->p++; x_unary(x); x_emit(x, X_NOT, 0, 0); return; }
+  if (*p == '~') { x->err = "bitwise operators are not supported"; return; }
+  if (*p == '(') {
+    x->p++; x_expr(x, 0); x_ws(x);
+    if (*x->p != ')') { if (!x->err) x->err = "missing `)'"; return; }
+    x->p++; return;
+  }
+  if (*p == '$') {
+    p++;
+    int brace = (*p == '{');
+    if (brace) p++;
+    char *e;
+    long c = strtol(p, &e, 10);
+    if (e == p || c < 1 || c > 65535) { x->err = "invalid column reference"; return; }
+    if (brace) { if (*e != '}') { x->err = "invalid column reference"; return; } e++; }
+    x->p = e;
+    if (c > x->maxcol) x->maxcol = (int) c;
+    x_emit(x, X_COL, (int) c, 0);
+    return;
+  }
+  if (*p == '"' || *p == '\'') { x->err = "string values are not supported"; return; }
+  static const struct { const char *name; int op; } fn[] = {
+    {"abs", X_ABS}, {"sqrt", X_SQRT}, {"ln", X_LN}, {"log", X_LOG}, {"isfinite", X_ISFIN}};
+  for (size_t k = 0; k < sizeof fn / sizeof fn[0]; k++) {
+    size_t l = strlen(fn[k].name);
+    if (!strncmp(p, fn[k].name, l)) {
+      const char *q = p + l;
+      while (isspace((unsigned char) *q)) q++;
+      if (*q == '(') {
+        x->p = q + 1; x_expr(x, 0); x_ws(x);
+        if (*x->p != ')') { if (!x->err) x->err = "missing `)'"; return; }
+        x->p++;
+        x_emit(x, fn[k].op, 0, 0);
+        return;
+      }
+    }
+  }
   char *e;
-  long c = strtol(p, &e, 10);
-  if (e == p || c < 1) return -1;
-  if (brace) { if (*e != '}') return -1; e++; }
-  while (isspace((unsigned char) *e)) e++;
-  return *e ? -1 : (int) c;
+  double v = strtod(p, &e);
+  if (e == p) { x->err = "unexpected token"; return; }
+  x->p = e;
+  x_emit(x, X_NUM, 0, v);
 }
 
-/* number of conversions of a C99 formatter, and which are suppressed */
-static int fmt_fields(const char *fmt, int *supp, int max) {
+/* binary operator at x->p: its op code, precedence and length (0: none) */
+static int x_binop(const char *p, int *prec, int *len) {
+  static const struct { const char *s; int op, prec; } ops[] = {
+    {"**", X_POW, 11}, {"*", X_MUL, 10}, {"/", X_DIV, 10}, {"%", X_REM, 10},
+    {"+", X_ADD, 9}, {"-", X_SUB, 9}, {"<=", X_LE, 7}, {">=", X_GE, 7},
+    {"<", X_LT, 7}, {">", X_GT, 7}, {"==", X_EQ, 6}, {"!=", X_NE, 6},
+    {"&&", X_AND, 2}, {"||", X_OR, 1}};
+  if (!strncmp(p, "<<", 2) || !strncmp(p, ">>", 2)) return -1;
+  for (size_t k = 0; k < sizeof ops / sizeof ops[0]; k++) {
+    size_t l = strlen(ops[k].s);
+    if (!strncmp(p, ops[k].s, l)) { *prec = ops[k].prec; *len = (int) l; return ops[k].op; }
+  }
+  if (*p == '&' || *p == '|' || *p == '^') return -1;
+  return 0;
+}
+
+static void x_expr(xprog_t *x, int minprec) {
+  x_unary(x);
+  for (;;) {
+    if (x->err) return;
+    x_ws(x);
+    int prec, len, op = x_binop(x->p, &prec, &len);
+    if (op < 0) { x->err = "bitwise and shift operators are not supported"; return; }
+    if (op == 0 || prec < minprec) return;
+    x->p += len;
+    x_expr(x, prec + 1);                 /* left-associative */
+    x_emit(x, op, 0, 0);
+  }
+}
+
+static int x_compile(const char *src, xprog_t *x) {
+  memset(x, 0, sizeof *x);
+  x->p = src;
+  x_expr(x, 0);
+  x_ws(x);
+  if (!x->err && *x->p) x->err = "unexpected characters";
+  if (!x->err && x->n == 0) x->err = "empty expression";
+  if (x->err) { P_ERR("cannot parse expression `%s': %s\n", src, x->err); free(x->code); x->code = NULL; return FCFC_ERR_AST; }
+  return 0;
+}
+
+static double x_eval(const xprog_t *x, const double *col, double *st) {
+  int sp = 0;
+  for (int i = 0; i < x->n; i++) {
+    const xop_t *o = &x->code[i];
+    double a, b;
+    switch (o->op) {
+      case X_NUM: st[sp++] = o->v; break;
+      case X_COL: st[sp++] = col[o->col - 1]; break;
+      case X_NEG: st[sp - 1] = -st[sp - 1]; break;
+      case X_NOT: st[sp - 1] = !st[sp - 1]; break;
+      case X_ABS: st[sp - 1] = fabs(st[sp - 1]); break;
+      case X_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
+      case X_LN: st[sp - 1] = log(st[sp - 1]); break;
+      case X_LOG: st[sp - 1] = log10(st[sp - 1]); break;
+      case X_ISFIN: st[sp - 1] = isfinite(st[sp - 1]) ? 1 : 0; break;
+      default:
+        b = st[--sp]; a = st[sp - 1];
+        switch (o->op) {
+          case X_POW: a = pow(a, b); break;
+          case X_MUL: a = a * b; break;
+          case X_DIV: a = a / b; break;
+          case X_REM: a = fmod(a, b); break;
+          case X_ADD: a = a + b; break;
+          case X_SUB: a = a - b; break;
+          case X_LT: a = a < b; break;
+          case X_LE: a = a <= b; break;
+          case X_GT: a = a > b; break;
+          case X_GE: a = a >= b; break;
+          case X_EQ: a = a == b; break;
+          case X_NE: a = a != b; break;
+          case X_AND: a = (a != 0) && (b != 0); break;
+          default: a = (a != 0) || (b != 0); break;
+        }
+        st[sp - 1] = a;
+    }
+  }
+  return st[0];
+}
+
+/* conversions of a C99 formatter: whether each is assigned (not %*...) and
+ * numeric (not %s, %c, %[...]) */
+static int fmt_fields(const char *fmt, int *supp, int *isnum, int max) {
   int n = 0;
   for (const char *p = fmt; *p; p++) {
     if (*p != '%') continue;
@@ -195,68 +341,112 @@ static int fmt_fields(const char *fmt, int *supp, int max) {
     if (*p == '*') { s = 1; p++; }
     while (*p && !isalpha((unsigned char) *p) && *p != '[') p++;
     while (*p == 'l' || *p == 'h' || *p == 'L' || *p == 'j' || *p == 'z' || *p == 't') p++;
+    int num = !(*p == 's' || *p == 'c' || *p == '[');
     if (*p == '[') { while (*p && *p != ']') p++; }
-    if (n < max) supp[n] = s;
+    if (n < max) { supp[n] = s; isnum[n] = num; }
     n++;
     if (!*p) break;
   }
   return n;
 }
 
+/* Read an ASCII catalogue: nout expressions (x, y, z[, w]) per selected
+ * row (read_ascii.c:read_ascii_data).  Columns are counted over the
+ * assigned conversions of the formatter ($n: the n-th of them); without a
+ * formatter, over whitespace- or comma-separated fields. */
 static int read_ascii(const char *fname, long skip, char cmt, const char *fmt,
-    const int col[4], int ncol, cat_t *c) {
+    xprog_t *out, int nout, const xprog_t *sel, cat_t *c) {
   FILE *fp = fopen(fname, "r");
   if (!fp) { P_ERR("cannot open catalogue `%s'\n", fname); return FCFC_ERR_FILE; }
-  int supp[256];
-  int nf = fmt ? fmt_fields(fmt, supp, 256) : 0;
-  /* token index of each requested column */
-  int tok[4];
-  for (int k = 0; k < ncol; k++) {
-    if (nf == 0) { tok[k] = col[k] - 1; continue; }
-    int seen = 0; tok[k] = -1;
-    for (int i = 0; i < nf && i < 256; i++) {
-      if (supp[i]) continue;
-      if (++seen == col[k]) { tok[k] = i; break; }
-    }
-    if (tok[k] < 0) { P_ERR("column $%d not in ASCII_FORMATTER\n", col[k]); fclose(fp); return FCFC_ERR_CFG; }
+  int supp[256], isnum[256];
+  int nf = fmt ? fmt_fields(fmt, supp, isnum, 256) : 0;
+  if (nf > 256) nf = 256;
+  int maxcol = sel ? sel->maxcol : 0, depth = sel ? sel->maxdepth : 1;
+  for (int k = 0; k < nout; k++) {
+    if (out[k].maxcol > maxcol) maxcol = out[k].maxcol;
+    if (out[k].maxdepth > depth) depth = out[k].maxdepth;
   }
+  /* field index of column n (1-based) */
+  int *field = malloc(sizeof(int) * (maxcol + 1));
+  int nfield = 0;
+  for (int cn = 1; cn <= maxcol; cn++) {
+    if (nf == 0) { field[cn] = cn - 1; continue; }
+    int seen = 0; field[cn] = -1;
+    for (int i = 0; i < nf; i++) {
+      if (supp[i]) continue;
+      if (++seen == cn) {
+        if (!isnum[i]) { P_ERR("column $%d is not numeric in ASCII_FORMATTER\n", cn); free(field); fclose(fp); return FCFC_ERR_CFG; }
+        field[cn] = i; break;
+      }
+    }
+    if (field[cn] < 0) { P_ERR("column $%d not in ASCII_FORMATTER\n", cn); free(field); fclose(fp); return FCFC_ERR_CFG; }
+  }
+  for (int cn = 1; cn <= maxcol; cn++) if (field[cn] + 1 > nfield) nfield = field[cn] + 1;
+  double *col = malloc(sizeof(double) * (maxcol + 1)), *stack = malloc(sizeof(double) * (depth + 1));
+  double *fv = malloc(sizeof(double) * (nfield + 1));
   size_t cap = 1 << 16;
   c->x = malloc(sizeof(double) * cap); c->y = malloc(sizeof(double) * cap);
-  c->z = malloc(sizeof(double) * cap); c->w = (ncol > 3) ? malloc(sizeof(double) * cap) : NULL;
+  c->z = malloc(sizeof(double) * cap); c->w = (nout > 3) ? malloc(sizeof(double) * cap) : NULL;
   c->n = 0;
   char *line = NULL; size_t lcap = 0;
   long lno = 0;
-  ssize_t len;
-  while ((len = getline(&line, &lcap, fp)) >= 0) {
+  int e = 0;
+  while (getline(&line, &lcap, fp) >= 0) {
     lno++;
     if (lno <= skip) continue;
     char *p = line;
     while (isspace((unsigned char) *p)) p++;
     if (!*p) continue;
     if (cmt && *p == cmt) continue;
-    double v[4];
-    int got = 0, ti = 0;
+    int ti = 0;
     char *save;
-    for (char *t = strtok_r(p, " \t\r\n,", &save); t; t = strtok_r(NULL, " \t\r\n,", &save), ti++) {
-      for (int k = 0; k < ncol; k++) if (tok[k] == ti) { v[k] = strtod(t, NULL); got++; }
-      if (got == ncol) break;
-    }
-    if (got < ncol) { P_ERR("line %ld of `%s' has too few columns\n", lno, fname); free(line); fclose(fp); return FCFC_ERR_ASCII; }
+    for (char *t = strtok_r(p, " \t\r\n,", &save); t && ti < nfield; t = strtok_r(NULL, " \t\r\n,", &save), ti++)
+      fv[ti] = strtod(t, NULL);
+    if (ti < nfield) { P_ERR("line %ld of `%s' has too few columns\n", lno, fname); e = FCFC_ERR_ASCII; break; }
+    for (int cn = 1; cn <= maxcol; cn++) col[cn - 1] = fv[field[cn]];
+    if (sel && x_eval(sel, col, stack) == 0) continue;
     if ((size_t) c->n == cap) {
       cap *= 2;
       c->x = realloc(c->x, sizeof(double) * cap); c->y = realloc(c->y, sizeof(double) * cap);
       c->z = realloc(c->z, sizeof(double) * cap);
       if (c->w) c->w = realloc(c->w, sizeof(double) * cap);
     }
-    c->x[c->n] = v[0]; c->y[c->n] = v[1]; c->z[c->n] = v[2];
-    if (c->w) c->w[c->n] = v[3];
+    c->x[c->n] = x_eval(&out[0], col, stack);
+    c->y[c->n] = x_eval(&out[1], col, stack);
+    c->z[c->n] = x_eval(&out[2], col, stack);
+    if (c->w) c->w[c->n] = x_eval(&out[3], col, stack);
     c->n++;
   }
-  free(line);
+  free(line); free(field); free(col); free(stack); free(fv);
   fclose(fp);
+  if (e) return e;
   c->loaded = 1;
   return 0;
 }
+
+#if FCFC_CLI_GEOM == 1
+/* the first two columns of an ASCII file (Z_CMVDST_FILE): lines that are
+ * empty or start with '#' are skipped (read_ascii.c:read_ascii_table) */
+static int read_table2(const char *fname, double **x, double **y, int64_t *n) {
+  FILE *fp = fopen(fname, "r");
+  if (!fp) { P_ERR("cannot open file for reading: `%s'\n", fname); return FCFC_ERR_FILE; }
+  size_t cap = 1024, m = 0;
+  double *a = malloc(sizeof(double) * cap), *b = malloc(sizeof(double) * cap);
+  char *line = NULL; size_t lcap = 0;
+  int e = 0;
+  while (getline(&line, &lcap, fp) >= 0) {
+    char *p = trim(line);
+    if (!*p || *p == '#') continue;
+    if (m == cap) { cap *= 2; a = realloc(a, sizeof(double) * cap); b = realloc(b, sizeof(double) * cap); }
+    if (sscanf(p, "%lf %lf", a + m, b + m) != 2) { P_ERR("failed to read line: %s\n", p); e = FCFC_ERR_FILE; break; }
+    m++;
+  }
+  free(line);
+  fclose(fp);
+  *x = a; *y = b; *n = (int64_t) m;
+  return e;
+}
+#endif
 
 /* ---------------------------------------------------------------------- */
 /* pair-count files: reader for the resume path (OVERWRITE = 1).           */
@@ -520,7 +710,12 @@ int main(int argc, char **argv) {
   memset(&bins, 0, sizeof bins);
   cat_t *cat = NULL;
   double **cnt = NULL, **ncnt = NULL, *norm = NULL;
-  int *weighted = NULL, *comp_pc = NULL;
+  int *weighted = NULL, *comp_pc = NULL, *cnvt = NULL, any_cnvt = 0;
+  char **sels = NULL;
+  int nsel = 0;
+  const char *zfile = NULL;
+  fcfc_cosmology cosmo;
+  memset(&cosmo, 0, sizeof cosmo);
 
   ncat = split_list(kv_get("CATALOG"), &cats);
   if (ncat < 1) { P_ERR("CATALOG is not set\n"); return FCFC_ERR_CFG; }
@@ -541,12 +736,42 @@ int main(int argc, char **argv) {
   if (npos != 3 * ncat) { P_ERR("POSITION needs 3 entries per catalogue\n"); FAIL(FCFC_ERR_CFG); }
   if (nfmt != ncat) { P_ERR("ASCII_FORMATTER is not set for every catalogue\n"); FAIL(FCFC_ERR_CFG); }
   for (int i = 0; i < ntype; i++) if (atoi(types[i]) != 0) { P_ERR("only ASCII catalogues (CATALOG_TYPE 0) are supported\n"); FAIL(FCFC_ERR_CFG); }
-  if (kv_get("SELECTION")) {
-    char **sel; int ns = split_list(kv_get("SELECTION"), &sel);
-    for (int i = 0; i < ns; i++) if (sel[i][0]) { P_ERR("SELECTION is not supported (catalogues are read whole)\n"); free_list(sel, ns); FAIL(FCFC_ERR_CFG); }
-    free_list(sel, ns);
+  nsel = split_list(kv_get("SELECTION"), &sels);
+  /* COORD_CONVERT (survey): one flag for every catalogue or one each, with
+   * the fiducial cosmology or a (redshift, distance) table
+   * (2pt/load_conf.c:886-935) */
+#if FCFC_CLI_GEOM == 1
+  {
+    char **cv; int ncv = split_list(kv_get("COORD_CONVERT"), &cv);
+    cnvt = calloc(ncat, sizeof(int));
+    if (ncv > 1 && ncv < ncat) { P_ERR("too few elements of COORD_CONVERT\n"); free_list(cv, ncv); FAIL(FCFC_ERR_CFG); }
+    for (int k = 0; k < ncat && ncv; k++) {
+      const char *v = cv[ncv == 1 ? 0 : k];
+      cnvt[k] = (v[0] == 'T' || v[0] == 't' || v[0] == '1' || v[0] == 'Y' || v[0] == 'y');
+      any_cnvt |= cnvt[k];
+    }
+    free_list(cv, ncv);
+    if (any_cnvt) {
+      zfile = kv_get("Z_CMVDST_FILE");
+      if (!zfile) {
+        double v;
+        cosmo.omega_m = get_double("OMEGA_M", &v) ? NAN : v;
+        cosmo.omega_l = get_double("OMEGA_LAMBDA", &v) ? NAN : v;
+        cosmo.w = get_double("DE_EOS_W", &v) ? NAN : v;
+        cosmo.err = get_double("CMVDST_ERR", &v) ? NAN : v;
+        if (isnan(cosmo.omega_m)) { P_ERR("OMEGA_M is not set\n"); FAIL(FCFC_ERR_CFG); }
+        if (fcfc_cosmology_check(&cosmo)) {
+          P_ERR("invalid cosmology for COORD_CONVERT (OMEGA_M in (0,1], OMEGA_LAMBDA >= 0, "
+                "DE_EOS_W <= -1/3, positive H^2, CMVDST_ERR >= machine epsilon)\n");
+          FAIL(FCFC_ERR_CFG);
+        }
+      }
+    }
   }
-  if (get_bool("COORD_CONVERT", 0)) { P_ERR("COORD_CONVERT is not supported (give comoving Cartesian coordinates)\n"); FAIL(FCFC_ERR_CFG); }
+#else
+  (void) zfile; (void) any_cnvt;
+  if (get_bool("COORD_CONVERT", 0)) { P_ERR("COORD_CONVERT applies to survey catalogues (FCFC_2PT)\n"); FAIL(FCFC_ERR_CFG); }
+#endif
   if (npc < 1 || npco != npc) { P_ERR("PAIR_COUNT and PAIR_COUNT_FILE must be set with equal lengths\n"); FAIL(FCFC_ERR_CFG); }
   if (ncf != ncfo) { P_ERR("CF_ESTIMATOR and CF_OUTPUT_FILE must have equal lengths\n"); FAIL(FCFC_ERR_CFG); }
   const int bintype = get_int("BINNING_SCHEME", 0);
@@ -705,21 +930,42 @@ int main(int argc, char **argv) {
     for (int s = 0; s < 2; s++) {
       int k = pci[s][i];
       if (cat[k].loaded) continue;
-      int col[4], nc = 3;
-      for (int d = 0; d < 3; d++) {
-        col[d] = colref(pos[3 * k + d]);
-        if (col[d] < 0) { P_ERR("POSITION `%s' must be a column reference ${n}\n", pos[3 * k + d]); FAIL(FCFC_ERR_CFG); }
-      }
-      if (has_wt[k]) {
-        col[3] = colref(wts[k]);
-        if (col[3] < 0) { P_ERR("WEIGHT `%s' must be a column reference ${n}\n", wts[k]); FAIL(FCFC_ERR_CFG); }
-        nc = 4;
-      }
+      /* POSITION, WEIGHT and SELECTION expressions of catalogue k */
+      xprog_t xp[4], xs;
+      int nc = has_wt[k] ? 4 : 3, hs = (k < nsel && sels[k][0]);
+      memset(xp, 0, sizeof xp);
+      memset(&xs, 0, sizeof xs);
+      int ce = 0;
+      for (int d = 0; d < 3 && !ce; d++) ce = x_compile(pos[3 * k + d], &xp[d]);
+      if (!ce && nc == 4) ce = x_compile(wts[k], &xp[3]);
+      if (!ce && hs) ce = x_compile(sels[k], &xs);
       printf("Construct the grid for catalog '%c' ...\n", lab[k]);
       long skip = (k < nskip) ? atol(skips[k]) : 0;
       char cm = (k < ncmt && cmts[k][0]) ? cmts[k][0] : '\0';
-      if ((ret = read_ascii(cats[k], skip, cm, fmts[k], col, nc, &cat[k]))) goto done;
+      if (!ce) ce = read_ascii(cats[k], skip, cm, fmts[k], xp, nc, hs ? &xs : NULL, &cat[k]);
+      for (int d = 0; d < 4; d++) free(xp[d].code);
+      free(xs.code);
+      if (ce) { ret = (ce == FCFC_ERR_AST) ? FCFC_ERR_CFG : ce; goto done; }
       if (verbose) printf("  Number of recorded objects: %lld\n", (long long) cat[k].n);
+#if FCFC_CLI_GEOM == 1
+      /* (RA, Dec, z) -> comoving Cartesian (cnvt_coord.c:cnvt_coord) */
+      if (cnvt && cnvt[k]) {
+        int e2, order = 0;
+        if (zfile) {
+          double *zs = NULL, *ds = NULL;
+          int64_t ns = 0;
+          if ((e2 = read_table2(zfile, &zs, &ds, &ns))) { free(zs); free(ds); ret = e2; goto done; }
+          e2 = fcfc_convert_coords_table(zs, ds, ns, cat[k].n, cat[k].x, cat[k].y, cat[k].z);
+          free(zs); free(ds);
+        }
+        else e2 = fcfc_convert_coords(&cosmo, cat[k].n, cat[k].x, cat[k].y, cat[k].z, &order);
+        if (e2) { P_ERR("coordinate conversion of catalog '%c' failed (%d)\n", lab[k], e2); ret = FCFC_ERR_CNVT; goto done; }
+        if (verbose) {
+          if (zfile) printf("  Coordinates converted using cubic spline interpolation\n");
+          else printf("  Coordinates converted using Legendre-Gauss integration with order %d\n", order);
+        }
+      }
+#endif
     }
     /* catalogue preparation (rescale, survey |r|^2) */
     fcfc_cat ca, cb;
@@ -862,7 +1108,8 @@ done:
   if (cat) for (int k = 0; k < ncat; k++) { free(cat[k].x); free(cat[k].y); free(cat[k].z); free(cat[k].w); }
   free(cat);
   if (cnt) for (int i = 0; i < npc; i++) { free(cnt[i]); free(ncnt[i]); }
-  free(cnt); free(ncnt); free(norm); free(weighted); free(comp_pc);
+  free(cnt); free(ncnt); free(norm); free(weighted); free(comp_pc); free(cnvt);
+  free_list(sels, nsel);
   free_list(cats, ncat); free_list(labels, nlab); free_list(pos, npos); free_list(wts, nwt);
   free_list(fmts, nfmt); free_list(pcs, npc); free_list(pcout, npco); free_list(cfs, ncf);
   free_list(cfout, ncfo); free_list(mpout, nmpo); free_list(wpout, nwpo); free_list(poles, npole);

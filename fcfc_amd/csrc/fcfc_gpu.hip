@@ -694,6 +694,7 @@ __global__ void __launch_bounds__(FK_NT, WT ? 6 : 8)
 k_count(const FkArgs a) {
   extern __shared__ __attribute__((aligned(16))) char lds[];
   constexpr bool F32 = (MODE == MODE_C32 || MODE == MODE_C32C);
+  constexpr bool CENS = (MODE == MODE_C32C);
   constexpr bool PLAIN = (MODE == MODE_P64);
   constexpr bool GRAD = (MODE == MODE_GRAD);
   constexpr bool NEED_S = (GEOM == FCFC_GEOM_SURVEY && BIN != FCFC_BIN_ISO);
@@ -716,7 +717,9 @@ k_count(const FkArgs a) {
   unsigned *ptab_raw = (unsigned *) (lds + a.off_ptab);
   const unsigned *stab = HYB ? stab_raw + a.stab_c0 : stab_raw;
   const unsigned *ptab = HYB ? ptab_raw + a.ptab_c0 : ptab_raw;
-  unsigned *wq = (unsigned *) (lds + a.off_queue) + wave * FK_QCAP;   /* FP32 re-evaluation queue */
+  /* FP32 re-evaluation queue of this wavefront: (A slot in the wavefront
+   * << 8) | chunk index */
+  uint16_t *wq = (uint16_t *) (lds + a.off_queue) + wave * FK_QCAP;
   double *s2e = (double *) (lds + a.off_edges);
   double *pe = s2e + (bn.ns + 1);
   FkStripLds *st = (FkStripLds *) (lds + a.off_strip);
@@ -873,18 +876,17 @@ k_count(const FkArgs a) {
     /* FP32: FP64 re-evaluation of the n queued pairs of this wavefront
      * (entries: A slot << 8 | chunk index), while the chunk is staged */
     const int lane = tid & 63;
-    const unsigned long long lanemask_lt = (1ull << lane) - 1;
     auto drain = [&](int n) {
       for (int e0 = 0; e0 < n; e0 += 64) {
-        const unsigned v = (e0 + lane < n) ? wq[e0 + lane] : 0xFFFFFFFFu;
-        const int p = t.p0 + (int) (v >> 8);
+        const unsigned v = (e0 + lane < n) ? (unsigned) wq[e0 + lane] : 0xFFFFu;
+        const int p = t.p0 + wave * 64 * R + (int) (v >> 8);
         /* entries of idle lanes (possible only when the windows cover
          * everything) are dropped */
         if (e0 + lane < n && p < t.p1) {
           const int j = (int) (v & 255u);
           const double qx = J.A.x[p], qy = J.A.y[p], qz = J.A.z[p];
-          const unsigned idx = fk_recheck<BIN, HYB, MODE == MODE_C32C>(bn, J, qx, qy, qz, cj[j], cc[j],
-                                                                       stab, ptab, s2e, pe, &ncensus);
+          const unsigned idx = fk_recheck<BIN, HYB, CENS>(bn, J, qx, qy, qz, cj[j], cc[j],
+                                                          stab, ptab, s2e, pe, &ncensus);
           nrecheck++;
           if (idx < ntot) {
             if (WT && jwt) atomicAdd(myhist_d + idx, J.A.w[p] * cw[j]);
@@ -971,8 +973,9 @@ k_count(const FkArgs a) {
             idx = (amb | ambm) ? FK_NONE : idx;
             if (m) {
               if ((m >> lane) & 1ull) {
-                const int pos = qn + __popcll(m & lanemask_lt);
-                wq[pos] = ((unsigned) (tid * R + r) << 8) | (unsigned) j;
+                const int pos = qn + (int) __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
+                wq[pos] = (uint16_t) (((unsigned) (lane * R + r) << 8) | (unsigned) j);
               }
               qn += __popcll(m);
               if (qn > FK_QCAP - 64) { drain(qn); qn = 0; }
@@ -1702,7 +1705,7 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   if (mode == MODE_C32) per += 4;
   o = align16(o + FK_CH * per);
   L.off_queue = o;
-  if (mode == MODE_C32) o += (FK_NT / 64) * FK_QCAP * 4;
+  if (mode == MODE_C32) o += (FK_NT / 64) * FK_QCAP * 2;
   o = align16(o);
   const int zero_based = (!hyb || mode == MODE_C32);
   L.stab_c0 = zero_based ? b->soff : 1;
@@ -1764,19 +1767,38 @@ static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int R
  * evaluated at the largest in-range s^2 (plus the first integer above it:
  * beyond that the table answers "outside" without a window).  The mu bin
  * floor(t), t = nmu |dz| / s, has |t32 - t| <= t 10u + (sqrt(3) u E t +
- * nmu u E) / s; with s^2 >= s2_floor = s2max / 64 and t <= nmu this is a
- * constant D, and the bin is certain when |fract(t32) - 1/2| <= 1/2 - D.
- * All bounds are inflated by 25%; pairs below s2_floor (about 1/512 of the
- * in-range volume) are always re-evaluated. */
+ * nmu u E) / s; with s^2 >= s2_floor (fp32_s2_floor) and t <= nmu this is
+ * a constant D, and the bin is certain when |fract(t32) - 1/2| <= 1/2 - D.
+ * All bounds are inflated by 25%; pairs below s2_floor are always
+ * re-evaluated. */
 static double fp32_extent(const FkGrid &g, int zmaxc) {
   return std::max(g.wx, std::max(g.wy, (zmaxc + 1) * g.wz));
 }
 
+static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double s2f,
+    FkWin *win);
+
 /* s^2 below which a job's pairs are always re-evaluated (the LDS table flags
- * those cells once per launch, so a launch uses the largest over its jobs) */
+ * those cells once per launch, so a launch uses the largest over its jobs).
+ * For (s, mu) bins the floor trades the pairs below it -- about
+ * ((floor + 1) / s2max)^(3/2) of a uniform catalogue's in-range pairs,
+ * counting the flagged cell above it -- against the mu windows, 2 D of the
+ * pairs above it with D growing as 1 / sqrt(floor): the candidate with the
+ * fewest expected re-evaluations is taken. */
 static double fp32_s2_floor(const fcfc_bins *b, const FkGrid &g, int zmaxc) {
   const double u = ldexp(1.0, -24);
-  return std::max(b->s2bin[b->ns] / 64.0, pow(1e4 * u * fp32_extent(g, zmaxc), 2));
+  const double s2max = b->s2bin[b->ns];
+  const double lo = pow(1e4 * u * fp32_extent(g, zmaxc), 2);
+  if (b->bintype != FCFC_BIN_SMU) return std::max(s2max / 64.0, lo);
+  double best = std::max(s2max / 64.0, lo), bcost = 1e300;
+  for (double x = 1.0 / 64; x >= 1.0 / 16384; x *= 0.5) {
+    const double s2f = std::max(x * s2max, lo);
+    FkWin w;
+    fp32_windows(b, g, zmaxc, s2f, &w);
+    const double cost = pow(std::min((s2f + 1) / s2max, 1.0), 1.5) + 2.0 * w.mu_sh;
+    if (cost < bcost) { bcost = cost; best = s2f; }
+  }
+  return best;
 }
 
 static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double s2f,
@@ -1954,6 +1976,11 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   }
   /* ---- kernel and LDS plan ---- */
   const int R = jobs.empty() ? 1 : jobs[0].P.R;
+  auto occ_of = [&](kfn_t f, const LdsPlan &P, int *occ) -> int {
+    HIPCHK(c, hipFuncSetAttribute((const void *) f, hipFuncAttributeMaxDynamicSharedMemorySize, P.total));
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, (const void *) f, FK_NT, P.total));
+    return 0;
+  };
   kfn_t fn = pick_kernel(b->bintype, geom, anywt, (m == MODE_C32 && c->census) ? MODE_C32C : m, hyb, R);
   /* histogram copies: as many as keep the occupancy of a single copy */
   LdsPlan L = lds_plan(b, m, anywt, need_s, 1, hyb, ssearch, psearch);
@@ -1962,19 +1989,14 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     return FCFC_ERR_CFG;
   }
   if (m != MODE_GRAD) {
-    auto occ_of = [&](const LdsPlan &P, int *occ) -> int {
-      HIPCHK(c, hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize, P.total));
-      HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, (const void *) fn, FK_NT, P.total));
-      return 0;
-    };
     int occ1 = 0;
-    if (int e = occ_of(L, &occ1)) return e;
+    if (int e = occ_of(fn, L, &occ1)) return e;
     const int nhmax = c->t_nh > 0 ? c->t_nh : 8;
     for (int nh = 2; nh <= nhmax; nh *= 2) {
       LdsPlan L2 = lds_plan(b, m, anywt, need_s, nh, hyb, ssearch, psearch);
       int occ = 0;
       if (L2.total > lds_max) break;
-      if (int e = occ_of(L2, &occ)) return e;
+      if (int e = occ_of(fn, L2, &occ)) return e;
       if (occ < occ1 && c->t_nh <= 0) break;
       L = L2;
     }

@@ -594,3 +594,196 @@ int fcfc_gen_sizes(uint64_t seed, int n, double lo, double hi, int64_t *out) {
   for (int i = 0; i < n; i++) out[i] = (int64_t) floor(gen_exp(a + (b - a) * rng_u01(&r)) + 0.5);
   return 0;
 }
+
+/* --------------------------------------------------------------------- */
+/* Survey coordinate conversion: (RA, Dec) in degrees and redshift ->     */
+/* comoving Cartesian coordinates in Mpc/h (src/fcfc/2pt/cnvt_coord.c).   */
+/* The comoving distance is the integral of c/H(z) (H0 = 100 h km/s/Mpc)  */
+/* by Gauss-Legendre quadrature of an order chosen by a convergence test  */
+/* over the catalogue's redshift range, or a natural cubic spline through */
+/* (z, distance) samples.  The arithmetic follows the reference's         */
+/* expression order (same libm calls, no contraction), so converted       */
+/* coordinates are bit-identical on the same C library.                   */
+#include "fcfc_legauss.h"
+
+#define CM_SPEED_OF_LIGHT 299792.458            /* define_comm.h:124 */
+#define CM_DEG2RAD        0x1.1df46a2529d39p-6   /* pi / 180, define_comm.h:137 */
+#define CM_NUM_ZSAMPLE    128                    /* 2pt/define.h:99 */
+
+int fcfc_cosmology_check(fcfc_cosmology *cm) {
+  if (!cm) return FCFC_ERR_ARG;
+  if (!(cm->omega_m > 0 && cm->omega_m <= 1)) return FCFC_ERR_CFG;   /* load_conf.c:700-705 */
+  if (isnan(cm->omega_l)) { cm->omega_l = 1 - cm->omega_m; cm->omega_k = 0; }
+  else if (cm->omega_l < 0) return FCFC_ERR_CFG;
+  else cm->omega_k = 1 - cm->omega_m - cm->omega_l;
+  if (isnan(cm->w)) cm->w = -1;
+  else if (cm->w > -1 / (double) 3) return FCFC_ERR_CFG;
+  /* H^2(z) > 0 for all z (load_conf.c:726-733) */
+  const double w3 = cm->w * 3, widx = w3 + 1;
+  if (cm->omega_k * pow(cm->omega_l * (-widx), widx / w3) <=
+      cm->omega_l * w3 * pow(cm->omega_m, widx / w3)) return FCFC_ERR_CFG;
+  if (isnan(cm->err)) cm->err = 1e-8;
+  else if (cm->err < 0x1p-52) return FCFC_ERR_CFG;
+  return 0;
+}
+
+/* c / H(z) (cnvt_coord.c:cnvt_integrand) */
+static double cm_integrand(const fcfc_cosmology *cm, double widx, double z) {
+  const double z1 = z + 1;
+  const double z2 = z1 * z1;
+  double d = cm->omega_m * z2 * z1;
+  if (cm->omega_k) d += cm->omega_k * z2;
+  if (widx) d += cm->omega_l * pow(z1, widx);
+  else d += cm->omega_l;
+  return CM_SPEED_OF_LIGHT * 0.01 / sqrt(d);
+}
+
+/* integral from 0 to z with the rule of the given order, abscissas from
+ * the outermost in and the centre last (cnvt_coord.c:cnvt_legauss) */
+static double cm_legauss(const fcfc_cosmology *cm, double widx, int order, double z) {
+  const fcfc_lg_rule *q = &fcfc_lg_rules[order - FCFC_LG_MIN_ORDER];
+  const double zp = z * 0.5;
+  double sum = 0;
+  for (int i = 0; i < order / 2; i++) {
+    const double z1 = zp * (1 + q->x[i]);
+    const double z2 = zp * (1 - q->x[i]);
+    sum += q->w[i] * (cm_integrand(cm, widx, z1) + cm_integrand(cm, widx, z2));
+  }
+  if (order & 1) sum += q->mid * cm_integrand(cm, widx, zp);
+  return sum * zp;
+}
+
+int fcfc_cmvdst_order(const fcfc_cosmology *cm, int64_t n, const double *z, int *order) {
+  if (!cm || !order || n < 0 || (n && !z)) return FCFC_ERR_ARG;
+  double zmin = 1.79769313486231570815e308, zmax = -1.79769313486231570815e308;
+  for (int64_t i = 0; i < n; i++) {
+    if (z[i] < 0) return FCFC_ERR_DATA;                 /* cnvt_coord.c:cnvt_z_sample */
+    if (zmax < z[i]) zmax = z[i];
+    if (zmin > z[i]) zmin = z[i];
+  }
+  if (zmin > zmax) return FCFC_ERR_DATA;
+  const double widx = (cm->w == -1) ? 0 : 3 * (1 + cm->w);
+  int best = 0;
+  for (int i = 0; i < CM_NUM_ZSAMPLE; i++) {
+    const double zs = zmin + i * (zmax - zmin) / (CM_NUM_ZSAMPLE - 1);
+    double oint, nint = 0;
+    int k = FCFC_LG_MIN_ORDER - 1;
+    do {
+      /* the reference would read past its table at order 33: here the
+       * test fails instead */
+      if (k + 1 > FCFC_LG_MAX_ORDER) return FCFC_ERR_CNVT;
+      oint = nint;
+      nint = cm_legauss(cm, widx, ++k, zs);
+    } while (fabs(nint - oint) > nint * cm->err);
+    if (best < k) best = k;
+  }
+  *order = best;
+  return 0;
+}
+
+static void cm_to_cartesian(double dist, double *x, double *y, double *z) {
+  const double ra = *x * CM_DEG2RAD;
+  const double dec = *y * CM_DEG2RAD;
+  *x = dist * cos(dec) * cos(ra);
+  *y = dist * cos(dec) * sin(ra);
+  *z = dist * sin(dec);
+}
+
+int fcfc_convert_coords(const fcfc_cosmology *cm, int64_t n, double *x, double *y,
+    double *z, int *order_used) {
+  if (!cm || n < 0 || (n && (!x || !y || !z))) return FCFC_ERR_ARG;
+  int order = 0;
+  int e = fcfc_cmvdst_order(cm, n, z, &order);
+  if (e) return e;
+  const double widx = (cm->w == -1) ? 0 : 3 * (1 + cm->w);
+  for (int64_t i = 0; i < n; i++)
+    cm_to_cartesian(cm_legauss(cm, widx, order, z[i]), x + i, y + i, z + i);
+  if (order_used) *order_used = order;
+  return 0;
+}
+
+/* Natural cubic spline through (zs[i], ds[i]), zs increasing: second
+ * derivatives by the tridiagonal sweep (forward elimination, back
+ * substitution), evaluated with the usual cubic form -- the operation
+ * order of src/math/cspline.c, so interpolated distances match bit for bit. */
+typedef struct { int64_t n; double *z, *d, *ypp; } cm_spline;
+
+static int cm_cmp_pair(const void *a, const void *b) {
+  const double *p = (const double *) a, *q = (const double *) b;
+  if (p[0] < q[0]) return -1;
+  if (p[0] > q[0]) return 1;
+  return (p[2] < q[2]) ? -1 : (p[2] > q[2]);      /* stable: by input position */
+}
+
+static int cm_spline_init(cm_spline *s, const double *zs, const double *ds, int64_t n) {
+  s->n = n;
+  s->z = malloc(sizeof(double) * n);
+  s->d = malloc(sizeof(double) * n);
+  s->ypp = malloc(sizeof(double) * n);
+  double *tmp = malloc(sizeof(double) * 3 * n), *cp = malloc(sizeof(double) * n);
+  if (!s->z || !s->d || !s->ypp || !tmp || !cp) { free(tmp); free(cp); return FCFC_ERR_MEMORY; }
+  for (int64_t i = 0; i < n; i++) { tmp[3 * i] = zs[i]; tmp[3 * i + 1] = ds[i]; tmp[3 * i + 2] = (double) i; }
+  qsort(tmp, (size_t) n, 3 * sizeof(double), cm_cmp_pair);
+  for (int64_t i = 0; i < n; i++) { s->z[i] = tmp[3 * i]; s->d[i] = tmp[3 * i + 1]; }
+  free(tmp);
+  const double *x = s->z, *y = s->d;
+  double *ypp = s->ypp;
+  double xn = x[1], yn = y[1], h = x[1] - x[0];
+  double slope = (y[1] - y[0]) / h;
+  cp[0] = cp[n - 1] = ypp[0] = ypp[n - 1] = 0;
+  int64_t j = 1;
+  for (; j < n - 1; j++) {
+    const double xo = xn, yo = yn, hprev = h, so = slope;
+    xn = x[j + 1];
+    yn = y[j + 1];
+    h = xn - xo;
+    slope = (yn - yo) / h;
+    const double diag = (h + hprev) * 2;
+    const double inv = 1 / (diag - hprev * cp[j - 1]);
+    const double rhs = (slope - so) * 6;
+    ypp[j] = (rhs - hprev * ypp[j - 1]) * inv;
+    cp[j] = h * inv;
+  }
+  while (j) {
+    j -= 1;
+    ypp[j] -= cp[j] * ypp[j + 1];
+  }
+  free(cp);
+  return 0;
+}
+
+static void cm_spline_free(cm_spline *s) { free(s->z); free(s->d); free(s->ypp); }
+
+/* distance at zv, z[0] <= zv < z[n-1]: the interval z[i] <= zv < z[i+1] */
+static double cm_spline_eval(const cm_spline *s, double zv) {
+  int64_t lo = 0, hi = s->n - 1;                /* z[lo] <= zv < z[hi] */
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (s->z[mid] <= zv) lo = mid; else hi = mid;
+  }
+  const int64_t i = lo, k = i + 1;
+  const double *x = s->z, *y = s->d, *ypp = s->ypp;
+  const double ba = x[k] - x[i];
+  const double xa = zv - x[i];
+  const double bx = x[k] - zv;
+  const double ba2 = ba * ba;
+  const double lower = xa * y[k] + bx * y[i];
+  const double c = (xa * xa - ba2) * xa * ypp[k];
+  const double d = (bx * bx - ba2) * bx * ypp[i];
+  return (lower + 0x1.5555555555555p-3 * (c + d)) / ba;      /* 1/6 */
+}
+
+int fcfc_convert_coords_table(const double *zs, const double *ds, int64_t nsample,
+    int64_t n, double *x, double *y, double *z) {
+  if (nsample < 2 || !zs || !ds || n < 0 || (n && (!x || !y || !z))) return FCFC_ERR_ARG;
+  cm_spline s;
+  int e = cm_spline_init(&s, zs, ds, nsample);
+  if (e) { cm_spline_free(&s); return e; }
+  /* a redshift outside the samples has no distance: the reference would
+   * carry an infinite coordinate on; here the conversion fails */
+  for (int64_t i = 0; i < n; i++)
+    if (!(z[i] >= s.z[0] && z[i] < s.z[s.n - 1])) { cm_spline_free(&s); return FCFC_ERR_CNVT; }
+  for (int64_t i = 0; i < n; i++) cm_to_cartesian(cm_spline_eval(&s, z[i]), x + i, y + i, z + i);
+  cm_spline_free(&s);
+  return 0;
+}

@@ -264,6 +264,41 @@ int fcfc_save_pair_count(const char *fname, const fcfc_bins *bins, int ofmt,
     const double *bsize_rescaled, int isauto, int weighted, uint64_t n1,
     uint64_t n2, double wt1, double wt2, const void *cnt);
 
+/* ---- survey coordinate conversion (src/fcfc/2pt/cnvt_coord.c) ---------- */
+/* Cosmology of COORD_CONVERT (src/fcfc/2pt/load_conf.c check_cosmo): pass
+ * NaN for an unset key.  omega_l unset: flat, 1 - omega_m; w unset: -1;
+ * err unset: 1e-8.  fcfc_cosmology_check fills the defaults and omega_k and
+ * validates (omega_m in (0,1], omega_l >= 0, w <= -1/3, H^2(z) > 0,
+ * err >= DBL_EPSILON), returning FCFC_ERR_CFG otherwise. */
+typedef struct {
+  double omega_m;        /* OMEGA_M                                         */
+  double omega_l;        /* OMEGA_LAMBDA                                    */
+  double omega_k;        /* 1 - omega_m - omega_l (set by the check)        */
+  double w;              /* DE_EOS_W                                        */
+  double err;            /* CMVDST_ERR: relative accuracy of the integral   */
+} fcfc_cosmology;
+int fcfc_cosmology_check(fcfc_cosmology *cm);
+
+/* Order of the Gauss-Legendre rule (4-32) that integrates c/H(z) to the
+ * relative accuracy cm->err at 128 redshifts spread over [min z, max z]
+ * (cnvt_coord.c:cnvt_legauss_order); FCFC_ERR_DATA for a negative redshift,
+ * FCFC_ERR_CNVT when order 32 does not converge. */
+int fcfc_cmvdst_order(const fcfc_cosmology *cm, int64_t n, const double *z,
+    int *order);
+
+/* In place: x = RA (deg), y = Dec (deg), z = redshift -> comoving Cartesian
+ * coordinates in Mpc/h (cnvt_coord.c:cnvt_coord_integr), with the order of
+ * fcfc_cmvdst_order (returned in order_used, may be NULL).  Host pointers. */
+int fcfc_convert_coords(const fcfc_cosmology *cm, int64_t n, double *x,
+    double *y, double *z, int *order_used);
+
+/* The same with distances from a natural cubic spline through nsample
+ * (redshift, distance) samples in any order (Z_CMVDST_FILE,
+ * cnvt_coord.c:cnvt_coord_interp).  A redshift outside [min zs, max zs)
+ * fails with FCFC_ERR_CNVT. */
+int fcfc_convert_coords_table(const double *zs, const double *ds,
+    int64_t nsample, int64_t n, double *x, double *y, double *z);
+
 /* ---- deterministic catalogue generator (tests and benchmarks) ----------- */
 /* Uniform box [0,L)^3. */
 int fcfc_gen_uniform(uint64_t seed, int64_t n, double L, double *x,

@@ -13,6 +13,8 @@ run with a weighted D and an unweighted R makes the upstream binary crash.
 Usage:  python tests/golden/make_golden.py          (needs oracle/_ref built)
         python tests/golden/make_golden.py --sizes  (the cfg_* cases at the
                                                      configurations' sizes)
+        python tests/golden/make_golden.py --cli    (the cli_* cases: outputs
+                                                     of the executables)
 """
 import hashlib
 import json
@@ -150,7 +152,129 @@ def run_case(name, code, kind, pairs, bk, flavours):
     return out
 
 
+# Cases of the drop-in executables: a configuration template ({d} = the
+# working directory holding the catalogues cat0.txt, cat1.txt, ...) run by
+# the upstream executable; the fixture keeps its output files, parsed.
+def cli_catalogs(kind):
+    """Text columns of the catalogues of a CLI case (lists of arrays)."""
+    if kind == "box_plain":
+        (x, y, z, w), (xr, yr, zr) = catalogs("box_clustered")[0]
+        return [[x, y, z], [xr, yr, zr]]
+    if kind == "box_expr":
+        # x, y, z, w plus a uniform flag column: POSITION / WEIGHT /
+        # SELECTION expressions over five columns
+        (x, y, z, w), _ = catalogs("box_clustered")[0]
+        flag = np.random.default_rng(51).random(len(x))
+        return [[x, y, z, w, flag]]
+    if kind == "survey_radec":
+        # RA, Dec in degrees over a 10 x 10 deg patch, redshift in [0.4, 0.45]
+        g = np.random.default_rng(61)
+        out = []
+        for n in (3000, 6000):
+            out.append([g.uniform(-5, 5, n), g.uniform(-5, 5, n), g.uniform(0.4, 0.45, n)])
+        return out
+    raise KeyError(kind)
+
+
+def eds_table():
+    """(z, comoving distance) samples of the Einstein-de Sitter closed form
+    D(z) = 2 (c / H0) (1 - 1 / sqrt(1 + z)), c / H0 = 2997.92458 Mpc/h, for
+    the Z_CMVDST_FILE case."""
+    z = np.linspace(0.3, 0.6, 61)
+    return z, 2 * 2997.92458 * (1 - 1 / np.sqrt(1 + z))
+
+
+SMU_CF = ("CATALOG = [{d}/cat0.txt, {d}/cat1.txt]\nCATALOG_LABEL = [D, R]\n"
+          'ASCII_FORMATTER = ["%lf %lf %lf", "%lf %lf %lf"]\n'
+          "POSITION = [$1, $2, $3, $1, $2, $3]\nBOX_SIZE = 1000\nBINNING_SCHEME = 1\n"
+          "PAIR_COUNT = [DD, DR, RR]\nPAIR_COUNT_FILE = [{d}/DD.txt, {d}/DR.txt, {d}/RR.txt]\n"
+          "CF_ESTIMATOR = [(DD - 2*DR + RR)/RR, DD/@@ - 1]\n"
+          "CF_OUTPUT_FILE = [{d}/xi_ls.txt, {d}/xi_nat.txt]\nMULTIPOLE = [4, 0, 2, 2]\n"
+          "MULTIPOLE_FILE = [{d}/mp_ls.txt, {d}/mp_nat.txt]\n"
+          "SEP_BIN_MIN = 0\nSEP_BIN_MAX = 150\nSEP_BIN_SIZE = 5\nMU_BIN_NUM = 20\n"
+          "OUTPUT_FORMAT = 1\nOVERWRITE = 2\n")
+SPI_WP = ("CATALOG = [{d}/cat0.txt, {d}/cat1.txt]\nCATALOG_LABEL = [D, R]\n"
+          'ASCII_FORMATTER = ["%lf %lf %lf", "%lf %lf %lf"]\n'
+          "POSITION = [$1, $2, $3, $1, $2, $3]\nBOX_SIZE = 1000\nBINNING_SCHEME = 2\n"
+          "PAIR_COUNT = [DD, DR, RR]\nPAIR_COUNT_FILE = [{d}/DD.bin, {d}/DR.bin, {d}/RR.bin]\n"
+          "CF_ESTIMATOR = (DD - 2*DR + RR)/RR\nCF_OUTPUT_FILE = {d}/xi.txt\n"
+          "PROJECTED_CF = T\nPROJECTED_FILE = {d}/wp.txt\n"
+          "SEP_BIN_MIN = 0\nSEP_BIN_MAX = 100\nSEP_BIN_SIZE = 5\n"
+          "PI_BIN_MIN = 0\nPI_BIN_MAX = 60\nPI_BIN_SIZE = 4\nOUTPUT_FORMAT = 0\nOVERWRITE = 2\n")
+EXPR = ("CATALOG = {d}/cat0.txt\nCATALOG_LABEL = D\n"
+        'ASCII_FORMATTER = "%lf %lf %lf %lf %lf"\n'
+        "POSITION = [$1*0.5 + 250, $2, ($3 + 100) % 1000]\nWEIGHT = $4 * 2 - 0.5*$5\n"
+        "SELECTION = $5 < 0.7 && ($4 > 0.8 || $1 < 300)\n"
+        "BOX_SIZE = 1000\nBINNING_SCHEME = 1\nPAIR_COUNT = DD\nPAIR_COUNT_FILE = {d}/DD.bin\n"
+        "SEP_BIN_MIN = 0\nSEP_BIN_MAX = 200\nSEP_BIN_SIZE = 5\nMU_BIN_NUM = 10\n"
+        "OUTPUT_FORMAT = 0\nOVERWRITE = 2\n")
+CONV = ("CATALOG = [{d}/cat0.txt, {d}/cat1.txt]\nCATALOG_LABEL = [D, R]\n"
+        'ASCII_FORMATTER = ["%lf %lf %lf", "%lf %lf %lf"]\n'
+        "POSITION = [$1, $2, $3, $1, $2, $3]\nCOORD_CONVERT = T\n{cosmo}"
+        "BINNING_SCHEME = 1\nPAIR_COUNT = [DD, DR]\nPAIR_COUNT_FILE = [{d}/DD.bin, {d}/DR.bin]\n"
+        "SEP_BIN_MIN = 0\nSEP_BIN_MAX = 100\nSEP_BIN_SIZE = 5\nMU_BIN_NUM = 10\n"
+        "OUTPUT_FORMAT = 0\nOVERWRITE = 2\n")
+CLI_CASES = [
+    # name, code, catalogue kind, template, outputs
+    ("box_smu_cf", "FCFC_2PT_BOX", "box_plain", SMU_CF,
+     ["DD.txt", "DR.txt", "RR.txt", "xi_ls.txt", "xi_nat.txt", "mp_ls.txt", "mp_nat.txt"]),
+    ("box_spi_wp", "FCFC_2PT_BOX", "box_plain", SPI_WP, ["DD.bin", "DR.bin", "RR.bin", "xi.txt", "wp.txt"]),
+    ("box_expr", "FCFC_2PT_BOX", "box_expr", EXPR, ["DD.bin"]),
+    ("survey_convert_integral", "FCFC_2PT", "survey_radec",
+     CONV.replace("{cosmo}", "OMEGA_M = 0.31\nDE_EOS_W = -0.9\nCMVDST_ERR = 1e-10\n"), ["DD.bin", "DR.bin"]),
+    ("survey_convert_table", "FCFC_2PT", "survey_radec",
+     CONV.replace("{cosmo}", "Z_CMVDST_FILE = {d}/zdist.txt\n"), ["DD.bin", "DR.bin"]),
+]
+
+
+def write_cli_inputs(kind, d):
+    """Catalogue files (and the distance table) of a CLI case into d."""
+    for k, cols in enumerate(cli_catalogs(kind)):
+        O.write_ascii(os.path.join(d, f"cat{k}.txt"), cols)
+    z, dist = eds_table()
+    O.write_ascii(os.path.join(d, "zdist.txt"), [z, dist])
+
+
+def read_output(path):
+    """A pair-count file (binary or ASCII) or a whitespace table."""
+    with open(path, "rb") as f:
+        head = f.readline().decode(errors="replace")
+    if "format = 0" in head:
+        r = O.read_pair_count(path)
+        return {"kind": "pc", "spec": r["spec"].tolist(), "norm": float(r["norm"]),
+                "num": r["num"].tolist(), "ncnt": r["ncnt"].tolist(), "cnt": r["cnt"].tolist()}
+    lines = open(path).read().splitlines()
+    body = [l for l in lines if l and not l.startswith("#")]
+    return {"kind": "table", "header": [l for l in lines if l.startswith("#")], "text": body}
+
+
+def run_cli_case(name, code, kind, template, outs):
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp(prefix="fcfc_cli_")
+    write_cli_inputs(kind, d)
+    conf = os.path.join(d, "fcfc.conf")
+    with open(conf, "w") as f:
+        f.write(template.replace("{d}", d))
+    exe = O.ref_binary(code, "avx512")
+    r = subprocess.run([exe, "-c", conf], cwd=d, capture_output=True, text=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="8"))
+    if r.returncode:
+        raise RuntimeError(f"{name}: reference failed\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    res = {o: read_output(os.path.join(d, o)) for o in outs}
+    shutil.rmtree(d)
+    return {"name": name, "code": code, "catalogs": kind, "template": template, "outputs": res,
+            "checksums": [checksum(*c) for c in cli_catalogs(kind)]}
+
+
 def main():
+    if "--cli" in sys.argv:
+        for case in CLI_CASES:
+            print("cli case", case[0], flush=True)
+            data = run_cli_case(*case)
+            with open(os.path.join(HERE, "cli_" + case[0] + ".json"), "w") as fp:
+                json.dump(data, fp, separators=(",", ":"))
+        return
     if "--sizes" in sys.argv:
         for f in SIZE_CASES:
             print("case", f[0], flush=True)

@@ -91,9 +91,155 @@ def test_survey_cli_matches_reference(tmp_path):
         np.testing.assert_allclose(mine["cnt"], np.array(ref["cnt"]), rtol=1e-12)
 
 
-def test_cli_rejects_unsupported(tmp_path):
-    r = subprocess.run([os.path.join(BIN, "FCFC_2PT_BOX_GPU"), "-i", "x.txt", "-f", "%lf %lf %lf",
-                        "-x", "[${1}*2,${2},${3}]", "-b", "1000", "-p", "DD", "-P", "o.bin",
-                        "--s-min", "0", "--s-max", "200", "--s-step", "5"],
-                       capture_output=True, text=True, cwd=str(tmp_path), timeout=60)
-    assert r.returncode != 0
+
+
+# ---- the executables against the upstream executables' own output files ----
+import glob  # noqa: E402
+import math  # noqa: E402
+import shutil  # noqa: E402
+
+from tests.golden.make_golden import (checksum, cli_catalogs, read_output,  # noqa: E402
+                                      write_cli_inputs)
+
+CLI_CASES = sorted(glob.glob(os.path.join(GOLD, "cli_*.json")))
+
+
+def _rows(text):
+    return [[float(v) for v in l.split()] for l in text]
+
+
+def _cmp_rows(mine, ref, rtol, what):
+    """numeric tables printed with %.10lg: equal within rtol, inf and nan in
+    the same places"""
+    assert len(mine) == len(ref), f"{what}: {len(mine)} rows, expected {len(ref)}"
+    for a, b in zip(_rows(mine), _rows(ref)):
+        assert len(a) == len(b), what
+        for u, v in zip(a, b):
+            if math.isnan(v) or math.isinf(v):
+                assert (math.isnan(u) and math.isnan(v)) or u == v, f"{what}: {u} vs {v}"
+            else:
+                assert u == pytest.approx(v, rel=rtol, abs=1e-300), f"{what}: {u} vs {v}"
+
+
+def _run(tmp, code, conf_text, *extra):
+    conf = os.path.join(tmp, "fcfc.conf")
+    with open(conf, "w") as f:
+        f.write(conf_text)
+    return subprocess.run([os.path.join(BIN, code + "_GPU"), "-c", conf, *extra],
+                          capture_output=True, text=True, cwd=tmp, timeout=300)
+
+
+def _check_outputs(tmp, outputs):
+    for fname, ref in outputs.items():
+        mine = read_output(os.path.join(tmp, fname))
+        assert mine["kind"] == ref["kind"], fname
+        if ref["kind"] == "pc":
+            assert mine["spec"] == ref["spec"], fname
+            if ref["spec"][4] == 1:
+                # weight sums W (and the norm W(W-1) or W1 W2) depend on the
+                # summation order: the reference's is its OpenMP reduction
+                assert mine["norm"] == pytest.approx(ref["norm"], rel=1e-13), fname
+                np.testing.assert_allclose(mine["num"], ref["num"], rtol=1e-13, err_msg=fname)
+            else:
+                assert mine["norm"] == ref["norm"], fname
+                np.testing.assert_array_equal(mine["num"], ref["num"], err_msg=fname)
+            if ref["spec"][4] == 1:
+                np.testing.assert_allclose(mine["cnt"], ref["cnt"], rtol=1e-12, err_msg=fname)
+                np.testing.assert_allclose(mine["ncnt"], ref["ncnt"], rtol=1e-12, err_msg=fname)
+            else:
+                np.testing.assert_array_equal(mine["cnt"], ref["cnt"], err_msg=fname)
+                np.testing.assert_array_equal(mine["ncnt"], ref["ncnt"], err_msg=fname)
+        else:
+            assert mine["header"] == ref["header"], fname
+            if fname.endswith(".txt") and ref["header"] and "Created by" in ref["header"][0]:
+                assert mine["text"] == ref["text"], fname      # unweighted ASCII pair counts
+            else:
+                _cmp_rows(mine["text"], ref["text"], 2e-9, fname)
+
+
+@pytest.mark.parametrize("path", CLI_CASES, ids=[os.path.basename(p)[4:-5] for p in CLI_CASES])
+@pytest.mark.parametrize("prec", ["0", "1"])
+def test_cli_outputs_match_reference(tmp_path, path, prec):
+    """The upstream executable's output files for the same configuration:
+    pair-count files (binary and ASCII), CF estimators including the
+    analytic RR (@@), multipoles (unsorted, duplicated MULTIPOLE list), w_p,
+    POSITION / WEIGHT / SELECTION expressions, and COORD_CONVERT by
+    integration and by a (z, distance) table."""
+    g = json.load(open(path))
+    assert [checksum(*c) for c in cli_catalogs(g["catalogs"])] == g["checksums"]
+    tmp = str(tmp_path)
+    write_cli_inputs(g["catalogs"], tmp)
+    print(f"case cli {g['name']} gpu-prec={prec}", flush=True)
+    r = _run(tmp, g["code"], g["template"].replace("{d}", tmp), "--gpu-prec", prec)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _check_outputs(tmp, g["outputs"])
+
+
+def test_cli_resume_reference_ascii_files(tmp_path):
+    """OVERWRITE = 1 with the upstream executable's ASCII pair-count files in
+    place: their normalised counts are read (no catalogue is counted) and the
+    CFs and multipoles equal the upstream ones."""
+    g = json.load(open(os.path.join(GOLD, "cli_box_smu_cf.json")))
+    tmp = str(tmp_path)
+    write_cli_inputs(g["catalogs"], tmp)
+    for p in ("DD", "DR", "RR"):
+        ref = g["outputs"][p + ".txt"]
+        with open(os.path.join(tmp, p + ".txt"), "w") as f:
+            f.write("\n".join(ref["header"] + ref["text"]) + "\n")
+    conf = g["template"].replace("{d}", tmp).replace("OVERWRITE = 2", "OVERWRITE = 1")
+    r = _run(tmp, g["code"], conf)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Reading DD pairs" in r.stdout and "Counting" not in r.stdout
+    # the estimators of the file's (10-digit) normalised counts; @@ is the
+    # analytic RR of the box (eval_cf.c:216-242), 20 mu bins, s edges 0-150
+    n = {p: np.array(_rows(g["outputs"][p + ".txt"]["text"]))[:, 4] for p in ("DD", "DR", "RR")}
+    s = np.arange(0, 155, 5.0)
+    rr = np.tile(4 * np.pi / (3 * 20 * 1000.0 ** 3) * (s[1:] ** 3 - s[:-1] ** 3), 20)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        want = {"xi_ls.txt": (n["DD"] - 2 * n["DR"] + n["RR"]) / n["RR"], "xi_nat.txt": n["DD"] / rr - 1}
+    for f, w in want.items():
+        got = np.array(_rows(open(os.path.join(tmp, f)).read().splitlines()[1:]))[:, 4]
+        np.testing.assert_allclose(got, w, rtol=2e-9, equal_nan=True, err_msg=f)
+
+
+def test_cli_resume_reference_binary_and_mismatches(tmp_path):
+    """A binary pair-count file written by the upstream FCFC_2PT_BOX (DR,
+    (s, mu) 40 x 100, box 1000) is resumed: CF = 2 DR reproduces twice its
+    normalised counts.  Settings that differ from the file's -- mu bins,
+    separation bins, box size -- are refused, as read_res.c refuses them."""
+    tmp = str(tmp_path)
+    shutil.copy(os.path.join(GOLD, "ref_box_smu_DR.bin"), os.path.join(tmp, "DR.bin"))
+    ref = O.read_pair_count(os.path.join(tmp, "DR.bin"))
+    base = ("CATALOG = [{d}/none0.txt, {d}/none1.txt]\nCATALOG_LABEL = [D, R]\n"
+            'ASCII_FORMATTER = ["%lf %lf %lf", "%lf %lf %lf"]\n'
+            "POSITION = [$1, $2, $3, $1, $2, $3]\nBOX_SIZE = {box}\nBINNING_SCHEME = 1\n"
+            "PAIR_COUNT = DR\nPAIR_COUNT_FILE = {d}/DR.bin\nCF_ESTIMATOR = DR*2\n"
+            "CF_OUTPUT_FILE = {d}/xi.txt\nSEP_BIN_MIN = 0\nSEP_BIN_MAX = {smax}\n"
+            "SEP_BIN_SIZE = 5\nMU_BIN_NUM = {nmu}\nOVERWRITE = 1\n")
+    conf = base.format(d=tmp, box=1000, smax=200, nmu=100)
+    r = _run(tmp, "FCFC_2PT_BOX", conf)
+    assert r.returncode == 0, r.stdout + r.stderr
+    xi = np.array(_rows(open(os.path.join(tmp, "xi.txt")).read().splitlines()[1:]))
+    np.testing.assert_allclose(xi[:, 4], 2 * ref["ncnt"], rtol=1e-9)
+    for kw, msg in ((dict(box=1000, smax=200, nmu=50), "mu bins"),
+                    (dict(box=1000, smax=150, nmu=100), "separation bins"),
+                    (dict(box=900, smax=200, nmu=100), "box size")):
+        if os.path.exists(os.path.join(tmp, "xi.txt")):
+            os.remove(os.path.join(tmp, "xi.txt"))
+        r = _run(tmp, "FCFC_2PT_BOX", base.format(d=tmp, **kw))
+        assert r.returncode != 0 and msg in r.stderr, r.stdout + r.stderr
+        assert not os.path.exists(os.path.join(tmp, "xi.txt"))
+
+
+def test_cli_refuses_unsupported(tmp_path):
+    """Bitwise operators and string columns, FITS/HDF5 catalogues and the
+    conversion in the box executable end with an error."""
+    tmp = str(tmp_path)
+    common = ["-f", "%lf %lf %lf", "-b", "1000", "-p", "DD", "-P", "o.bin", "--s-min", "0",
+              "--s-max", "200", "--s-step", "5"]
+    for extra in (["-i", "x.txt", "-x", "[${1}&1,${2},${3}]"],
+                  ["-i", "x.txt", "-x", "[$1,$2,$3]", "-T", "1"],
+                  ["-i", "x.txt", "-x", "[$1,$2,$3]", "--convert", "T"]):
+        r = subprocess.run([os.path.join(BIN, "FCFC_2PT_BOX_GPU")] + extra + common,
+                           capture_output=True, text=True, cwd=tmp, timeout=60)
+        assert r.returncode != 0, extra

@@ -21,9 +21,10 @@ from tests.golden.make_golden import catalogs, checksum
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # cfg_*: the cases at the configurations' own sizes (tests/test_gpu_sizes.py
-# and the size tests below)
+# and the size tests below); cli_*: output files of the executables
+# (tests/test_gpu_cli.py)
 CASES = sorted(c for c in glob.glob(os.path.join(GOLD, "*.json"))
-               if not os.path.basename(c).startswith("cfg_"))
+               if not os.path.basename(c).startswith(("cfg_", "cli_")))
 
 
 def _bins(code, bk, L=1000.0):
