@@ -264,6 +264,8 @@ struct FkWin {
  * above it, the last one without flags). */
 #define FK_F_EDGE 0x10000u    /* a bin edge at the lower end of the integer cell */
 #define FK_F_ALW  0x20000u    /* always re-evaluate (with FK_F_EDGE set too)     */
+#define FK_F_ALWAYS 0x4000u   /* FP32 threshold (high half) of FK_F_ALW cells: 2 */
+#define FK_F_NEVER  0xBF80u   /* ... of cells without flags: -1                  */
 
 /* One counting job: a catalogue pair and its histogram.  A launch counts the
  * tiles of one or many jobs (batched mocks); tiles carry their job index. */
@@ -391,7 +393,8 @@ __device__ __forceinline__ int fk_hslot(int i, int ns, int nsh) {
  * check is one min.  Hybrid tables (pointer at cell 0) refine downwards
  * against the edges (FCFC_LOOKUP_HYBRID, metric_common.c:56-64): the result
  * is the last edge at or below s2 (bin 0 below the first edge).  Tables too
- * long for LDS are replaced by a binary search for that same edge. */
+ * long for LDS are replaced by a binary search for that same edge.  MK: the
+ * table is an FP32 one, whose cells hold the bin in their low half. */
 __device__ __forceinline__ unsigned fk_edge_search(const double *e, int n, double v) {
   int lo = 0, hi = n - 1;                 /* last i in [0, n) with e[i] <= v, else 0 */
   while (lo < hi) {
@@ -600,9 +603,14 @@ __device__ unsigned fk_box_variant(int var, const double a[3], const double bo[3
  * at (int) u is the cell of s2 when s2 is clear of the integers, and the
  * cell above the nearby integer otherwise, whose FK_F_EDGE flag says
  * whether that integer is a bin edge (if not, both cells give the same
- * bin).  Flags sit above the bin bits, ordered so the test is one compare:
- * entry >= (near ? FK_F_EDGE : FK_F_ALW).  A pair whose s (or pi) is
- * certainly out of range reads a flagless sentinel. */
+ * bin).  A cell of the FP32 tables holds the bin in its low 16 bits and in
+ * its high 16 bits those of a float threshold: -1 without flags, the window
+ * 2e rounded up (the largest of the launch's jobs) with FK_F_EDGE, 2 with
+ * FK_F_ALW.  Read as a float, the cell lies within 2^-7 relative of its
+ * threshold, on the side away from zero (thresholds are normal numbers:
+ * comparisons with subnormals are slow), so the test is one compare of the
+ * cell itself: fract(u) < cell.  A pair whose s (or pi) is certainly out of
+ * range reads a flagless sentinel. */
 template <int BIN>
 __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     const FkBins &bn, const FkWin &wn, const unsigned *stab,
@@ -617,13 +625,13 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
    * (as are the entries below s2min) */
   const float u = s2 + wn.e_s2;
   const unsigned e = stab[min((int) u, bn.scap32)];
-  const bool amb = e >= (__builtin_amdgcn_fractf(u) < wn.e_s2x2 ? FK_F_EDGE : FK_F_ALW);
+  const bool amb = __builtin_amdgcn_fractf(u) < __uint_as_float(e);
   const unsigned sb = e & 0xFFFFu;
   if (BIN == FCFC_BIN_ISO) { ambig = amb; return sb; }
   if (BIN == FCFC_BIN_SPI) {
     const float up = __builtin_fabsf(dz) + wn.e_pi;
     const unsigned ep = ptab[min((int) up, bn.pcap32)];
-    const bool ambp = ep >= (__builtin_amdgcn_fractf(up) < wn.e_pix2 ? FK_F_EDGE : FK_F_ALW);
+    const bool ambp = __builtin_amdgcn_fractf(up) < __uint_as_float(ep);
     const unsigned pb = ep & 0xFFFFu;
     const bool out = ((sb == FK_OUT) & !amb) | ((pb == FK_OUT) & !ambp);
     ambig = (amb | ambp) & !out;
@@ -775,12 +783,22 @@ k_count(const FkArgs a) {
         f |= FK_F_ALW | FK_F_EDGE;
       return v | f;
     };
+    /* FP32 tables: bin and the high half of the threshold of fract(u),
+     * the window rounded up */
+    auto put = [&](unsigned *raw, int i, unsigned v, float win2) {
+      if (!F32) { raw[i] = v; return; }
+      const unsigned th = (v & FK_F_ALW) ? FK_F_ALWAYS
+                          : ((v & FK_F_EDGE) ? (__float_as_uint(win2) + 0xFFFFu) >> 16 : FK_F_NEVER);
+      raw[i] = (th << 16) | (v & 0xFFFFu);
+    };
     const double s2f = (F32 && BIN == FCFC_BIN_SMU) ? (double) a.jobs[0].win.s2_floor : 0.0;
     for (int i = tid; i < a.stab_c0 + bn.stab_len + 2; i += FK_NT)
-      stab_raw[i] = entry(bn.stab, bn.stab_len, bn.ns, i - a.stab_c0, bn.soff, s2f, bn.s2min);
+      put(stab_raw, i, entry(bn.stab, bn.stab_len, bn.ns, i - a.stab_c0, bn.soff, s2f, bn.s2min),
+          bn.win2s);
     if (BIN == FCFC_BIN_SPI)
       for (int i = tid; i < a.ptab_c0 + bn.ptab_len + 2; i += FK_NT)
-        ptab_raw[i] = entry(bn.ptab, bn.ptab_len, bn.np, i - a.ptab_c0, bn.poff, 0.0, bn.pmin);
+        put(ptab_raw, i, entry(bn.ptab, bn.ptab_len, bn.np, i - a.ptab_c0, bn.poff, 0.0, bn.pmin),
+            bn.win2p);
   }
   if (HYB) {
     for (int i = tid; i <= bn.ns; i += FK_NT) s2e[i] = bn.s2bin[i];
@@ -957,6 +975,7 @@ k_count(const FkArgs a) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
           unsigned idx;
+          bool queued = false;      /* FP32: left to the FP64 re-evaluation */
           if (F32) {
             bool amb, ambm;
             idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw, ptab_raw,
@@ -970,7 +989,6 @@ k_count(const FkArgs a) {
              * own mask) and the lane's bit is read back from the mask inside
              * the rare branch, so no lane value is kept across it */
             const unsigned long long m = __builtin_amdgcn_ballot_w64(amb) | __builtin_amdgcn_ballot_w64(ambm);
-            idx = (amb | ambm) ? FK_NONE : idx;
             if (m) {
               if ((m >> lane) & 1ull) {
                 const int pos = qn + (int) __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32),
@@ -980,6 +998,7 @@ k_count(const FkArgs a) {
               qn += __popcll(m);
               if (qn > FK_QCAP - 64) { drain(qn); qn = 0; }
             }
+            queued = amb | ambm;
           }
           else if (GEOM == FCFC_GEOM_BOX)
             idx = fk_box64<BIN, HYB, false, PLAIN>(bx - ax[r], by - ay[r], bz - az[r], bn, stab,
@@ -988,7 +1007,7 @@ k_count(const FkArgs a) {
             idx = fk_sv64<BIN, HYB, PLAIN>(ax[r], ay[r], az[r], as[r], bx, by, bz, bs, bn,
                                            stab, ptab, s2e, pe);
           if (!F32 && SELF && !(jj > ai[r])) idx = FK_NONE;
-          if (idx < ntot) {
+          if (idx < ntot && !queued) {
             if (GRAD) {
               const double gk = hist_d[idx];
               gacc[r] += gk * wj;
@@ -2065,6 +2084,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   hstr.reserve(nstr);
   std::vector<FkJob> hj(jobs.size());
   double s2f = 0;
+  float win2s = 0.0f, win2p = 0.0f;
   if (m == MODE_C32)
     for (const PJob &J : jobs) s2f = std::max(s2f, fp32_s2_floor(b, J.P.g, J.P.zmaxc));
   for (size_t k = 0; k < jobs.size(); k++) {
@@ -2081,7 +2101,11 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     f.hist_off = J.hist_off;
     f.gA = J.gA;
     f.gB = J.gB;
-    if (m == MODE_C32) fp32_windows(b, J.P.g, J.P.zmaxc, s2f, &f.win);
+    if (m == MODE_C32) {
+      fp32_windows(b, J.P.g, J.P.zmaxc, s2f, &f.win);
+      win2s = std::max(win2s, f.win.e_s2x2);
+      win2p = std::max(win2p, f.win.e_pix2);
+    }
     hstr.insert(hstr.end(), J.P.strips.begin(), J.P.strips.end());
   }
   djobs = (FkJob *) al.get(WS_JOBS, sizeof(FkJob) * jobs.size());
@@ -2122,6 +2146,8 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
       : fb.s2max * (1 + ldexp(1.0, -30));
   fb.pskip = (b->bintype == FCFC_BIN_SPI) ? fb.pmax * (1 + ldexp(1.0, -30)) : 1e300;
   fb.s2skip32 = (float) (fb.s2max + 2.0);
+  fb.win2s = win2s;
+  fb.win2p = win2p;
   fb.pskip32 = (b->bintype == FCFC_BIN_SPI) ? (float) (fb.pmax + 2.0) : 3.0e38f;
   fb.mu_sh = ldexp((double) b->nmu, -20);
   fb.mu_hi = 1.0 - 2.0 * fb.mu_sh;

@@ -57,6 +57,8 @@ struct FkBins {
   double s2_exact;           /* ... or s^2 is below this value               */
   double s2skip, pskip;      /* box: a pair at s^2 >= s2skip or pi >= pskip  */
   float s2skip32, pskip32;   /* is out of range (FP64 / FP32 values)         */
+  float win2s, win2p;        /* FP32 tables: window 2 e of the edge cells,   */
+                             /* the largest over the jobs of a launch        */
   const uint16_t *stab;      /* device tables (global), staged to LDS       */
   const uint16_t *ptab;
   const double *s2bin;       /* edges for the hybrid refinement              */
