@@ -335,10 +335,10 @@ def survey_workload(args):
     torch.cuda.synchronize()
     log(f"survey workload ready in {time.time() - t0:.1f} s: D={D.n} R={R.n}")
     eng = F.Engine(0)
-    prec = "fp64" if args.precision == "fp32" else args.precision   # survey: FP64 forms
+    prec = args.precision
+    others = [f for f in ("fp64", "fp32", "plain") if f != prec and f not in args.skip]
     if not args.no_autotune:
-        forms = [prec] + ([] if "plain" in args.skip or prec == "plain" else ["plain"])
-        for p in forms:
+        for p in [prec] + others:
             for name, a, c, au in (("DD", D, None, True), ("DR", D, R, False), ("RR", R, None, True)):
                 log(f"autotune {p} {name}: {eng.autotune(a, c, b, isauto=au, withwt=True, precision=p)}")
 
@@ -370,23 +370,25 @@ def survey_workload(args):
     line = {
         "metric": METRIC, "value": ev_tot / dt, "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": "survey shell r in [1000,2000] Mpc/h, 60x60 deg cap, FKP weights; "
                                "D=1e6, R=2e7; weighted DD+DR+RR (s_perp,pi) 40x40 to 200 + w_p",
                    "scale": args.scale, "n_data": D.n, "n_random": R.n, "precision": prec},
         "kernel_ms": km, "evals_per_step": ev_tot / args.steps,
         "wp_first_bins": [float(v) for v in wp(xi, b.pbin_raw)[:5]],
     }
-    if "plain" not in args.skip and prec != "plain":
+    line["forms"] = {}
+    for other in others:
         t2 = time.time()
-        rp, evp, kmp = step("plain")
+        rp, evp, kmp = step(other)
         torch.cuda.synchronize()
         dtp = time.time() - t2
-        line["forms"] = {"plain": {"evals_per_s": evp / dtp, "kernel_ms": kmp,
-                                   "weighted_max_rel": max(float(np.max(np.abs(res[k] - rp[k]) /
-                                                           np.maximum(np.abs(rp[k]), 1e-300)))
-                                                           for k in res)}}
+        line["forms"][other] = {"evals_per_s": evp / dtp, "ms_per_step": dtp * 1e3, "kernel_ms": kmp,
+                                "weighted_max_rel": max(float(np.max(np.abs(res[k] - rp[k]) /
+                                                        np.maximum(np.abs(rp[k]), 1e-300)))
+                                                        for k in res)}
+        log(f"survey {other} step done")
     eng.close()
     print(json.dumps(line), flush=True)
 

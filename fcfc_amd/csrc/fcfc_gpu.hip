@@ -257,11 +257,11 @@ struct FkWin {
   float mu_sh, mu_hi;         /* mu bin certain if fract(t - mu_sh) < mu_hi */
   float s2_floor;             /* pairs below this s^2 are re-evaluated      */
   float nmu_f;
-};
+  float m2_floor;             /* survey: pairs with |a + b|^2 below this    */
+};                            /* are re-evaluated                          */
 
-/* FP32 tables: entry = bin (low 16 bits) | flags; index (int) s2 - soff + 1
- * (entry 0: the integer cell below the table, entries len + 1 and len + 2:
- * above it, the last one without flags). */
+/* Flags of the integer cells of the FP32 tables, set while they are staged
+ * (the LDS cells then carry a float threshold in place of the flags). */
 #define FK_F_EDGE 0x10000u    /* a bin edge at the lower end of the integer cell */
 #define FK_F_ALW  0x20000u    /* always re-evaluate (with FK_F_EDGE set too)     */
 #define FK_F_ALWAYS 0x4000u   /* FP32 threshold (high half) of FK_F_ALW cells: 2 */
@@ -513,21 +513,21 @@ __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
  * t = 2 a.b, s = |a|^2 + |b|^2, d = |a|^2 - |b|^2, pi^2 = d^2 / (t + s),
  * s^2 = s - t, s_perp^2 = (s - t) - pi^2.  (s, mu) bins need pi^2 only in
  * the exact branch of the mu bin: sqrt(pi^2) = |d| / sqrt(t + s). */
-template <int BIN, bool HYB, bool PLAIN = false>
+template <int BIN, bool HYB, bool PLAIN = false, bool MK = false>
 __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
     double as, double bx, double by, double bz, double bs, const FkBins &bn,
     const unsigned *stab, const unsigned *ptab, const double *s2e, const double *pe) {
   if (BIN == FCFC_BIN_ISO) {
     const double dx = ax - bx, dy = ay - by, dz = az - bz;
     const double s2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-    return fk_sbin<HYB>(s2, bn, stab, s2e);
+    return fk_sbin<HYB, MK>(s2, bn, stab, s2e);
   }
   const double t = __builtin_fma(az, bz, __builtin_fma(ay, by, ax * bx)) * 2.0;
   const double s = as + bs;
   const double d = as - bs;
   if (BIN == FCFC_BIN_SMU) {
     const double s2 = s - t;
-    const unsigned sb = fk_sbin<HYB>(s2, bn, stab, s2e);
+    const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
     if (PLAIN) return sb + __umul24((unsigned) bn.nsh, fk_mu_plain((d * d) / (t + s), s2, bn));
     const double tm = __builtin_fma(__builtin_fabs(d) * bn.nmu_d,
         __builtin_amdgcn_rsq(t + s) * __builtin_amdgcn_rsq(s2), -bn.mu_sh);
@@ -539,8 +539,8 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
    * no lower bound on s_perp^2 (a rounding-negative value lands in bin 0) */
   const double p2 = (d * d) / (t + s);
   const double sp2 = (s - t) - p2;
-  const unsigned sb = fk_sbin<HYB>(sp2, bn, stab, s2e);
-  const unsigned pb = fk_pbin<HYB>(p2, bn, ptab, pe);
+  const unsigned sb = fk_sbin<HYB, MK>(sp2, bn, stab, s2e);
+  const unsigned pb = fk_pbin<HYB, MK>(p2, bn, ptab, pe);
   return sb + __umul24((unsigned) bn.nsh, pb);
 }
 
@@ -651,6 +651,57 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
   return sb + __umul24((unsigned) bn.nsh, (unsigned) (int) t);
 }
 
+/* FP32 survey pair.  d = b - a from coordinates relative to the A tile
+ * centre (as for box pairs) and m = a + b = 2 a + d (m* = 2 a in FP32), so
+ * that s^2 = |d|^2, pi = d.m / |m| and mu = pi / s carry the FP32 error of
+ * differences, not the cancellation of |a|^2 + |b|^2 - 2 a.b that the
+ * reference's FP64 formulas have (metric_common.c:338-364): its values
+ * differ from the exact ones by its own rounding, which the windows include
+ * (fp32_windows).  Pairs with |m|^2 below m2_floor (both objects near the
+ * observer, on opposite sides) are always re-evaluated, as are pairs of
+ * cells flagged for it (the three conditions come back apart, so that each
+ * ballot is its compare's own mask).  (s_perp, pi): the tables hold pi^2 and s_perp^2 =
+ * s^2 - pi^2, whose FP32 value may come out slightly negative (index 0, as
+ * the reference's truncation gives); lanes whose indices come out below -1
+ * (pairs below m2_floor) read the "out" sentinel. */
+template <int BIN>
+__device__ __forceinline__ unsigned fk_sv32(float dx, float dy, float dz, float mx, float my,
+    float mz, const FkBins &bn, const FkWin &wn, const unsigned *stab, const unsigned *ptab,
+    bool &ambig, bool &ambig_mu, bool &ambig_m) {
+  ambig_mu = false;
+  ambig_m = false;
+  const float s2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+  if (BIN == FCFC_BIN_ISO) {
+    const float u = s2 + wn.e_s2;
+    const unsigned e = stab[min((int) u, bn.scap32)];
+    ambig = __builtin_amdgcn_fractf(u) < __uint_as_float(e);
+    return e & 0xFFFFu;
+  }
+  const float p = __builtin_fmaf(dz, mz, __builtin_fmaf(dy, my, dx * mx));
+  const float m2 = __builtin_fmaf(mz, mz, __builtin_fmaf(my, my, mx * mx));
+  ambig_m = m2 < wn.m2_floor;          /* near the observer: re-evaluate */
+  if (BIN == FCFC_BIN_SPI) {
+    const float p2 = (p * p) * __builtin_amdgcn_rcpf(m2);
+    const float sp2 = s2 - p2;
+    const float u = sp2 + wn.e_s2;
+    const unsigned e = stab[min((unsigned) (int) u, (unsigned) bn.scap32)];
+    const bool amb = __builtin_amdgcn_fractf(u) < __uint_as_float(e);
+    const float up = p2 + wn.e_pi;
+    const unsigned ep = ptab[min((unsigned) (int) up, (unsigned) bn.pcap32)];
+    const bool ambp = __builtin_amdgcn_fractf(up) < __uint_as_float(ep);
+    ambig = amb;
+    ambig_mu = ambp;
+    return (e & 0xFFFFu) + __umul24((unsigned) bn.nsh, ep & 0xFFFFu);
+  }
+  const float u = s2 + wn.e_s2;
+  const unsigned e = stab[min((int) u, bn.scap32)];
+  ambig = __builtin_amdgcn_fractf(u) < __uint_as_float(e);
+  const float t = __builtin_fmaf(wn.nmu_f * __builtin_fabsf(p),
+      __builtin_amdgcn_rsqf(m2) * __builtin_amdgcn_rsqf(s2), -wn.mu_sh);
+  ambig_mu = !(__builtin_amdgcn_fractf(t) < wn.mu_hi);
+  return (e & 0xFFFFu) + __umul24((unsigned) bn.nsh, (unsigned) (int) t);
+}
+
 /* Exact FP64 re-evaluation of a pair flagged by the FP32 windows (rare: the
  * pairs are queued and re-evaluated together after each chunk).  The tables
  * are the flagged FP32 ones, read through their bin bits.  With census
@@ -688,6 +739,18 @@ __device__ __forceinline__ unsigned fk_recheck(const FkBins &bn, const FkJob &J,
   return idx;
 }
 
+/* The same for a survey pair: A object p, B object jg, with the reference's
+ * FP64 formulas (fk_sv64). */
+template <int BIN, bool HYB>
+__device__ __forceinline__ unsigned fk_recheck_sv(const FkBins &bn, const FkJob &J, int p,
+    int jg, const unsigned *stab, const unsigned *ptab, const double *s2e, const double *pe) {
+  const double as = (BIN == FCFC_BIN_ISO) ? 0.0 : J.A.s[p];
+  const double bs = (BIN == FCFC_BIN_ISO) ? 0.0 : J.B.s[jg];
+  unsigned idx = fk_sv64<BIN, HYB, false, true>(J.A.x[p], J.A.y[p], J.A.z[p], as, J.B.x[jg],
+                                                J.B.y[jg], J.B.z[jg], bs, bn, stab, ptab, s2e, pe);
+  return idx >= (unsigned) bn.ntoh ? FK_NONE : idx;
+}
+
 /* ---------------------------- the kernel --------------------------------- */
 
 /* One persistent kernel for counts (FP64 or FP32 + recheck) and for the
@@ -705,7 +768,10 @@ k_count(const FkArgs a) {
   constexpr bool CENS = (MODE == MODE_C32C);
   constexpr bool PLAIN = (MODE == MODE_P64);
   constexpr bool GRAD = (MODE == MODE_GRAD);
-  constexpr bool NEED_S = (GEOM == FCFC_GEOM_SURVEY && BIN != FCFC_BIN_ISO);
+  constexpr bool SV32 = (GEOM == FCFC_GEOM_SURVEY && (MODE == MODE_C32 || MODE == MODE_C32C));
+  /* |r|^2 of the objects: survey (s, mu) and (s_perp, pi) in FP64 (the FP32
+   * form reads them from the catalogue when it re-evaluates a pair) */
+  constexpr bool NEED_S = (GEOM == FCFC_GEOM_SURVEY && BIN != FCFC_BIN_ISO && !SV32);
   constexpr bool NEED_W = WT || GRAD;
   constexpr bool NEED_C = F32;
   const int tid = threadIdx.x;
@@ -888,6 +954,13 @@ k_count(const FkArgs a) {
       fay[r] = v ? (float) (ay[r] - tcy) : 0.0f;
       faz[r] = v ? (float) (az[r] - tcz) : J.win.idle_z;
     }
+    /* survey FP32: m = a + b = 2 a' + d + 2 c (a' = a - c, c = tile centre).
+     * Idle lanes (a' = (10^6, 0, 0): s^2 and pi^2 far out of range) may
+     * look ambiguous in mu: their queue bits are masked off */
+    const float c2x = (float) (2.0 * tcx), c2y = (float) (2.0 * tcy), c2z = (float) (2.0 * tcz);
+    unsigned long long actm[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) actm[r] = __builtin_amdgcn_ballot_w64(t.p0 + tid * R + r < t.p1);
     const int nva = min(t.p1 - t.p0, R * FK_NT);
     const bool jwt = WT && J.wt;
 
@@ -902,9 +975,12 @@ k_count(const FkArgs a) {
          * everything) are dropped */
         if (e0 + lane < n && p < t.p1) {
           const int j = (int) (v & 255u);
-          const double qx = J.A.x[p], qy = J.A.y[p], qz = J.A.z[p];
-          const unsigned idx = fk_recheck<BIN, HYB, CENS>(bn, J, qx, qy, qz, cj[j], cc[j],
-                                                          stab, ptab, s2e, pe, &ncensus);
+          unsigned idx;
+          if (GEOM == FCFC_GEOM_SURVEY)
+            idx = fk_recheck_sv<BIN, HYB>(bn, J, p, cj[j], stab, ptab, s2e, pe);
+          else
+            idx = fk_recheck<BIN, HYB, CENS>(bn, J, J.A.x[p], J.A.y[p], J.A.z[p], cj[j], cc[j],
+                                             stab, ptab, s2e, pe, &ncensus);
           nrecheck++;
           if (idx < ntot) {
             if (WT && jwt) atomicAdd(myhist_d + idx, J.A.w[p] * cw[j]);
@@ -960,7 +1036,11 @@ k_count(const FkArgs a) {
           bool any = false;
 #pragma unroll
           for (int r = 0; r < R; r++) {
-            if (BIN == FCFC_BIN_ISO) {
+            if (SV32) {
+              const float ex = gx - fax[r], ey = gy - fay[r], ez = gz - faz[r];
+              any |= __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) < bn.s2skip32;
+            }
+            else if (BIN == FCFC_BIN_ISO) {
               const double ex = ax[r] - bx, ey = ay[r] - by, ez = az[r] - bz;
               any |= __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex)) < bn.s2skip;
             }
@@ -978,8 +1058,17 @@ k_count(const FkArgs a) {
           bool queued = false;      /* FP32: left to the FP64 re-evaluation */
           if (F32) {
             bool amb, ambm;
-            idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw, ptab_raw,
-                                amb, ambm);
+            bool ambx = false;
+            if (SV32) {
+              const float ex = gx - fax[r], ey = gy - fay[r], ez = gz - faz[r];
+              idx = fk_sv32<BIN>(ex, ey, ez, __builtin_fmaf(2.0f, fax[r], ex) + c2x,
+                                 __builtin_fmaf(2.0f, fay[r], ey) + c2y,
+                                 __builtin_fmaf(2.0f, faz[r], ez) + c2z, bn, J.win, stab_raw,
+                                 ptab_raw, amb, ambm, ambx);
+            }
+            else
+              idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw,
+                                  ptab_raw, amb, ambm);
             if (SELF) {
               const bool keep = jj > ai[r];
               amb = amb & keep; ambm = ambm & keep; idx = keep ? idx : FK_NONE;
@@ -988,7 +1077,10 @@ k_count(const FkArgs a) {
              * two conditions are balloted apart (each ballot is its compare's
              * own mask) and the lane's bit is read back from the mask inside
              * the rare branch, so no lane value is kept across it */
-            const unsigned long long m = __builtin_amdgcn_ballot_w64(amb) | __builtin_amdgcn_ballot_w64(ambm);
+            if (SELF) ambx = ambx & (jj > ai[r]);
+            unsigned long long m = __builtin_amdgcn_ballot_w64(amb) |
+                __builtin_amdgcn_ballot_w64(ambm) | __builtin_amdgcn_ballot_w64(ambx);
+            if (SV32) m &= actm[r];
             if (m) {
               if ((m >> lane) & 1ull) {
                 const int pos = qn + (int) __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32),
@@ -998,7 +1090,7 @@ k_count(const FkArgs a) {
               qn += __popcll(m);
               if (qn > FK_QCAP - 64) { drain(qn); qn = 0; }
             }
-            queued = amb | ambm;
+            queued = amb | ambm | ambx;
           }
           else if (GEOM == FCFC_GEOM_BOX)
             idx = fk_box64<BIN, HYB, false, PLAIN>(bx - ax[r], by - ay[r], bz - az[r], bn, stab,
@@ -1717,7 +1809,7 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   L.off_chunk = o;
   /* per staged B object: coordinates, |r|^2 (survey), weight, index, image code */
   int per = (mode == MODE_C32) ? 12 : 24;
-  if (need_s) per += 8;
+  if (need_s && mode != MODE_C32) per += 8;
   if (wt || mode == MODE_GRAD) per += 8;
   if (mode == MODE_GRAD) per += 8;             /* gradient share of each B object */
   per += 4;
@@ -1757,9 +1849,9 @@ static kfn_t pick5(int hyb, int R) {
 template <int BIN, int GEOM>
 static kfn_t pick3(int wt, int mode, int hyb, int R) {
   if (mode == MODE_GRAD) return pick5<BIN, GEOM, true, MODE_GRAD>(hyb, R);
+  if (mode == MODE_C32)
+    return wt ? pick5<BIN, GEOM, true, MODE_C32>(hyb, R) : pick5<BIN, GEOM, false, MODE_C32>(hyb, R);
   if constexpr (GEOM == FCFC_GEOM_BOX) {
-    if (mode == MODE_C32)
-      return wt ? pick5<BIN, GEOM, true, MODE_C32>(hyb, R) : pick5<BIN, GEOM, false, MODE_C32>(hyb, R);
     if (mode == MODE_C32C)
       return wt ? pick5<BIN, GEOM, true, MODE_C32C>(hyb, R) : pick5<BIN, GEOM, false, MODE_C32C>(hyb, R);
   }
@@ -1789,7 +1881,23 @@ static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int R
  * nmu u E) / s; with s^2 >= s2_floor (fp32_s2_floor) and t <= nmu this is
  * a constant D, and the bin is certain when |fract(t32) - 1/2| <= 1/2 - D.
  * All bounds are inflated by 25%; pairs below s2_floor are always
- * re-evaluated. */
+ * re-evaluated.
+ *
+ * Survey pairs (fk_sv32) add two things.  The reference's own values carry
+ * the rounding of its FP64 formulas: s^2 = (|a|^2 + |b|^2) - 2 a.b is off
+ * the exact value by at most about 8 eps R^2 (R = largest |r| of the grid,
+ * eps = 2^-53, |r|^2 given with 2 eps), and its pi = (|a|^2 - |b|^2) / |m|
+ * by 5 eps R^2 / |m| (m = a + b).  And the FP32 m = (2 a' + d) + 2 c (c the
+ * tile centre, a' = a - c) has error |dm| <= 2u|c| + 2 sqrt(3) ek + 2u|m| +
+ * 2uE (ek = u (2 s + E) + tiny, 2|c| <= |m| + s + E), so p = d.m is off by
+ * at most |m| (sqrt(3) ek + 6 u s) + s (u (s + 3E) + 2 sqrt(3) ek) and pi
+ * by e_pi = sqrt(3) ek + 6 u s + s (u (s + 3E) + 2 sqrt(3) ek) / |m|.  With |m|^2 >= m2_floor (smaller |m| is re-evaluated) these are
+ * constants: the pi^2 window is 2 pi e_pi + e_pi^2 + pi^2 (relative error of
+ * |m|^2, 4u of the reciprocal, 2u of the products) plus the reference's
+ * terms, the s_perp^2 window the sum of the s^2 and pi^2 windows plus the
+ * rounding of the difference, and the mu window (t = nmu |p| / (|m| s))
+ * gets e_pi / s plus half the relative errors of s^2 and |m|^2 and 4u for
+ * each reciprocal square root. */
 static double fp32_extent(const FkGrid &g, int zmaxc) {
   return std::max(g.wx, std::max(g.wy, (zmaxc + 1) * g.wz));
 }
@@ -1820,28 +1928,70 @@ static double fp32_s2_floor(const fcfc_bins *b, const FkGrid &g, int zmaxc) {
   return best;
 }
 
+/* largest |r| of a survey grid (its corners) */
+static double grid_rmax(const FkGrid &g) {
+  double r2 = 0;
+  for (int k = 0; k < 8; k++) {
+    const double x = (k & 1) ? g.ox + g.ncx * g.wx : g.ox;
+    const double y = (k & 2) ? g.oy + g.ncy * g.wy : g.oy;
+    const double z = (k & 4) ? g.oz + g.ncz * g.wz : g.oz;
+    r2 = std::max(r2, x * x + y * y + z * z);
+  }
+  return sqrt(r2);
+}
+
 static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double s2f,
     FkWin *win) {
-  const double u = ldexp(1.0, -24);
+  const double u = ldexp(1.0, -24), eps = ldexp(1.0, -53);
+  const bool sv = (b->geometry == FCFC_GEOM_SURVEY);
   const double s2max = b->s2bin[b->ns];
+  /* box (s_perp, pi): pbin holds pi edges; survey: pi^2 edges */
   const double pmx = (b->bintype == FCFC_BIN_SPI) ? b->pbin[b->np] : 0;
-  const double S2 = s2max + pmx * pmx + 2.0;
+  const double S2 = s2max + (sv ? pmx : pmx * pmx) + 2.0;
   const double s = sqrt(S2);
   const double E = fp32_extent(g, zmaxc);
-  const double tiny = ldexp(1.0, -44) * std::max(g.Lx, std::max(g.Ly, g.Lz)) * 4;
+  const double R = sv ? grid_rmax(g) : 0.0;
+  const double tiny = ldexp(1.0, -44) * std::max(std::max(g.Lx, std::max(g.Ly, g.Lz)), R) * 4;
   const double ek = u * (2 * s + E) + tiny;
-  const double es2 = 7 * u * S2 + 2 * sqrt(3.0) * u * E * s + 3 * ek * ek + 6 * ek * tiny;
+  double es2 = 7 * u * S2 + 2 * sqrt(3.0) * u * E * s + 3 * ek * ek + 6 * ek * tiny;
+  double epi = u * (2 * pmx + E) + tiny;          /* box: window on pi */
+  const double m2f = sv ? S2 : 0.0;
+  const double mf = sqrt(m2f);
+  double dmu = 0;                                 /* survey mu window terms */
+  if (sv) {
+    es2 += 8 * eps * R * R + 4 * eps * S2;        /* the reference's s^2 */
+    const double epa = sqrt(3.0) * ek + 6 * u * s + s * (u * (s + 3 * E) + 2 * sqrt(3.0) * ek) / mf;
+    const double m2rel = u * (9 + 2 * (s + 3 * E) / mf) + 4 * sqrt(3.0) * ek / mf;
+    if (b->bintype == FCFC_BIN_SPI) {
+      const double pi = sqrt(pmx);
+      const double ep2 = 2 * pi * epa + epa * epa + pmx * (m2rel + 6 * u)
+                         + 2 * pi * 5 * eps * R * R / mf + pmx * 6 * eps * R * R / m2f + 4 * eps * pmx;
+      epi = ep2;                                  /* window on pi^2 */
+      es2 = es2 + ep2 + u * S2;                   /* window on s_perp^2 */
+    }
+    else {
+      const double sf = sqrt(std::max(s2f, 1e-300));
+      const double es2rel = 7 * u + 2 * sqrt(3.0) * u * E / sf + 3 * pow(u * (2 + E / sf), 2);
+      dmu = (sqrt(3.0) * u * (2 + E / sf) + sqrt(3.0) * tiny / sf + 5 * u + u * s / mf
+             + sqrt(3.0) * ek / mf) + 0.5 * es2rel + 0.5 * m2rel + 8 * u + 3 * u
+            + 5 * eps * R * R / (mf * sf) + 4 * eps * R * R / (sf * sf);
+    }
+  }
   FkWin &w = *win;
   w.e_s2 = (float) (es2 * 1.25 + 1e-7);
-  w.e_pi = (float) ((u * (2 * pmx + E) + tiny) * 1.25 + 1e-7);
+  w.e_pi = (float) (epi * 1.25 + 1e-7);
   w.e_s2x2 = 2.0f * w.e_s2;
   w.e_pix2 = 2.0f * w.e_pi;
   const double cz = 0.5 / std::max(b->nmu, 1);
-  w.idle_z = (float) (1e18 * cz);
-  w.idle_x = (float) (1e18 * sqrt(1.0 - cz * cz));
+  if (sv) { w.idle_x = 1.0e6f; w.idle_z = 0.0f; }
+  else {
+    w.idle_z = (float) (1e18 * cz);
+    w.idle_x = (float) (1e18 * sqrt(1.0 - cz * cz));
+  }
   const double nmu = b->nmu;
-  const double D = 1.25 * (nmu * 10 * u + (sqrt(3.0) * u * E * nmu + nmu * (u * E + tiny)) / sqrt(s2f))
-                   + 4 * nmu * u;
+  double D = 1.25 * (nmu * 10 * u + (sqrt(3.0) * u * E * nmu + nmu * (u * E + tiny)) / sqrt(s2f))
+             + 4 * nmu * u;
+  if (sv) D = 1.25 * nmu * dmu + 4 * nmu * u;
   /* mu_sh >= D; mu_hi <= 1 - 2 mu_sh (0 when the windows cover everything) */
   float sh = (float) std::min(D, 0.5);
   if ((double) sh < std::min(D, 0.5)) sh = nextafterf(sh, 1.0f);
@@ -1852,6 +2002,7 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double 
   w.mu_hi = hf;
   w.s2_floor = (float) s2f;
   w.nmu_f = (float) b->nmu;
+  w.m2_floor = (float) m2f;
 }
 
 /* One counting (or gradient) job on device-resident, sorted catalogues. */
@@ -1950,7 +2101,6 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     unsigned long long *dstats, unsigned int *dqueue, Launch *out) {
   const int geom = b->geometry;
   int m = mode;
-  if (m == MODE_C32 && geom != FCFC_GEOM_BOX) m = MODE_C64;    /* survey FP32: FP64 kernel */
   const bool zb = zb_fits(b);
   if (m == MODE_C32 && !zb) m = MODE_C64;                      /* FP32 tables are zero-based */
   int hyb = (b->tabtype || !zb) ? 1 : 0;
@@ -2000,7 +2150,9 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, (const void *) f, FK_NT, P.total));
     return 0;
   };
-  kfn_t fn = pick_kernel(b->bintype, geom, anywt, (m == MODE_C32 && c->census) ? MODE_C32C : m, hyb, R);
+  /* the rounding census compares the box arithmetic paths */
+  const int km = (m == MODE_C32 && c->census && geom == FCFC_GEOM_BOX) ? MODE_C32C : m;
+  kfn_t fn = pick_kernel(b->bintype, geom, anywt, km, hyb, R);
   /* histogram copies: as many as keep the occupancy of a single copy */
   LdsPlan L = lds_plan(b, m, anywt, need_s, 1, hyb, ssearch, psearch);
   if (L.total > lds_max) {
@@ -2145,7 +2297,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
       ? (fb.s2max + fb.pmax) * (1 + ldexp(1.0, -30)) + 1e-6 * (fb.s2max + fb.pmax + 1)
       : fb.s2max * (1 + ldexp(1.0, -30));
   fb.pskip = (b->bintype == FCFC_BIN_SPI) ? fb.pmax * (1 + ldexp(1.0, -30)) : 1e300;
-  fb.s2skip32 = (float) (fb.s2max + 2.0);
+  fb.s2skip32 = (float) (fb.s2max + (geom == FCFC_GEOM_SURVEY ? fb.pmax : 0.0) + 2.0);
   fb.win2s = win2s;
   fb.win2p = win2p;
   fb.pskip32 = (b->bintype == FCFC_BIN_SPI) ? (float) (fb.pmax + 2.0) : 3.0e38f;

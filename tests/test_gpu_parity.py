@@ -99,7 +99,7 @@ def test_box_auto(eng, cats, bt, prec):
 
 
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
-@pytest.mark.parametrize("prec", ["fp64", "plain"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "plain"])
 def test_survey(eng, bt, prec):
     x, y, z, w = F.gen_survey(21, 4000)
     xr, yr, zr, wr = F.gen_survey(22, 6000)
@@ -111,10 +111,34 @@ def test_survey(eng, bt, prec):
     refa, cen = O.count(1, None, (d.x, d.y, d.z, d.s, None), None, True, ob, variant=0,
                         census_mask=0b111111)
     ga = eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec)
+    st = eng.stats()
     _cmp(ga, refa, False, cen)
+    if prec == "fp32" and bt == F.BIN_SMU:
+        # the FP32 kernel ran: pairs below its s^2 floor are re-evaluated
+        assert st["rechecked"] > 0
     refx, _ = O.count(1, None, _otuple(d), _otuple(r), False, ob, variant=0, weighted=True)
     gx = eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec)
     _cmp(gx, refx, True)
+
+
+@pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
+def test_survey_near_observer(eng, bt):
+    """Objects around the observer: many pairs have both objects on opposite
+    sides of it (|a + b| small), where the reference's pi and mu lose their
+    accuracy and the FP32 form re-evaluates every pair in FP64."""
+    rng = np.random.default_rng(31)
+    n = 3000
+    v = rng.normal(size=(n, 3))
+    v *= (rng.uniform(0, 1, n) ** (1 / 3) * 120.0 / np.linalg.norm(v, axis=1))[:, None]
+    x, y, z = v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy()
+    b = _bins(F.GEOM_SURVEY, bt)
+    d = F.Catalog(b, x, y, z)
+    ob = _obins(b)
+    ref, cen = O.count(1, None, (d.x, d.y, d.z, d.s, None), None, True, ob, variant=0,
+                       census_mask=0b111111)
+    for prec in ("fp64", "fp32"):
+        print(f"case survey near observer bt={bt} prec={prec} census={cen}", flush=True)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), ref, False, cen)
 
 
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
