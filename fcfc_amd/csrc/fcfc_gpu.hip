@@ -419,7 +419,9 @@ __device__ __forceinline__ unsigned fk_sbin(double s2, const FkBins &bn,
     }
     return sb;
   }
-  const int k = min((int) s2, bn.soff + bn.stab_len);
+  /* unsigned: a value below -1 (a survey s_perp^2 from an infinite pi^2)
+   * reads the "out" sentinel, as for values above the range */
+  const unsigned k = min((unsigned) (int) s2, (unsigned) (bn.soff + bn.stab_len));
   return MK ? (stab[k] & 0xFFFFu) : stab[k];
 }
 
@@ -438,7 +440,7 @@ __device__ __forceinline__ unsigned fk_pbin(double p, const FkBins &bn,
     }
     return pb;
   }
-  const int k = min((int) p, bn.poff + bn.ptab_len);
+  const unsigned k = min((unsigned) (int) p, (unsigned) (bn.poff + bn.ptab_len));
   return MK ? (ptab[k] & 0xFFFFu) : ptab[k];
 }
 
