@@ -594,6 +594,13 @@ def main():
     line["kernel_ms"] = {k: v / args.steps for k, v in kms.items()}
     # in-range pairs of RR (unordered), the algorithm-independent rate
     line["inrange_pairs_per_s_RR"] = float(res["RR"].sum()) / (rr_ms * 1e-3)
+    # the same over the whole step: the unordered in-range pairs of DD and
+    # DR come from one unweighted count of each, outside the timed region
+    npair = float(res["RR"].sum())
+    for name, a, c, au in (("DD", D, None, True), ("DR", D, R, False)):
+        npair += float(eng.count_pairs(a, c, b, isauto=au, withwt=False, precision=prec).sum())
+    line["inrange_pairs_per_step"] = npair
+    line["inrange_pairs_per_s"] = npair * args.steps / dt
 
     # the other forms, one step each, on the same data
     forms = {}

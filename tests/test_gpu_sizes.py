@@ -142,3 +142,23 @@ def test_gpu_census_matches_oracle(eng, case):
         assert int(st["census"]) == cen, f"GPU census {st['census']} != oracle {cen}"
         # the oracle's census of these catalogues (computed on the CPU)
         assert cen == (9728 if case == "lattice7" else 0)
+
+
+@pytest.mark.parametrize("bt", [F.BIN_SMU, F.BIN_SPI])
+def test_survey_fp32_equals_fp64_at_scale(eng, bt):
+    """A size-independent property of the FP32 survey form: its windows
+    send every pair whose bin could differ to the FP64 re-evaluation, so its
+    integer counts equal the FP64 form's exactly -- here for 3e5 objects of
+    the survey shell (about 6e8 in-range pairs per count), auto and cross."""
+    x, y, z, _ = F.gen_survey(41, 300000)
+    xr, yr, zr, _ = F.gen_survey(42, 300000)
+    b = F.Bins.linear(F.GEOM_SURVEY, bt, 0, 200, 5, nmu=100 if bt == F.BIN_SMU else 1,
+                      pmin=0, pmax=200 if bt == F.BIN_SPI else 0, dpi=5)
+    d, r = F.Catalog(b, x, y, z), F.Catalog(b, xr, yr, zr)
+    for other in (None, r):
+        print(f"case survey fp32 vs fp64 bt={bt} cross={other is not None}", flush=True)
+        c64 = eng.count_pairs(d, other, b, isauto=other is None, precision="fp64")
+        c32 = eng.count_pairs(d, other, b, isauto=other is None, precision="fp32")
+        assert eng.stats()["rechecked"] > 0
+        assert c64.sum() > 1e8
+        np.testing.assert_array_equal(c32, c64)
