@@ -179,13 +179,18 @@ typedef struct { int64_t n; double *x, *y, *z, *w; double sumw; int loaded; } ca
 /* then binary ** ; * / % ; + - ; < <= > >= ; == != ; && ; || from the     */
 /* tightest, all binary operators left-associative and unary ones binding */
 /* tighter than any binary one (libast's precedence table).  Evaluated in */
-/* double (% is fmod, ** is pow, a comparison gives 1 or 0); a row is     */
-/* selected when its SELECTION is non-zero.  Columns are read as double;  */
-/* bitwise and shift operators and string columns are refused.            */
+/* double (% is fmod, ** is pow, a comparison gives 1 or 0) for POSITION  */
+/* and WEIGHT, whose columns are converted to double as libast converts   */
+/* them for a real-typed tree.  SELECTION keeps libast's dynamic types    */
+/* (x_eval_sel): integer columns and all-digit literals are long, so / and */
+/* % of two of them are C integer operations and ** an integer power;      */
+/* mixed long and double operands promote to double; a row is selected   */
+/* when the result is true (non-zero).  Bitwise and shift operators and   */
+/* string columns are refused.                                             */
 enum { X_NUM, X_COL, X_NEG, X_NOT, X_ABS, X_SQRT, X_LN, X_LOG, X_ISFIN, X_POW,
        X_MUL, X_DIV, X_REM, X_ADD, X_SUB, X_LT, X_LE, X_GT, X_GE, X_EQ, X_NE,
        X_AND, X_OR };
-typedef struct { int op; int col; double v; } xop_t;
+typedef struct { int op; int col; double v; long lv; int islong; } xop_t;
 typedef struct { xop_t *code; int n, cap, depth, maxdepth, maxcol; const char *p, *err; } xprog_t;
 
 static void x_emit(xprog_t *x, int op, int col, double v) {
@@ -194,7 +199,7 @@ static void x_emit(xprog_t *x, int op, int col, double v) {
     x->cap = x->cap ? 2 * x->cap : 16;
     x->code = realloc(x->code, sizeof(xop_t) * x->cap);
   }
-  x->code[x->n++] = (xop_t) {op, col, v};
+  x->code[x->n++] = (xop_t) {op, col, v, 0, 0};
   if (op == X_NUM || op == X_COL) { if (++x->depth > x->maxdepth) x->maxdepth = x->depth; }
   else if (op >= X_POW) x->depth--;
 }
@@ -247,6 +252,13 @@ static void x_unary(xprog_t *x) {
   if (e == p) { x->err = "unexpected token"; return; }
   x->p = e;
   x_emit(x, X_NUM, 0, v);
+  /* digits only: a long literal in SELECTION (libast.c:ast_parse_num) */
+  int digits = 1;
+  for (const char *q = p; q < e; q++) if (!isdigit((unsigned char) *q)) { digits = 0; break; }
+  if (digits && !x->err && v < 9.2e18) {
+    x->code[x->n - 1].islong = 1;
+    x->code[x->n - 1].lv = strtol(p, NULL, 10);
+  }
 }
 
 /* binary operator at x->p: its op code, precedence and length (0: none) */
@@ -290,6 +302,115 @@ static int x_compile(const char *src, xprog_t *x) {
   return 0;
 }
 
+/* a column value or an intermediate of SELECTION: long, double or bool */
+enum { XT_LONG, XT_DBL, XT_BOOL };
+typedef struct { int t; long l; double d; } xval_t;
+
+static double xv_dbl(const xval_t *v) { return v->t == XT_DBL ? v->d : (double) v->l; }
+static int xv_true(const xval_t *v) { return v->t == XT_DBL ? v->d != 0 : v->l != 0; }
+
+/* integer power with libast's conventions (libast.c:ipow): exponents
+ * above 255 give 1, +-1 or 0, the exponent is taken modulo 256, and from
+ * 64 on only bases 1 and -1 survive; products wrap around */
+static long x_ipow(long b, long e) {
+  if (e > 255) return b == 1 ? 1 : (b == -1 ? 1 - 2 * (e & 1) : 0);
+  unsigned ue = (unsigned char) e;
+  if (ue >= 64) return b == 1 ? 1 : (b == -1 ? 1 - 2 * (long) (ue & 1) : 0);
+  unsigned long r = 1, x = (unsigned long) b;
+  for (; ue; ue >>= 1, x *= x) if (ue & 1) r *= x;
+  return (long) r;
+}
+
+/* SELECTION with libast's dynamic types (libast.c:ast_eval_bool) */
+static int x_eval_sel(const xprog_t *x, const xval_t *col, xval_t *st) {
+  int sp = 0;
+  for (int i = 0; i < x->n; i++) {
+    const xop_t *o = &x->code[i];
+    xval_t *v;
+    switch (o->op) {
+      case X_NUM:
+        st[sp].t = o->islong ? XT_LONG : XT_DBL; st[sp].l = o->lv; st[sp].d = o->v; sp++;
+        break;
+      case X_COL: st[sp++] = col[o->col - 1]; break;
+      case X_NEG: case X_ABS: case X_SQRT: case X_LN: case X_LOG: case X_NOT: case X_ISFIN:
+        v = &st[sp - 1];
+        if (o->op == X_NOT) { v->l = !xv_true(v); v->t = XT_BOOL; }
+        else if (o->op == X_ISFIN) { v->l = v->t == XT_DBL ? isfinite(v->d) != 0 : 1; v->t = XT_BOOL; }
+        else if ((o->op == X_NEG || o->op == X_ABS) && v->t == XT_LONG)
+          v->l = (o->op == X_NEG || v->l < 0) ? -v->l : v->l;
+        else {
+          const double a = xv_dbl(v);
+          v->d = (o->op == X_NEG) ? -a : (o->op == X_ABS) ? fabs(a) : (o->op == X_SQRT) ? sqrt(a)
+                 : (o->op == X_LN) ? log(a) : log10(a);
+          v->t = XT_DBL;
+        }
+        break;
+      default: {
+        xval_t *a = &st[sp - 2], *b = &st[sp - 1];
+        sp--;
+        if (o->op == X_AND || o->op == X_OR) {
+          const int r = (o->op == X_AND) ? (xv_true(a) && xv_true(b)) : (xv_true(a) || xv_true(b));
+          a->t = XT_BOOL; a->l = r;
+          break;
+        }
+        const int lng = (a->t != XT_DBL && b->t != XT_DBL);
+        if (o->op >= X_LT) {
+          int r;
+          if (lng) {
+            switch (o->op) {
+              case X_LT: r = a->l < b->l; break;
+              case X_LE: r = a->l <= b->l; break;
+              case X_GT: r = a->l > b->l; break;
+              case X_GE: r = a->l >= b->l; break;
+              case X_EQ: r = a->l == b->l; break;
+              default: r = a->l != b->l; break;
+            }
+          }
+          else {
+            const double u = xv_dbl(a), w = xv_dbl(b);
+            switch (o->op) {
+              case X_LT: r = u < w; break;
+              case X_LE: r = u <= w; break;
+              case X_GT: r = u > w; break;
+              case X_GE: r = u >= w; break;
+              case X_EQ: r = u == w; break;
+              default: r = u != w; break;
+            }
+          }
+          a->t = XT_BOOL; a->l = r;
+          break;
+        }
+        if (lng) {
+          const long u = a->l, w = b->l;
+          if ((o->op == X_DIV || o->op == X_REM) && w == 0) return -1;
+          switch (o->op) {
+            case X_POW: a->l = x_ipow(u, w); break;
+            case X_MUL: a->l = (long) ((unsigned long) u * (unsigned long) w); break;
+            case X_DIV: a->l = u / w; break;
+            case X_REM: a->l = u % w; break;
+            case X_ADD: a->l = (long) ((unsigned long) u + (unsigned long) w); break;
+            default: a->l = (long) ((unsigned long) u - (unsigned long) w); break;
+          }
+          a->t = XT_LONG;
+        }
+        else {
+          const double u = xv_dbl(a), w = xv_dbl(b);
+          switch (o->op) {
+            case X_POW: a->d = pow(u, w); break;
+            case X_MUL: a->d = u * w; break;
+            case X_DIV: a->d = u / w; break;
+            case X_REM: a->d = fmod(u, w); break;
+            case X_ADD: a->d = u + w; break;
+            default: a->d = u - w; break;
+          }
+          a->t = XT_DBL;
+        }
+      }
+    }
+  }
+  return xv_true(&st[0]);
+}
+
 static double x_eval(const xprog_t *x, const double *col, double *st) {
   int sp = 0;
   for (int i = 0; i < x->n; i++) {
@@ -329,61 +450,106 @@ static double x_eval(const xprog_t *x, const double *col, double *st) {
   return st[0];
 }
 
-/* conversions of a C99 formatter: whether each is assigned (not %*...) and
- * numeric (not %s, %c, %[...]) */
-static int fmt_fields(const char *fmt, int *supp, int *isnum, int max) {
-  int n = 0;
-  for (const char *p = fmt; *p; p++) {
-    if (*p != '%') continue;
-    if (p[1] == '%') { p++; continue; }
-    p++;
-    int s = 0;
-    if (*p == '*') { s = 1; p++; }
-    while (*p && !isalpha((unsigned char) *p) && *p != '[') p++;
-    while (*p == 'l' || *p == 'h' || *p == 'L' || *p == 'j' || *p == 'z' || *p == 't') p++;
-    int num = !(*p == 's' || *p == 'c' || *p == '[');
-    if (*p == '[') { while (*p && *p != ']') p++; }
-    if (n < max) { supp[n] = s; isnum[n] = num; }
-    n++;
-    if (!*p) break;
+/* The conversions of an ASCII_FORMATTER, split as ascii_fmtr.c splits
+ * them: each piece is the literal text before one conversion plus the
+ * conversion, scanned with sscanf and "%n" for its length.  Its type
+ * decides how the column reaches the expressions (read_ascii.c:
+ * ascii_col_init): %d / %i are int, %ld / %li long, %f %e %g %a float,
+ * with l double; strings and the other types are columns that are read
+ * past but cannot be used; %* conversions are not columns. */
+enum { FT_SKIP, FT_NULL, FT_STR, FT_INT, FT_LONG, FT_FLT, FT_DBL };
+typedef struct { char *fmt; int type; } fseg_t;
+
+static void fsegs_free(fseg_t *f, int n) { for (int i = 0; i < n; i++) free(f[i].fmt); free(f); }
+
+static int fmt_split(const char *fmt, fseg_t **out) {
+  int n = 0, cap = 8;
+  fseg_t *f = malloc(sizeof(fseg_t) * cap);
+  const char *seg = fmt, *p = fmt;
+  while (*p) {
+    if (*p != '%') { p++; continue; }
+    if (p[1] == '%') { p += 2; continue; }
+    const char *pc = p++;                      /* the '%' */
+    int supp = 0;
+    if (*p == '*') { supp = 1; p++; }
+    while (isdigit((unsigned char) *p)) p++;
+    char lm = 0;
+    if (*p == 'h' || *p == 'l') { lm = *p++; if (*p == lm) { lm = (char) (lm == 'h' ? 'H' : 'L'); p++; } }
+    else if (*p == 'j' || *p == 'z' || *p == 't' || *p == 'L') { lm = (*p == 'L') ? 'D' : *p; p++; }
+    const char cv = *p;
+    if (!cv) break;
+    int type = FT_NULL;
+    if (cv == '[') { p++; if (*p == '^') p++; if (*p == ']') p++; while (*p && *p != ']') p++; }
+    if (cv == 'd' || cv == 'i') type = (lm == 0) ? FT_INT : (lm == 'l') ? FT_LONG : FT_NULL;
+    else if (strchr("aAeEfFgG", cv)) type = (lm == 0) ? FT_FLT : (lm == 'l') ? FT_DBL : FT_NULL;
+    else if (cv == 's' || cv == 'c' || cv == '[') type = (lm == 0) ? FT_STR : FT_NULL;
+    if (*p) p++;
+    if (supp) type = FT_SKIP;
+    /* the piece, with '*' for values that are not stored, and "%n" */
+    const size_t pre = (size_t) (pc - seg), rest = (size_t) (p - pc - 1);
+    char *t = malloc(pre + rest + 6);
+    memcpy(t, seg, pre);
+    size_t k = pre;
+    t[k++] = '%';
+    if (!supp && (type == FT_NULL || type == FT_STR)) t[k++] = '*';
+    memcpy(t + k, pc + 1, rest);
+    k += rest;
+    memcpy(t + k, "%n", 3);
+    if (n == cap) { cap *= 2; f = realloc(f, sizeof(fseg_t) * cap); }
+    f[n].fmt = t; f[n].type = type; n++;
+    seg = p;
   }
+  *out = f;
   return n;
 }
 
 /* Read an ASCII catalogue: nout expressions (x, y, z[, w]) per selected
- * row (read_ascii.c:read_ascii_data).  Columns are counted over the
- * assigned conversions of the formatter ($n: the n-th of them); without a
- * formatter, over whitespace- or comma-separated fields. */
+ * row (read_ascii.c:read_ascii_data).  Columns are the formatter's
+ * conversions that are not suppressed ($n: the n-th of them); without a
+ * formatter, whitespace- or comma-separated fields read as double. */
 static int read_ascii(const char *fname, long skip, char cmt, const char *fmt,
     xprog_t *out, int nout, const xprog_t *sel, cat_t *c) {
   FILE *fp = fopen(fname, "r");
   if (!fp) { P_ERR("cannot open catalogue `%s'\n", fname); return FCFC_ERR_FILE; }
-  int supp[256], isnum[256];
-  int nf = fmt ? fmt_fields(fmt, supp, isnum, 256) : 0;
-  if (nf > 256) nf = 256;
+  fseg_t *fs = NULL;
+  int nf = fmt ? fmt_split(fmt, &fs) : 0;
   int maxcol = sel ? sel->maxcol : 0, depth = sel ? sel->maxdepth : 1;
   for (int k = 0; k < nout; k++) {
     if (out[k].maxcol > maxcol) maxcol = out[k].maxcol;
     if (out[k].maxdepth > depth) depth = out[k].maxdepth;
   }
-  /* field index of column n (1-based) */
+  /* the formatter piece (or the field) of column n (1-based); only the
+   * columns the expressions use must be numeric */
   int *field = malloc(sizeof(int) * (maxcol + 1));
+  char *used = calloc((size_t) maxcol + 1, 1);
+  for (int k = 0; k <= nout; k++) {
+    const xprog_t *xp = (k < nout) ? &out[k] : sel;
+    if (xp) for (int i = 0; i < xp->n; i++) if (xp->code[i].op == X_COL) used[xp->code[i].col] = 1;
+  }
   int nfield = 0;
   for (int cn = 1; cn <= maxcol; cn++) {
-    if (nf == 0) { field[cn] = cn - 1; continue; }
+    if (!fmt) { field[cn] = cn - 1; continue; }
     int seen = 0; field[cn] = -1;
     for (int i = 0; i < nf; i++) {
-      if (supp[i]) continue;
+      if (fs[i].type == FT_SKIP) continue;
       if (++seen == cn) {
-        if (!isnum[i]) { P_ERR("column $%d is not numeric in ASCII_FORMATTER\n", cn); free(field); fclose(fp); return FCFC_ERR_CFG; }
+        if (used[cn] && (fs[i].type == FT_STR || fs[i].type == FT_NULL)) {
+          P_ERR("column $%d of ASCII_FORMATTER is not an int, long, float or double conversion\n", cn);
+          free(field); free(used); fsegs_free(fs, nf); fclose(fp); return FCFC_ERR_CFG;
+        }
         field[cn] = i; break;
       }
     }
-    if (field[cn] < 0) { P_ERR("column $%d not in ASCII_FORMATTER\n", cn); free(field); fclose(fp); return FCFC_ERR_CFG; }
+    if (field[cn] < 0) {
+      P_ERR("column $%d not in ASCII_FORMATTER\n", cn);
+      free(field); free(used); fsegs_free(fs, nf); fclose(fp); return FCFC_ERR_CFG;
+    }
   }
-  for (int cn = 1; cn <= maxcol; cn++) if (field[cn] + 1 > nfield) nfield = field[cn] + 1;
-  double *col = malloc(sizeof(double) * (maxcol + 1)), *stack = malloc(sizeof(double) * (depth + 1));
-  double *fv = malloc(sizeof(double) * (nfield + 1));
+  if (!fmt) for (int cn = 1; cn <= maxcol; cn++) if (field[cn] + 1 > nfield) nfield = field[cn] + 1;
+  if (fmt) nfield = nf;
+  xval_t *fv = malloc(sizeof(xval_t) * (nfield + 1)), *col = malloc(sizeof(xval_t) * (maxcol + 1));
+  xval_t *xst = malloc(sizeof(xval_t) * (depth + 1));
+  double *cold = malloc(sizeof(double) * (maxcol + 1)), *stack = malloc(sizeof(double) * (depth + 1));
   size_t cap = 1 << 16;
   c->x = malloc(sizeof(double) * cap); c->y = malloc(sizeof(double) * cap);
   c->z = malloc(sizeof(double) * cap); c->w = (nout > 3) ? malloc(sizeof(double) * cap) : NULL;
@@ -398,26 +564,62 @@ static int read_ascii(const char *fname, long skip, char cmt, const char *fmt,
     while (isspace((unsigned char) *p)) p++;
     if (!*p) continue;
     if (cmt && *p == cmt) continue;
-    int ti = 0;
-    char *save;
-    for (char *t = strtok_r(p, " \t\r\n,", &save); t && ti < nfield; t = strtok_r(NULL, " \t\r\n,", &save), ti++)
-      fv[ti] = strtod(t, NULL);
-    if (ti < nfield) { P_ERR("line %ld of `%s' has too few columns\n", lno, fname); e = FCFC_ERR_ASCII; break; }
-    for (int cn = 1; cn <= maxcol; cn++) col[cn - 1] = fv[field[cn]];
-    if (sel && x_eval(sel, col, stack) == 0) continue;
+    if (fmt) {
+      /* one sscanf per conversion (read_ascii.c:ascii_read_line) */
+      const char *q = line;
+      for (int i = 0; i < nf && !e; i++) {
+        int n = 0, r, iv;
+        long lv;
+        float fv32;
+        double dv;
+        switch (fs[i].type) {
+          case FT_INT: r = sscanf(q, fs[i].fmt, &iv, &n); fv[i].t = XT_LONG; fv[i].l = iv; break;
+          case FT_LONG: r = sscanf(q, fs[i].fmt, &lv, &n); fv[i].t = XT_LONG; fv[i].l = lv; break;
+          case FT_FLT: r = sscanf(q, fs[i].fmt, &fv32, &n); fv[i].t = XT_DBL; fv[i].d = (double) fv32; break;
+          case FT_DBL: r = sscanf(q, fs[i].fmt, &dv, &n); fv[i].t = XT_DBL; fv[i].d = dv; break;
+          case FT_STR:
+            while (isspace((unsigned char) *q)) q++;
+            r = sscanf(q, fs[i].fmt, &n) + 1;
+            break;
+          default: r = sscanf(q, fs[i].fmt, &n) + 1; break;
+        }
+        if (r != 1 || n == 0) e = FCFC_ERR_ASCII;
+        q += n;
+      }
+      if (e) { P_ERR("line %ld of `%s' does not match ASCII_FORMATTER\n", lno, fname); break; }
+    }
+    else {
+      int ti = 0;
+      char *save;
+      for (char *t = strtok_r(p, " \t\r\n,", &save); t && ti < nfield; t = strtok_r(NULL, " \t\r\n,", &save), ti++) {
+        fv[ti].t = XT_DBL; fv[ti].d = strtod(t, NULL);
+      }
+      if (ti < nfield) { P_ERR("line %ld of `%s' has too few columns\n", lno, fname); e = FCFC_ERR_ASCII; break; }
+    }
+    for (int cn = 1; cn <= maxcol; cn++) {
+      if (!used[cn]) continue;
+      col[cn - 1] = fv[field[cn]];
+      cold[cn - 1] = xv_dbl(&col[cn - 1]);
+    }
+    if (sel) {
+      const int r = x_eval_sel(sel, col, xst);
+      if (r < 0) { P_ERR("integer division by zero in SELECTION (line %ld of `%s')\n", lno, fname); e = FCFC_ERR_AST; break; }
+      if (!r) continue;
+    }
     if ((size_t) c->n == cap) {
       cap *= 2;
       c->x = realloc(c->x, sizeof(double) * cap); c->y = realloc(c->y, sizeof(double) * cap);
       c->z = realloc(c->z, sizeof(double) * cap);
       if (c->w) c->w = realloc(c->w, sizeof(double) * cap);
     }
-    c->x[c->n] = x_eval(&out[0], col, stack);
-    c->y[c->n] = x_eval(&out[1], col, stack);
-    c->z[c->n] = x_eval(&out[2], col, stack);
-    if (c->w) c->w[c->n] = x_eval(&out[3], col, stack);
+    c->x[c->n] = x_eval(&out[0], cold, stack);
+    c->y[c->n] = x_eval(&out[1], cold, stack);
+    c->z[c->n] = x_eval(&out[2], cold, stack);
+    if (c->w) c->w[c->n] = x_eval(&out[3], cold, stack);
     c->n++;
   }
-  free(line); free(field); free(col); free(stack); free(fv);
+  free(line); free(field); free(used); free(col); free(cold); free(stack); free(fv); free(xst);
+  if (fs) fsegs_free(fs, nf);
   fclose(fp);
   if (e) return e;
   c->loaded = 1;

@@ -166,6 +166,14 @@ def cli_catalogs(kind):
         (x, y, z, w), _ = catalogs("box_clustered")[0]
         flag = np.random.default_rng(51).random(len(x))
         return [[x, y, z, w, flag]]
+    if kind == "box_types":
+        # float, double, float, int, a string column, float, a suppressed
+        # column: typed ASCII_FORMATTER conversions and integer SELECTION
+        (x, y, z, w), _ = catalogs("box_clustered")[0]
+        g = np.random.default_rng(52)
+        ids = g.integers(0, 10, len(x)).astype(float)
+        junk = g.random(len(x))
+        return [[x, y, z, ids, junk, w, junk]]
     if kind == "survey_radec":
         # RA, Dec in degrees over a 10 x 10 deg patch, redshift in [0.4, 0.45]
         g = np.random.default_rng(61)
@@ -208,6 +216,13 @@ EXPR = ("CATALOG = {d}/cat0.txt\nCATALOG_LABEL = D\n"
         "BOX_SIZE = 1000\nBINNING_SCHEME = 1\nPAIR_COUNT = DD\nPAIR_COUNT_FILE = {d}/DD.bin\n"
         "SEP_BIN_MIN = 0\nSEP_BIN_MAX = 200\nSEP_BIN_SIZE = 5\nMU_BIN_NUM = 10\n"
         "OUTPUT_FORMAT = 0\nOVERWRITE = 2\n")
+TYPES = ("CATALOG = {d}/cat0.txt\nCATALOG_LABEL = D\n"
+         'ASCII_FORMATTER = "%f %lf %f %d %s %f %*lf"\n'
+         "POSITION = [$1, $2, ($3 + $4 / 3) % 1000]\nWEIGHT = $6 * 2\n"
+         "SELECTION = ($4 / 3 == 1 || $4 % 4 == 0) && 2 ** $4 < 200 && $6 > 0.4\n"
+         "BOX_SIZE = 1000\nBINNING_SCHEME = 1\nPAIR_COUNT = DD\nPAIR_COUNT_FILE = {d}/DD.bin\n"
+         "SEP_BIN_MIN = 0\nSEP_BIN_MAX = 200\nSEP_BIN_SIZE = 5\nMU_BIN_NUM = 10\n"
+         "OUTPUT_FORMAT = 0\nOVERWRITE = 2\n")
 CONV = ("CATALOG = [{d}/cat0.txt, {d}/cat1.txt]\nCATALOG_LABEL = [D, R]\n"
         'ASCII_FORMATTER = ["%lf %lf %lf", "%lf %lf %lf"]\n'
         "POSITION = [$1, $2, $3, $1, $2, $3]\nCOORD_CONVERT = T\n{cosmo}"
@@ -220,6 +235,7 @@ CLI_CASES = [
      ["DD.txt", "DR.txt", "RR.txt", "xi_ls.txt", "xi_nat.txt", "mp_ls.txt", "mp_nat.txt"]),
     ("box_spi_wp", "FCFC_2PT_BOX", "box_plain", SPI_WP, ["DD.bin", "DR.bin", "RR.bin", "xi.txt", "wp.txt"]),
     ("box_expr", "FCFC_2PT_BOX", "box_expr", EXPR, ["DD.bin"]),
+    ("box_types", "FCFC_2PT_BOX", "box_types", TYPES, ["DD.bin"]),
     ("survey_convert_integral", "FCFC_2PT", "survey_radec",
      CONV.replace("{cosmo}", "OMEGA_M = 0.31\nDE_EOS_W = -0.9\nCMVDST_ERR = 1e-10\n"), ["DD.bin", "DR.bin"]),
     ("survey_convert_table", "FCFC_2PT", "survey_radec",
@@ -269,7 +285,10 @@ def run_cli_case(name, code, kind, template, outs):
 
 def main():
     if "--cli" in sys.argv:
+        only = [a for a in sys.argv[1:] if not a.startswith("--")]
         for case in CLI_CASES:
+            if only and case[0] not in only:
+                continue
             print("cli case", case[0], flush=True)
             data = run_cli_case(*case)
             with open(os.path.join(HERE, "cli_" + case[0] + ".json"), "w") as fp:
