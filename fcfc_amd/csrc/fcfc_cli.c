@@ -10,11 +10,11 @@
  * write the pair-count file (same binary/ASCII layout), then evaluate the
  * CF_ESTIMATOR expressions, multipoles and projected CFs.
  *
- * Supported subset: ASCII catalogues (CATALOG_TYPE 0); POSITION, WEIGHT
- * and SELECTION as the numeric and logical subset of FCFC's expressions
- * (columns read as double; bitwise operators and strings are refused);
- * COORD_CONVERT with the fiducial cosmology or a (z, distance) table
- * (FCFC_2PT).  FITS and HDF5 catalogues are refused.  Extra key with
+ * Supported subset: ASCII catalogues (CATALOG_TYPE 0) and FITS binary
+ * tables (1); POSITION, WEIGHT and SELECTION as the numeric and logical
+ * subset of FCFC's expressions (bitwise operators and strings are
+ * refused); COORD_CONVERT with the fiducial cosmology or a (z, distance)
+ * table (FCFC_2PT).  HDF5 catalogues are refused.  Extra key with
  * no counterpart in the reference: GPU_PRECISION = 0 (FP64 kernel, default)
  * or 1 (FP32 distances with FP64 re-evaluation near bin edges); CLI --gpu-prec.
  * Compiled twice with -DFCFC_CLI_GEOM=0 (box) or 1 (survey).
@@ -24,6 +24,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <strings.h>
 #include <sys/stat.h>
 #include "fcfc_gpu.h"
 
@@ -191,7 +192,10 @@ enum { X_NUM, X_COL, X_NEG, X_NOT, X_ABS, X_SQRT, X_LN, X_LOG, X_ISFIN, X_POW,
        X_MUL, X_DIV, X_REM, X_ADD, X_SUB, X_LT, X_LE, X_GT, X_GE, X_EQ, X_NE,
        X_AND, X_OR };
 typedef struct { int op; int col; double v; long lv; int islong; } xop_t;
-typedef struct { xop_t *code; int n, cap, depth, maxdepth, maxcol; const char *p, *err; } xprog_t;
+typedef struct {
+  xop_t *code; int n, cap, depth, maxdepth, maxcol; const char *p, *err;
+  char (*names)[72]; int nnames;       /* FITS column names for ${NAME} */
+} xprog_t;
 
 static void x_emit(xprog_t *x, int op, int col, double v) {
   if (x->err) return;
@@ -223,6 +227,18 @@ static void x_unary(xprog_t *x) {
     if (brace) p++;
     char *e;
     long c = strtol(p, &e, 10);
+    if (brace && x->names && (e == p || *e != '}')) {
+      /* ${NAME}: a FITS column, by name without regard to case
+       * (read_fits.c:get_fts_colnum, CASEINSEN) */
+      const char *q = strchr(p, '}');
+      if (!q || q == p || q - p >= 72) { x->err = "invalid column name"; return; }
+      c = 0;
+      for (int k = 0; k < x->nnames && !c; k++)
+        if ((size_t) (q - p) == strlen(x->names[k]) && !strncasecmp(p, x->names[k], (size_t) (q - p)))
+          c = k + 1;
+      if (!c) { x->err = "no FITS column of this name"; return; }
+      e = (char *) q;
+    }
     if (e == p || c < 1 || c > 65535) { x->err = "invalid column reference"; return; }
     if (brace) { if (*e != '}') { x->err = "invalid column reference"; return; } e++; }
     x->p = e;
@@ -291,8 +307,10 @@ static void x_expr(xprog_t *x, int minprec) {
   }
 }
 
-static int x_compile(const char *src, xprog_t *x) {
+static int x_compile_named(const char *src, xprog_t *x, char (*names)[72], int nnames) {
   memset(x, 0, sizeof *x);
+  x->names = names;
+  x->nnames = nnames;
   x->p = src;
   x_expr(x, 0);
   x_ws(x);
@@ -626,6 +644,215 @@ static int read_ascii(const char *fname, long skip, char cmt, const char *fmt,
   return 0;
 }
 
+/* FITS binary tables (CATALOG_TYPE 1; the reference reads them with
+ * cfitsio, src/io/read_fits.c).  Written from the FITS standard: header
+ * blocks of 2880 bytes holding 80-character cards, the first HDU with data
+ * must be a BINTABLE extension, rows of NAXIS1 bytes, big-endian fields.
+ * Columns are TTYPEn (${NAME}, without regard to case) or numbers;
+ * B, I, J and K columns are integers (long in SELECTION), E float, D
+ * double -- the types read_fits.c:fits_read_real accepts; other types,
+ * vector columns and scaled columns (TSCALn, TZEROn) are refused when an
+ * expression uses them. */
+typedef struct { char name[72]; char type; long repeat; long off; int scaled; } fcol_t;
+typedef struct { FILE *fp; long nrow, rowlen, data; int ncol; char (*names)[72]; fcol_t *col; } fits_t;
+
+static void fits_close(fits_t *f) {
+  if (f->fp) fclose(f->fp);
+  free(f->col); free(f->names);
+  memset(f, 0, sizeof *f);
+}
+
+/* value of a header card: the text after "= " up to a comment, unquoted */
+static void fits_value(const char *card, char *val) {
+  const char *p = card + 10, *end = card + 80;
+  while (p < end && *p == ' ') p++;
+  size_t k = 0;
+  if (p < end && *p == '\'') {
+    for (p++; p < end; p++) {
+      if (*p == '\'') { if (p + 1 < end && p[1] == '\'') { val[k++] = '\''; p++; continue; } break; }
+      val[k++] = *p;
+    }
+    while (k && val[k - 1] == ' ') k--;
+  }
+  else {
+    while (p < end && *p != '/' && *p != ' ') val[k++] = *p++;
+  }
+  val[k] = '\0';
+}
+
+static int fits_open(const char *fname, fits_t *f) {
+  memset(f, 0, sizeof *f);
+  if (strchr(fname, '[')) { P_ERR("FITS extended file names are not supported: `%s'\n", fname); return FCFC_ERR_FILE; }
+  if (!(f->fp = fopen(fname, "rb"))) { P_ERR("cannot open catalogue `%s'\n", fname); return FCFC_ERR_FILE; }
+  char block[2880], val[80];
+  long pos = 0;
+  for (int hdu = 0;; hdu++) {
+    long bitpix = 0, naxis = 0, nax[9] = {0}, pcount = 0, gcount = 1, tfields = 0;
+    int bintable = 0, end = 0;
+    fcol_t *col = NULL;
+    while (!end) {
+      if (fread(block, 1, 2880, f->fp) != 2880) {
+        free(col);
+        P_ERR("no binary table in FITS file `%s'\n", fname);
+        fits_close(f); return FCFC_ERR_FILE;
+      }
+      pos += 2880;
+      for (int i = 0; i < 36 && !end; i++) {
+        const char *card = block + 80 * i;
+        char key[9];
+        memcpy(key, card, 8); key[8] = '\0';
+        for (int k = 7; k >= 0 && key[k] == ' '; k--) key[k] = '\0';
+        if (!strcmp(key, "END")) { end = 1; break; }
+        if (card[8] != '=' || card[9] != ' ') continue;
+        fits_value(card, val);
+        if (hdu == 0 && !strcmp(key, "SIMPLE") && strcmp(val, "T")) {
+          P_ERR("`%s' is not a FITS file\n", fname); free(col); fits_close(f); return FCFC_ERR_FILE;
+        }
+        if (!strcmp(key, "XTENSION")) bintable = !strcmp(val, "BINTABLE");
+        else if (!strcmp(key, "BITPIX")) bitpix = atol(val);
+        else if (!strcmp(key, "NAXIS")) naxis = atol(val);
+        else if (!strncmp(key, "NAXIS", 5) && isdigit((unsigned char) key[5])) {
+          int a = atoi(key + 5);
+          if (a >= 1 && a <= 8) nax[a] = atol(val);
+        }
+        else if (!strcmp(key, "PCOUNT")) pcount = atol(val);
+        else if (!strcmp(key, "GCOUNT")) gcount = atol(val);
+        else if (!strcmp(key, "TFIELDS")) {
+          tfields = atol(val);
+          if (tfields < 0 || tfields > 999) { P_ERR("bad TFIELDS in `%s'\n", fname); free(col); fits_close(f); return FCFC_ERR_FILE; }
+          col = calloc((size_t) tfields + 1, sizeof(fcol_t));
+          for (long c = 0; c < tfields; c++) col[c].repeat = -1;
+        }
+        else if (col && (!strncmp(key, "TTYPE", 5) || !strncmp(key, "TFORM", 5) ||
+                         !strncmp(key, "TSCAL", 5) || !strncmp(key, "TZERO", 5))) {
+          long c = atol(key + 5) - 1;
+          if (c < 0 || c >= tfields) continue;
+          if (key[1] == 'T') snprintf(col[c].name, sizeof col[c].name, "%.71s", val);
+          else if (key[1] == 'F') {
+            char *q;
+            long r = strtol(val, &q, 10);
+            col[c].repeat = (q == val) ? 1 : r;
+            col[c].type = *q;
+          }
+          else if ((key[1] == 'S' && atof(val) != 1.0) || (key[1] == 'Z' && atof(val) != 0.0))
+            col[c].scaled = 1;
+        }
+      }
+    }
+    long size = 0;
+    if (naxis > 0) {
+      size = 1;
+      for (long a = 1; a <= naxis && a <= 8; a++) size *= nax[a];
+      size = (labs(bitpix) / 8) * gcount * (pcount + size);
+    }
+    if (bintable) {
+      f->nrow = nax[2]; f->rowlen = nax[1]; f->ncol = (int) tfields; f->col = col; f->data = pos;
+      long off = 0;
+      f->names = calloc((size_t) tfields + 1, sizeof *f->names);
+      for (long c = 0; c < tfields; c++) {
+        static const char tc[] = "LXBIJKAEDCMPQ";
+        static const int tw[] = {1, 0, 1, 2, 4, 8, 1, 4, 8, 8, 16, 8, 16};
+        const char *t = col[c].type ? strchr(tc, col[c].type) : NULL;
+        if (!t || col[c].repeat < 0) { P_ERR("bad TFORM%ld in `%s'\n", c + 1, fname); fits_close(f); return FCFC_ERR_FILE; }
+        col[c].off = off;
+        off += (col[c].type == 'X') ? (col[c].repeat + 7) / 8 : col[c].repeat * tw[t - tc];
+        memcpy(f->names[c], col[c].name, sizeof col[c].name);
+      }
+      if (off != f->rowlen) { P_ERR("row length of `%s' does not match its columns\n", fname); fits_close(f); return FCFC_ERR_FILE; }
+      return 0;
+    }
+    free(col);
+    if (hdu == 0 && size > 0) {
+      P_ERR("the first HDU with data in `%s' is not a binary table\n", fname);
+      fits_close(f); return FCFC_ERR_FILE;
+    }
+    /* skip the data of this HDU */
+    const long skip = (size + 2879) / 2880 * 2880;
+    if (fseek(f->fp, skip, SEEK_CUR)) { P_ERR("truncated FITS file `%s'\n", fname); fits_close(f); return FCFC_ERR_FILE; }
+    pos += skip;
+  }
+}
+
+static uint64_t fits_be(const unsigned char *b, int n) {
+  uint64_t v = 0;
+  for (int i = 0; i < n; i++) v = (v << 8) | b[i];
+  return v;
+}
+
+/* the rows of a FITS table: nout expressions per selected row */
+static int read_fits(const char *fname, fits_t *f, xprog_t *out, int nout, const xprog_t *sel,
+    cat_t *c) {
+  int maxcol = sel ? sel->maxcol : 0, depth = sel ? sel->maxdepth : 1;
+  for (int k = 0; k < nout; k++) {
+    if (out[k].maxcol > maxcol) maxcol = out[k].maxcol;
+    if (out[k].maxdepth > depth) depth = out[k].maxdepth;
+  }
+  if (maxcol > f->ncol) { P_ERR("column %d not in FITS table `%s'\n", maxcol, fname); return FCFC_ERR_CFG; }
+  char *used = calloc((size_t) maxcol + 1, 1);
+  for (int k = 0; k <= nout; k++) {
+    const xprog_t *xp = (k < nout) ? &out[k] : sel;
+    if (xp) for (int i = 0; i < xp->n; i++) if (xp->code[i].op == X_COL) used[xp->code[i].col] = 1;
+  }
+  for (int cn = 1; cn <= maxcol; cn++) {
+    if (!used[cn]) continue;
+    const fcol_t *fc = &f->col[cn - 1];
+    if (!strchr("BIJKED", fc->type) || fc->repeat != 1 || fc->scaled) {
+      P_ERR("FITS column `%s' of `%s' is not a scalar B, I, J, K, E or D column without scaling\n",
+            fc->name, fname);
+      free(used); return FCFC_ERR_CFG;
+    }
+  }
+  xval_t *col = malloc(sizeof(xval_t) * (maxcol + 1)), *xst = malloc(sizeof(xval_t) * (depth + 1));
+  double *cold = malloc(sizeof(double) * (maxcol + 1)), *stack = malloc(sizeof(double) * (depth + 1));
+  const long chunk = 4096;
+  unsigned char *buf = malloc((size_t) (f->rowlen ? f->rowlen : 1) * chunk);
+  size_t cap = (size_t) (f->nrow > 0 ? f->nrow : 1);
+  c->x = malloc(sizeof(double) * cap); c->y = malloc(sizeof(double) * cap);
+  c->z = malloc(sizeof(double) * cap); c->w = (nout > 3) ? malloc(sizeof(double) * cap) : NULL;
+  c->n = 0;
+  int e = 0;
+  if (fseek(f->fp, f->data, SEEK_SET)) e = FCFC_ERR_FILE;
+  for (long r0 = 0; r0 < f->nrow && !e; r0 += chunk) {
+    const long nr = (f->nrow - r0 < chunk) ? f->nrow - r0 : chunk;
+    if (fread(buf, (size_t) f->rowlen, (size_t) nr, f->fp) != (size_t) nr) {
+      P_ERR("truncated FITS table `%s'\n", fname); e = FCFC_ERR_FILE; break;
+    }
+    for (long r = 0; r < nr && !e; r++) {
+      const unsigned char *row = buf + r * f->rowlen;
+      for (int cn = 1; cn <= maxcol; cn++) {
+        if (!used[cn]) continue;
+        const fcol_t *fc = &f->col[cn - 1];
+        const unsigned char *b = row + fc->off;
+        xval_t *v = &col[cn - 1];
+        uint64_t u;
+        switch (fc->type) {
+          case 'B': v->t = XT_LONG; v->l = b[0]; break;
+          case 'I': v->t = XT_LONG; v->l = (int16_t) fits_be(b, 2); break;
+          case 'J': v->t = XT_LONG; v->l = (int32_t) fits_be(b, 4); break;
+          case 'K': v->t = XT_LONG; v->l = (long) (int64_t) fits_be(b, 8); break;
+          case 'E': { uint32_t w32 = (uint32_t) fits_be(b, 4); float fv; memcpy(&fv, &w32, 4); v->t = XT_DBL; v->d = fv; break; }
+          default: { u = fits_be(b, 8); double dv; memcpy(&dv, &u, 8); v->t = XT_DBL; v->d = dv; break; }
+        }
+        cold[cn - 1] = xv_dbl(v);
+      }
+      if (sel) {
+        const int rs = x_eval_sel(sel, col, xst);
+        if (rs < 0) { P_ERR("integer division by zero in SELECTION (row %ld of `%s')\n", r0 + r + 1, fname); e = FCFC_ERR_AST; break; }
+        if (!rs) continue;
+      }
+      c->x[c->n] = x_eval(&out[0], cold, stack);
+      c->y[c->n] = x_eval(&out[1], cold, stack);
+      c->z[c->n] = x_eval(&out[2], cold, stack);
+      if (c->w) c->w[c->n] = x_eval(&out[3], cold, stack);
+      c->n++;
+    }
+  }
+  free(used); free(col); free(xst); free(cold); free(stack); free(buf);
+  if (e) return e;
+  c->loaded = 1;
+  return 0;
+}
+
 #if FCFC_CLI_GEOM == 1
 /* the first two columns of an ASCII file (Z_CMVDST_FILE): lines that are
  * empty or start with '#' are skipped (read_ascii.c:read_ascii_table) */
@@ -936,8 +1163,17 @@ int main(int argc, char **argv) {
   nmpo = split_list(kv_get("MULTIPOLE_FILE"), &mpout);
   nwpo = split_list(kv_get("PROJECTED_FILE"), &wpout);
   if (npos != 3 * ncat) { P_ERR("POSITION needs 3 entries per catalogue\n"); FAIL(FCFC_ERR_CFG); }
-  if (nfmt != ncat) { P_ERR("ASCII_FORMATTER is not set for every catalogue\n"); FAIL(FCFC_ERR_CFG); }
-  for (int i = 0; i < ntype; i++) if (atoi(types[i]) != 0) { P_ERR("only ASCII catalogues (CATALOG_TYPE 0) are supported\n"); FAIL(FCFC_ERR_CFG); }
+  /* CATALOG_TYPE: 0 ASCII, 1 FITS (2, HDF5, is not supported); one for
+   * every catalogue or one for all */
+  if (ntype > 1 && ntype != ncat) { P_ERR("CATALOG_TYPE needs one entry per catalogue\n"); FAIL(FCFC_ERR_CFG); }
+  int any_ascii = (ntype == 0);
+  for (int i = 0; i < ntype; i++) {
+    const int t = atoi(types[i]);
+    if (t == 2) { P_ERR("HDF5 catalogues (CATALOG_TYPE 2) are not supported\n"); FAIL(FCFC_ERR_CFG); }
+    if (t != 0 && t != 1) { P_ERR("unknown CATALOG_TYPE %d\n", t); FAIL(FCFC_ERR_CFG); }
+    if (t == 0) any_ascii = 1;
+  }
+  if (any_ascii && nfmt != ncat) { P_ERR("ASCII_FORMATTER is not set for every catalogue\n"); FAIL(FCFC_ERR_CFG); }
   nsel = split_list(kv_get("SELECTION"), &sels);
   /* COORD_CONVERT (survey): one flag for every catalogue or one each, with
    * the fiducial cosmology or a (redshift, distance) table
@@ -1138,13 +1374,19 @@ int main(int argc, char **argv) {
       memset(xp, 0, sizeof xp);
       memset(&xs, 0, sizeof xs);
       int ce = 0;
-      for (int d = 0; d < 3 && !ce; d++) ce = x_compile(pos[3 * k + d], &xp[d]);
-      if (!ce && nc == 4) ce = x_compile(wts[k], &xp[3]);
-      if (!ce && hs) ce = x_compile(sels[k], &xs);
+      const int fits = (ntype == 1) ? atoi(types[0]) == 1 : (ntype > k && atoi(types[k]) == 1);
+      fits_t ft;
+      memset(&ft, 0, sizeof ft);
+      if (fits) ce = fits_open(cats[k], &ft);
+      for (int d = 0; d < 3 && !ce; d++) ce = x_compile_named(pos[3 * k + d], &xp[d], ft.names, ft.ncol);
+      if (!ce && nc == 4) ce = x_compile_named(wts[k], &xp[3], ft.names, ft.ncol);
+      if (!ce && hs) ce = x_compile_named(sels[k], &xs, ft.names, ft.ncol);
       printf("Construct the grid for catalog '%c' ...\n", lab[k]);
       long skip = (k < nskip) ? atol(skips[k]) : 0;
       char cm = (k < ncmt && cmts[k][0]) ? cmts[k][0] : '\0';
-      if (!ce) ce = read_ascii(cats[k], skip, cm, fmts[k], xp, nc, hs ? &xs : NULL, &cat[k]);
+      if (!ce && fits) ce = read_fits(cats[k], &ft, xp, nc, hs ? &xs : NULL, &cat[k]);
+      else if (!ce) ce = read_ascii(cats[k], skip, cm, fmts[k], xp, nc, hs ? &xs : NULL, &cat[k]);
+      fits_close(&ft);
       for (int d = 0; d < 4; d++) free(xp[d].code);
       free(xs.code);
       if (ce) { ret = (ce == FCFC_ERR_AST) ? FCFC_ERR_CFG : ce; goto done; }

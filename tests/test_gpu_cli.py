@@ -232,14 +232,119 @@ def test_cli_resume_reference_binary_and_mismatches(tmp_path):
 
 
 def test_cli_refuses_unsupported(tmp_path):
-    """Bitwise operators and string columns, FITS/HDF5 catalogues and the
-    conversion in the box executable end with an error."""
+    """Bitwise operators, HDF5 catalogues and the conversion in the box
+    executable end with an error."""
     tmp = str(tmp_path)
     common = ["-f", "%lf %lf %lf", "-b", "1000", "-p", "DD", "-P", "o.bin", "--s-min", "0",
               "--s-max", "200", "--s-step", "5"]
     for extra in (["-i", "x.txt", "-x", "[${1}&1,${2},${3}]"],
-                  ["-i", "x.txt", "-x", "[$1,$2,$3]", "-T", "1"],
+                  ["-i", "x.txt", "-x", "[$1,$2,$3]", "-T", "2"],
                   ["-i", "x.txt", "-x", "[$1,$2,$3]", "--convert", "T"]):
         r = subprocess.run([os.path.join(BIN, "FCFC_2PT_BOX_GPU")] + extra + common,
                            capture_output=True, text=True, cwd=tmp, timeout=60)
         assert r.returncode != 0, extra
+
+
+def _card(key, val=None):
+    if val is None:
+        return key.ljust(80)
+    if isinstance(val, bool):
+        v = ("T" if val else "F").rjust(20)
+    elif isinstance(val, str):
+        v = ("'" + val.replace("'", "''").ljust(8) + "'").ljust(20)
+    else:
+        v = str(val).rjust(20)
+    return f"{key:<8}= {v}".ljust(80)
+
+
+def _write_fits(path, cols, image=False):
+    """A FITS file written from the standard: an empty primary HDU (or a
+    small image) and one BINTABLE of the columns (name, TFORM, values)."""
+    kinds = {"E": ">f4", "D": ">f8", "J": ">i4", "I": ">i2", "K": ">i8", "B": "u1"}
+    dt = []
+    for name, form, _ in cols:
+        r = int(form[:-1]) if len(form) > 1 else 1
+        dt.append((name, f"S{r}") if form[-1] == "A" else
+                  ((name, kinds[form[-1]], (r,)) if r > 1 else (name, kinds[form[-1]])))
+    rec = np.zeros(len(cols[0][2]), dtype=dt)
+    for name, _, v in cols:
+        rec[name] = v
+    def block(cards):
+        t = "".join(cards + [_card("END")])
+        return (t + " " * (-len(t) % 2880)).encode()
+    prim = [_card("SIMPLE", True), _card("BITPIX", 16 if image else 8), _card("NAXIS", 1 if image else 0)]
+    if image:
+        prim.append(_card("NAXIS1", 4))
+    prim.append(_card("EXTEND", True))
+    data = rec.tobytes()
+    ext = [_card("XTENSION", "BINTABLE"), _card("BITPIX", 8), _card("NAXIS", 2),
+           _card("NAXIS1", rec.dtype.itemsize), _card("NAXIS2", len(rec)), _card("PCOUNT", 0),
+           _card("GCOUNT", 1), _card("TFIELDS", len(cols))]
+    for k, (name, form, _) in enumerate(cols):
+        ext += [_card(f"TTYPE{k + 1}", name), _card(f"TFORM{k + 1}", form)]
+    with open(path, "wb") as f:
+        f.write(block(prim))
+        if image:
+            f.write(b"\0" * 2880)
+        f.write(block(ext))
+        f.write(data + b"\0" * (-len(data) % 2880))
+
+
+def test_cli_fits_catalogue_equals_ascii(tmp_path):
+    """CATALOG_TYPE 1: a FITS binary table gives the same pair-count file as
+    the same values in an ASCII file read with the matching formatter types
+    (E float, D double, J and I int, K long, B byte, an unused string
+    column); columns by name, without regard to case, in POSITION, WEIGHT
+    and an integer SELECTION.  The ASCII path is pinned to the upstream
+    executable (cli_box_types); the reference reads FITS through cfitsio,
+    which this image lacks, so FITS is pinned through that equivalence."""
+    tmp = str(tmp_path)
+    (x, y, z, w), _ = catalogs("box_clustered")[0]
+    g = np.random.default_rng(71)
+    n = len(x)
+    xf, zf = x.astype(np.float32), z.astype(np.float32)
+    ids = g.integers(0, 10, n).astype(np.int32)
+    flag = g.integers(-50, 50, n).astype(np.int16)
+    big = g.integers(-2**40, 2**40, n).astype(np.int64)
+    byte = g.integers(0, 256, n).astype(np.uint8)
+    _write_fits(os.path.join(tmp, "cat.fits"),
+                [("X", "E", xf), ("Y", "D", y), ("Z", "E", zf), ("ID", "J", ids), ("FLAG", "I", flag),
+                 ("BIG", "K", big), ("BYTE", "B", byte), ("NAME", "8A", np.full(n, b"obj")),
+                 ("W", "D", w), ("V", "2E", np.zeros((n, 2), np.float32))])
+    with open(os.path.join(tmp, "cat.txt"), "w") as f:
+        for r in range(n):
+            f.write(f"{float(xf[r])!r} {float(y[r])!r} {float(zf[r])!r} {ids[r]} {flag[r]} {big[r]} "
+                    f"{byte[r]} obj {float(w[r])!r}\n")
+    tail = ("BOX_SIZE = 1000\nBINNING_SCHEME = 1\nPAIR_COUNT = DD\nPAIR_COUNT_FILE = {d}/{o}\n"
+            "SEP_BIN_MIN = 0\nSEP_BIN_MAX = 200\nSEP_BIN_SIZE = 5\nMU_BIN_NUM = 10\n"
+            "OUTPUT_FORMAT = 0\nOVERWRITE = 2\n")
+    asc = ("CATALOG = {d}/cat.txt\nCATALOG_LABEL = D\n"
+           'ASCII_FORMATTER = "%f %lf %f %d %d %ld %d %s %lf"\n'
+           "POSITION = [$1, $2, ($3 + $4 / 3) % 1000]\nWEIGHT = $9 * 2\n"
+           "SELECTION = ($4 / 3 == 1 || $5 % 4 == 0) && $6 < 100000000000 && $7 > 3 && $9 > 0.4\n")
+    fits = ("CATALOG = {d}/cat.fits\nCATALOG_TYPE = 1\nCATALOG_LABEL = D\n"
+            "POSITION = [${x}, ${Y}, (${z} + ${Id} / 3) % 1000]\nWEIGHT = ${W} * 2\n"
+            "SELECTION = (${ID} / 3 == 1 || ${flag} % 4 == 0) && ${BIG} < 100000000000 && "
+            "${byte} > 3 && ${w} > 0.4\n")
+    for conf, out in ((asc, "a.bin"), (fits, "f.bin")):
+        r = _run(tmp, "FCFC_2PT_BOX", (conf + tail).replace("{d}", tmp).replace("{o}", out))
+        assert r.returncode == 0, r.stdout + r.stderr
+    ra, rf = O.read_pair_count(os.path.join(tmp, "a.bin")), O.read_pair_count(os.path.join(tmp, "f.bin"))
+    # the catalogues are identical, so are the weight sums; the weighted
+    # counts are FP64 sums whose order the GPU's atomics choose
+    assert ra["norm"] == rf["norm"] and ra["num"].tolist() == rf["num"].tolist()
+    assert 0 < ra["num"][0] < 2 * n
+    np.testing.assert_allclose(ra["cnt"], rf["cnt"], rtol=1e-12)
+    # a vector column, a missing name and a primary image are refused
+    for bad, msg in (("POSITION = [${V}, ${Y}, ${Z}]\n", "not a scalar"),
+                     ("POSITION = [${X}, ${Y}, ${nope}]\n", "no FITS column")):
+        conf = ("CATALOG = {d}/cat.fits\nCATALOG_TYPE = 1\nCATALOG_LABEL = D\n" + bad + tail)
+        r = _run(tmp, "FCFC_2PT_BOX", conf.replace("{d}", tmp).replace("{o}", "b.bin"))
+        assert r.returncode != 0 and msg in r.stderr, r.stdout + r.stderr
+    _write_fits(os.path.join(tmp, "img.fits"), [("X", "E", xf[:10]), ("Y", "E", xf[:10]),
+                                                ("Z", "E", xf[:10])], image=True)
+    conf = ("CATALOG = {d}/img.fits\nCATALOG_TYPE = 1\nCATALOG_LABEL = D\n"
+            "POSITION = [${X}, ${Y}, ${Z}]\n" + tail)
+    r = _run(tmp, "FCFC_2PT_BOX", conf.replace("{d}", tmp).replace("{o}", "c.bin"))
+    assert r.returncode != 0 and "not a binary table" in r.stderr, r.stdout + r.stderr
+
