@@ -63,7 +63,7 @@ def build(verbose=True):
     if os.path.exists(cli):
         for name, geom in (("FCFC_2PT_BOX_GPU", "0"), ("FCFC_2PT_GPU", "1")):
             out = os.path.join(BIN, name)
-            _run([CC] + C_FLAGS + ["-DFCFC_CLI_GEOM=" + geom, "-I" + INC, "-o", out, cli,
+            _run([CC] + C_FLAGS + ["-fopenmp", "-DFCFC_CLI_GEOM=" + geom, "-I" + INC, "-o", out, cli,
                                    "-L" + LIB, "-lfcfc_gpu", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
     return os.path.join(LIB, "libfcfc_gpu.so")
 

@@ -26,6 +26,9 @@
 #include <string.h>
 #include <strings.h>
 #include <sys/stat.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "fcfc_gpu.h"
 
 #ifndef FCFC_CLI_GEOM
@@ -476,7 +479,12 @@ static double x_eval(const xprog_t *x, const double *col, double *st) {
  * with l double; strings and the other types are columns that are read
  * past but cannot be used; %* conversions are not columns. */
 enum { FT_SKIP, FT_NULL, FT_STR, FT_INT, FT_LONG, FT_FLT, FT_DBL };
-typedef struct { char *fmt; int type; } fseg_t;
+/* fast: the piece is only blanks and one %d / %i / %ld / %li / %f / %lf
+ * (e, g, a alike) / %s conversion without width, suppressed or not, which
+ * is scanned with strtol / strtof / strtod -- the conversions sscanf itself
+ * applies (C99 7.19.6.2) -- instead of re-reading the format every row;
+ * 'i' when the base is detected from the prefix */
+typedef struct { char *fmt; int type; int fast; char conv; int base; } fseg_t;
 
 static void fsegs_free(fseg_t *f, int n) { for (int i = 0; i < n; i++) free(f[i].fmt); free(f); }
 
@@ -515,19 +523,100 @@ static int fmt_split(const char *fmt, fseg_t **out) {
     memcpy(t + k, "%n", 3);
     if (n == cap) { cap *= 2; f = realloc(f, sizeof(fseg_t) * cap); }
     f[n].fmt = t; f[n].type = type; n++;
+    /* the fast path applies without a width, with blanks only before the % */
+    int blanks = 1;
+    for (const char *q = seg; q < pc; q++) if (!isspace((unsigned char) *q)) { blanks = 0; break; }
+    const char *w = pc + 1 + supp;
+    f[n - 1].fast = blanks && !isdigit((unsigned char) *w) && (lm == 0 || lm == 'l') &&
+                    (strchr("dieEfFgGaA", cv) || (cv == 's' && lm == 0));
+    f[n - 1].conv = (cv == 's') ? 's' : (cv == 'd' || cv == 'i') ? 'd' : 'f';
+    f[n - 1].base = (cv == 'i') ? 0 : 10;
     seg = p;
   }
   *out = f;
   return n;
 }
 
+/* One row of an ASCII catalogue into the typed fields fv (the formatter's
+ * pieces, or whitespace/comma fields without a formatter).  0: the row is
+ * read; FCFC_ERR_ASCII: it does not match. */
+static int ascii_fields(const char *line, const fseg_t *fs, int nf, int nfield, xval_t *fv) {
+  if (!fs) {
+    const char *q = line;
+    for (int ti = 0; ti < nfield; ti++) {
+      while (*q == ' ' || *q == '\t' || *q == '\r' || *q == ',') q++;
+      char *t;
+      fv[ti].t = XT_DBL;
+      fv[ti].d = strtod(q, &t);
+      if (!*q) return FCFC_ERR_ASCII;
+      if (t == q) while (*t && !isspace((unsigned char) *t) && *t != ',') t++;
+      q = t;
+    }
+    return 0;
+  }
+  /* one conversion at a time (read_ascii.c:ascii_read_line) */
+  const char *q = line;
+  for (int i = 0; i < nf; i++) {
+    int n = 0, r, iv;
+    long lv;
+    float fv32;
+    double dv;
+    if (fs[i].fast) {
+      const char *b = q;
+      char *t;
+      while (isspace((unsigned char) *b)) b++;
+      if (fs[i].conv == 's') {
+        t = (char *) b;
+        while (*t && !isspace((unsigned char) *t)) t++;
+      }
+      else if (fs[i].conv == 'd') {
+        lv = strtol(b, &t, fs[i].base);
+        fv[i].t = XT_LONG; fv[i].l = (fs[i].type == FT_INT) ? (int) lv : lv;
+      }
+      else if (fs[i].type == FT_FLT) { fv32 = strtof(b, &t); fv[i].t = XT_DBL; fv[i].d = (double) fv32; }
+      else { dv = strtod(b, &t); fv[i].t = XT_DBL; fv[i].d = dv; }
+      if (t == b) return FCFC_ERR_ASCII;
+      q = t;
+      continue;
+    }
+    switch (fs[i].type) {
+      case FT_INT: r = sscanf(q, fs[i].fmt, &iv, &n); fv[i].t = XT_LONG; fv[i].l = iv; break;
+      case FT_LONG: r = sscanf(q, fs[i].fmt, &lv, &n); fv[i].t = XT_LONG; fv[i].l = lv; break;
+      case FT_FLT: r = sscanf(q, fs[i].fmt, &fv32, &n); fv[i].t = XT_DBL; fv[i].d = (double) fv32; break;
+      case FT_DBL: r = sscanf(q, fs[i].fmt, &dv, &n); fv[i].t = XT_DBL; fv[i].d = dv; break;
+      case FT_STR:
+        while (isspace((unsigned char) *q)) q++;
+        r = sscanf(q, fs[i].fmt, &n) + 1;
+        break;
+      default: r = sscanf(q, fs[i].fmt, &n) + 1; break;
+    }
+    if (r != 1 || n == 0) return FCFC_ERR_ASCII;
+    q += n;
+  }
+  return 0;
+}
+
+/* the rows one thread reads: growable coordinate arrays */
+typedef struct { double *v[4]; int64_t n, cap; long bad; int err; } rows_t;
+
+static void rows_push(rows_t *r, int nout, const double *v) {
+  if (r->n == r->cap) {
+    r->cap = r->cap ? 2 * r->cap : 4096;
+    for (int k = 0; k < nout; k++) r->v[k] = realloc(r->v[k], sizeof(double) * (size_t) r->cap);
+  }
+  for (int k = 0; k < nout; k++) r->v[k][r->n] = v[k];
+  r->n++;
+}
+
 /* Read an ASCII catalogue: nout expressions (x, y, z[, w]) per selected
  * row (read_ascii.c:read_ascii_data).  Columns are the formatter's
  * conversions that are not suppressed ($n: the n-th of them); without a
- * formatter, whitespace- or comma-separated fields read as double. */
+ * formatter, whitespace- or comma-separated fields read as double.  The
+ * file is read in blocks of whole lines, each block's lines split among
+ * the OpenMP threads and their rows appended in file order. */
 static int read_ascii(const char *fname, long skip, char cmt, const char *fmt,
     xprog_t *out, int nout, const xprog_t *sel, cat_t *c) {
-  FILE *fp = fopen(fname, "r");
+  FILE *fp = fopen(fname, "rb");
   if (!fp) { P_ERR("cannot open catalogue `%s'\n", fname); return FCFC_ERR_FILE; }
   fseg_t *fs = NULL;
   int nf = fmt ? fmt_split(fmt, &fs) : 0;
@@ -565,78 +654,119 @@ static int read_ascii(const char *fname, long skip, char cmt, const char *fmt,
   }
   if (!fmt) for (int cn = 1; cn <= maxcol; cn++) if (field[cn] + 1 > nfield) nfield = field[cn] + 1;
   if (fmt) nfield = nf;
-  xval_t *fv = malloc(sizeof(xval_t) * (nfield + 1)), *col = malloc(sizeof(xval_t) * (maxcol + 1));
-  xval_t *xst = malloc(sizeof(xval_t) * (depth + 1));
-  double *cold = malloc(sizeof(double) * (maxcol + 1)), *stack = malloc(sizeof(double) * (depth + 1));
-  size_t cap = 1 << 16;
-  c->x = malloc(sizeof(double) * cap); c->y = malloc(sizeof(double) * cap);
-  c->z = malloc(sizeof(double) * cap); c->w = (nout > 3) ? malloc(sizeof(double) * cap) : NULL;
+
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  rows_t *part = calloc((size_t) nth, sizeof(rows_t));
+  c->x = c->y = c->z = c->w = NULL;
   c->n = 0;
-  char *line = NULL; size_t lcap = 0;
-  long lno = 0;
-  int e = 0;
-  while (getline(&line, &lcap, fp) >= 0) {
-    lno++;
-    if (lno <= skip) continue;
-    char *p = line;
-    while (isspace((unsigned char) *p)) p++;
-    if (!*p) continue;
-    if (cmt && *p == cmt) continue;
-    if (fmt) {
-      /* one sscanf per conversion (read_ascii.c:ascii_read_line) */
-      const char *q = line;
-      for (int i = 0; i < nf && !e; i++) {
-        int n = 0, r, iv;
-        long lv;
-        float fv32;
-        double dv;
-        switch (fs[i].type) {
-          case FT_INT: r = sscanf(q, fs[i].fmt, &iv, &n); fv[i].t = XT_LONG; fv[i].l = iv; break;
-          case FT_LONG: r = sscanf(q, fs[i].fmt, &lv, &n); fv[i].t = XT_LONG; fv[i].l = lv; break;
-          case FT_FLT: r = sscanf(q, fs[i].fmt, &fv32, &n); fv[i].t = XT_DBL; fv[i].d = (double) fv32; break;
-          case FT_DBL: r = sscanf(q, fs[i].fmt, &dv, &n); fv[i].t = XT_DBL; fv[i].d = dv; break;
-          case FT_STR:
-            while (isspace((unsigned char) *q)) q++;
-            r = sscanf(q, fs[i].fmt, &n) + 1;
-            break;
-          default: r = sscanf(q, fs[i].fmt, &n) + 1; break;
+  int64_t cap = 0;
+  const size_t BLOCK = (size_t) 32 << 20;
+  char *buf = malloc(BLOCK + 1);
+  size_t have = 0;
+  long lines_before = 0;           /* lines of the file before this block */
+  int e = 0, eof = 0;
+  /* ASCII_SKIP: the first lines are not read */
+  for (int ch; lines_before < skip && (ch = fgetc(fp)) != EOF;) if (ch == '\n') lines_before++;
+  while (!e && !eof) {
+    const size_t want = BLOCK - have;
+    const size_t got = fread(buf + have, 1, want, fp);
+    have += got;
+    eof = got < want;
+    /* whole lines: up to the last newline (all of it at the end of file) */
+    size_t len = have;
+    if (!eof) {
+      while (len > 0 && buf[len - 1] != '\n') len--;
+      if (len == 0) { P_ERR("line too long in `%s'\n", fname); e = FCFC_ERR_ASCII; break; }
+    }
+    if (len == 0) break;
+    const char saved = buf[len];
+    buf[len] = '\0';
+    /* split the block among the threads at line starts */
+    size_t *cut = malloc(sizeof(size_t) * (size_t) (nth + 1));
+    long *nlines = calloc((size_t) nth, sizeof(long));
+    cut[0] = 0;
+    for (int t = 1; t < nth; t++) {
+      size_t p = len * (size_t) t / (size_t) nth;
+      if (p < cut[t - 1]) p = cut[t - 1];
+      while (p < len && p > 0 && buf[p - 1] != '\n') p++;
+      cut[t] = p;
+    }
+    cut[nth] = len;
+    for (int t = 0; t < nth; t++) { part[t].n = 0; part[t].err = 0; part[t].bad = 0; }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 1) num_threads(nth)
+#endif
+    for (int t = 0; t < nth; t++) {
+      xval_t *fv = malloc(sizeof(xval_t) * (size_t) (nfield + 1));
+      xval_t *col = malloc(sizeof(xval_t) * (size_t) (maxcol + 1));
+      xval_t *xst = malloc(sizeof(xval_t) * (size_t) (depth + 1));
+      double *cold = malloc(sizeof(double) * (size_t) (maxcol + 1));
+      double *stack = malloc(sizeof(double) * (size_t) (depth + 1));
+      long lno = 0;                    /* lines of this part so far */
+      for (size_t p = cut[t]; p < cut[t + 1] && !part[t].err;) {
+        char *line = buf + p;
+        char *nl = memchr(line, '\n', cut[t + 1] - p);
+        size_t next = nl ? (size_t) (nl - buf) + 1 : cut[t + 1];
+        if (nl) *nl = '\0';
+        p = next;
+        lno++;
+        const char *q = line;
+        while (isspace((unsigned char) *q)) q++;
+        if (!*q || (cmt && *q == cmt)) continue;
+        if (ascii_fields(line, fmt ? fs : NULL, nf, nfield, fv)) { part[t].err = FCFC_ERR_ASCII; part[t].bad = lno; break; }
+        for (int cn = 1; cn <= maxcol; cn++) {
+          if (!used[cn]) continue;
+          col[cn - 1] = fv[field[cn]];
+          cold[cn - 1] = xv_dbl(&col[cn - 1]);
         }
-        if (r != 1 || n == 0) e = FCFC_ERR_ASCII;
-        q += n;
+        if (sel) {
+          const int r = x_eval_sel(sel, col, xst);
+          if (r < 0) { part[t].err = FCFC_ERR_AST; part[t].bad = lno; break; }
+          if (!r) continue;
+        }
+        double v[4];
+        for (int k = 0; k < nout; k++) v[k] = x_eval(&out[k], cold, stack);
+        rows_push(&part[t], nout, v);
       }
-      if (e) { P_ERR("line %ld of `%s' does not match ASCII_FORMATTER\n", lno, fname); break; }
+      free(fv); free(col); free(xst); free(cold); free(stack);
+      nlines[t] = lno;
     }
-    else {
-      int ti = 0;
-      char *save;
-      for (char *t = strtok_r(p, " \t\r\n,", &save); t && ti < nfield; t = strtok_r(NULL, " \t\r\n,", &save), ti++) {
-        fv[ti].t = XT_DBL; fv[ti].d = strtod(t, NULL);
+    for (int t = 0; t < nth && !e; t++) {
+      if (part[t].err) {
+        part[t].bad += lines_before;
+        for (int u = 0; u < t; u++) part[t].bad += nlines[u];
+        if (part[t].err == FCFC_ERR_AST)
+          P_ERR("integer division by zero in SELECTION (line %ld of `%s')\n", part[t].bad, fname);
+        else P_ERR("line %ld of `%s' does not match ASCII_FORMATTER\n", part[t].bad, fname);
+        e = part[t].err;
+        break;
       }
-      if (ti < nfield) { P_ERR("line %ld of `%s' has too few columns\n", lno, fname); e = FCFC_ERR_ASCII; break; }
+      if (c->n + part[t].n > cap) {
+        while (c->n + part[t].n > cap) cap = cap ? 2 * cap : 65536;
+        c->x = realloc(c->x, sizeof(double) * (size_t) cap);
+        c->y = realloc(c->y, sizeof(double) * (size_t) cap);
+        c->z = realloc(c->z, sizeof(double) * (size_t) cap);
+        if (nout > 3) c->w = realloc(c->w, sizeof(double) * (size_t) cap);
+      }
+      double *dst[4] = {c->x, c->y, c->z, c->w};
+      for (int k = 0; k < nout; k++) memcpy(dst[k] + c->n, part[t].v[k], sizeof(double) * (size_t) part[t].n);
+      c->n += part[t].n;
     }
-    for (int cn = 1; cn <= maxcol; cn++) {
-      if (!used[cn]) continue;
-      col[cn - 1] = fv[field[cn]];
-      cold[cn - 1] = xv_dbl(&col[cn - 1]);
-    }
-    if (sel) {
-      const int r = x_eval_sel(sel, col, xst);
-      if (r < 0) { P_ERR("integer division by zero in SELECTION (line %ld of `%s')\n", lno, fname); e = FCFC_ERR_AST; break; }
-      if (!r) continue;
-    }
-    if ((size_t) c->n == cap) {
-      cap *= 2;
-      c->x = realloc(c->x, sizeof(double) * cap); c->y = realloc(c->y, sizeof(double) * cap);
-      c->z = realloc(c->z, sizeof(double) * cap);
-      if (c->w) c->w = realloc(c->w, sizeof(double) * cap);
-    }
-    c->x[c->n] = x_eval(&out[0], cold, stack);
-    c->y[c->n] = x_eval(&out[1], cold, stack);
-    c->z[c->n] = x_eval(&out[2], cold, stack);
-    if (c->w) c->w[c->n] = x_eval(&out[3], cold, stack);
-    c->n++;
+    for (int t = 0; t < nth; t++) lines_before += nlines[t];
+    free(cut); free(nlines);
+    buf[len] = saved;
+    memmove(buf, buf + len, have - len);
+    have -= len;
   }
-  free(line); free(field); free(used); free(col); free(cold); free(stack); free(fv); free(xst);
+  if (!c->x) {
+    c->x = malloc(sizeof(double)); c->y = malloc(sizeof(double)); c->z = malloc(sizeof(double));
+    if (nout > 3) c->w = malloc(sizeof(double));
+  }
+  for (int t = 0; t < nth; t++) for (int k = 0; k < 4; k++) free(part[t].v[k]);
+  free(part); free(buf); free(field); free(used);
   if (fs) fsegs_free(fs, nf);
   fclose(fp);
   if (e) return e;
@@ -1215,6 +1345,13 @@ int main(int argc, char **argv) {
   const int bintype = get_int("BINNING_SCHEME", 0);
   const int ofmt = get_int("OUTPUT_FORMAT", 0);
   const int overwrite = get_int("OVERWRITE", 0);
+  /* DATA_STRUCT: 0 (k-d tree) or 1 (ball tree) in the reference
+   * (load_conf.c:825-834); both give the same counts, so the choice has no
+   * effect here */
+  {
+    const int ds = get_int("DATA_STRUCT", 0);
+    if (ds != 0 && ds != 1) { P_ERR("invalid DATA_STRUCT: %d\n", ds); FAIL(FCFC_ERR_CFG); }
+  }
   const int prec = get_int("GPU_PRECISION", 0);
   const int want_wp = get_bool("PROJECTED_CF", 0);
 
