@@ -342,6 +342,8 @@ def survey_workload(args):
             for name, a, c, au in (("DD", D, None, True), ("DR", D, R, False), ("RR", R, None, True)):
                 log(f"autotune {p} {name}: {eng.autotune(a, c, b, isauto=au, withwt=True, precision=p)}")
 
+    rechk = [0.0]
+
     def step(p):
         out, ev, km = {}, 0.0, {}
         for name, a, c, au in (("DD", D, None, True), ("DR", D, R, False), ("RR", R, None, True)):
@@ -349,6 +351,8 @@ def survey_workload(args):
             st = eng.stats()
             ev += st["evals"]
             km[name] = st["kernel_ms"]
+            if p == prec:
+                rechk[0] += st["rechecked"]
         return out, ev, km
 
     for i in range(args.warmup):
@@ -378,6 +382,8 @@ def survey_workload(args):
         "kernel_ms": km, "evals_per_step": ev_tot / args.steps,
         "wp_first_bins": [float(v) for v in wp(xi, b.pbin_raw)[:5]],
     }
+    if prec == "fp32":
+        line["rechecked_fraction"] = rechk[0] / max(ev_tot * (1 + args.warmup / max(args.steps, 1)), 1)
     line["forms"] = {}
     for other in others:
         t2 = time.time()
