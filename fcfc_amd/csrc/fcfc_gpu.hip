@@ -254,7 +254,9 @@ struct FkWin {
   float e_s2, e_s2x2;         /* absolute window on s^2 (or s_perp^2), 2 e_s2 */
   float e_pi, e_pix2;         /* absolute window on pi, 2 e_pi                */
   float idle_x, idle_z;       /* FP32 coordinates of idle lanes             */
-  float mu_sh, mu_hi;         /* mu bin certain if fract(t - mu_sh) < mu_hi */
+  float mu_sh;                /* error bound of the FP32 mu value t         */
+  float mu_c;                 /* 129 - sh: the fixed-point t (fk_mu32)      */
+  unsigned mu_t16;            /* mu bin certain if its fraction is below    */
   float s2_floor;             /* pairs below this s^2 are re-evaluated      */
   float nmu_f;
   float m2_floor;             /* survey: pairs with |a + b|^2 below this    */
@@ -382,6 +384,7 @@ __device__ __forceinline__ int fk_hslot(int i, int ns, int nsh) {
   return col < ns ? col + ns * row : -1;
 }
 #define FK_NONE 0xFFFFFFFFu
+#define FK_NMU32 126          /* most mu bins of the FP32 (s, mu) form      */
 #define FK_DBL_EPS 2.220446049250313e-16
 #define FK_FAR 1.0e30          /* coordinates of idle lanes: every pair out */
 
@@ -450,16 +453,34 @@ __device__ __forceinline__ unsigned fk_pbin(double p, const FkBins &bn,
  * when m >= nmu^2 and bins it in mutab[m] = isqrt(m) (setup_cf.c:513-531).
  * Since isqrt(floor(q)) = floor(sqrt(q)) for q >= 0, the bin is floor(t),
  * t = sqrt(a / s2), and the drop rule is floor(t) >= nmu.
- * The caller evaluates t from hardware reciprocal square roots (relative
- * error below 2^-21 including the products) and passes the shifted value
- * t_sh = t - w, w = nmu 2^-20, formed by the fma of the last product.  When
- * t is farther than w from every integer, i.e. fract(t_sh) < 1 - 2w, then
- * floor(t_sh) is the bin (t_sh in [k, k + 1 - 2w) puts the exact t in
- * (k, k + 1)).  Otherwise, and for all s2 below s2_exact = s2max 2^-12
- * (which covers s2 < DBL_EPSILON and bounds t by nmu (1 + 2^-21) on the
- * fast path), the bin is settled exactly: k = rint(t_sh), which is within
- * 1/2 of t, is the bin if a - k^2 s2 >= 0 (one fma, exact sign; k^2 is
- * exact), else k - 1. */
+ *
+ * Fast path: x = nmu |dz| rsq(s2) (hardware reciprocal square root; |x - t|
+ * < D = nmu 2^-20 including the products) goes into one fma with the addend
+ * mu_c = 2^36 + 1 - sh, sh a multiple of 2^-16 with sh >= D + 2^-16.  The
+ * sum lies in [2^36, 2^37), where doubles are spaced 2^-16, so the fma
+ * rounds it once onto that grid and the low word of the result is the
+ * fixed-point v = round((x - sh + 1) 2^16): its high half is K =
+ * floor(x - sh) + 1 and its low half f the fraction.  When f < mu_t16 =
+ * floor(2^16 (1 - 2 sh)), x lies in [K - 1 + sh - 2^-17, K - 1 + 1 - sh) and
+ * the exact t strictly inside (K - 1, K): the bin is K - 1, and the
+ * histogram index plus nsh is K nsh + sb, one 16-bit multiply-add of the
+ * high half (fk_mad_hi16).  A t within sh of an integer (K = 0 included)
+ * fails the fraction test.  Those pairs, and all with s2 below s2_exact =
+ * s2max 2^-12 (which covers s2 < DBL_EPSILON and bounds t by nmu (1 +
+ * 2^-21) on the fast path), are settled exactly: t_sh = v 2^-16 - 1 (exact:
+ * the fma result minus 2^36 + 1) is within 1/2 of t, so k = rint(t_sh) is
+ * the bin if a - k^2 s2 >= 0 (one fma, exact sign; k^2 is exact), else
+ * k - 1. */
+#define FK_MU_C0 68719476737.0          /* 2^36 + 1 */
+
+/* hi16(v) * m + c (m < 2^16) in one instruction: the 16-bit multiply-add
+ * reading the high half of its first operand */
+__device__ __forceinline__ unsigned fk_mad_hi16(unsigned v, unsigned m, unsigned c) {
+  unsigned r;
+  asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]" : "=v"(r) : "v"(v), "s"(m), "v"(c));
+  return r;
+}
+
 __device__ __forceinline__ unsigned fk_mu_exact(double a, double s2, double t_sh,
     const FkBins &bn) {
   if (!(s2 >= FK_DBL_EPS)) return 0u;
@@ -483,12 +504,27 @@ __device__ __forceinline__ unsigned fk_mu_plain(double num, double s2, const FkB
   return (unsigned) fk_isqrt((int) m);
 }
 
-/* t_sh = t - w (negative t_sh: t below w, settled exactly as well) */
-__device__ __forceinline__ bool fk_mu_near(double t_sh, double s2, const FkBins &bn) {
-  return !(__builtin_amdgcn_fract(t_sh) < bn.mu_hi) || s2 < bn.s2_exact;
+/* The (s, mu) index plus nsh from x = anmu r = nmu |dz| rsq(s2) >= 0; fa()
+ * gives a = dz^2 nmu^2 for the exact path (evaluated there only).  Every
+ * index that leaves the fast path is at least nsh (K >= 1).  CAP: survey
+ * pairs, whose x has no bound near a = -b (|a + b|^2 cancels) and may be
+ * NaN there, have the value capped at K = nmu + 1 (not counted, as the
+ * reference drops mu >= 1). */
+template <bool CAP, class FA>
+__device__ __forceinline__ unsigned fk_smu_index(double anmu, double r, FA fa, double s2,
+    unsigned sb, const FkBins &bn) {
+  double tm = __builtin_fma(anmu, r, bn.mu_c);
+  if (CAP) tm = __builtin_fmin(tm, bn.mu_cmax);
+  const unsigned v = (unsigned) __builtin_bit_cast(unsigned long long, tm);
+  unsigned idx = fk_mad_hi16(v, (unsigned) bn.nsh, sb);
+  if (!((v & 0xFFFFu) < bn.mu_t16) || s2 < bn.s2_exact)
+    idx = sb + __umul24((unsigned) bn.nsh, fk_mu_exact(fa(), s2, tm - FK_MU_C0, bn) + 1u);
+  return idx;
 }
 
-/* Box pair, B already at its periodic image (b' = fl(b + D)). */
+/* Box pair, B already at its periodic image (b' = fl(b + D)).  Returns the
+ * histogram index, plus nsh for (s, mu) bins outside the plain form
+ * (fk_smu_index); >= ntot (+ nsh): not counted. */
 template <int BIN, bool HYB, bool MK = false, bool PLAIN = false>
 __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
     const FkBins &bn, const unsigned *stab, const unsigned *ptab,
@@ -505,10 +541,8 @@ __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
   const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
   if (BIN == FCFC_BIN_ISO) return sb;
   if (PLAIN) return sb + __umul24((unsigned) bn.nsh, fk_mu_plain(dz2, s2, bn));
-  const double t = __builtin_fma(__builtin_fabs(dz) * bn.nmu_d, __builtin_amdgcn_rsq(s2), -bn.mu_sh);
-  unsigned mb = (unsigned) (int) t;
-  if (fk_mu_near(t, s2, bn)) mb = fk_mu_exact(dz2 * bn.nmu2, s2, t, bn);
-  return sb + __umul24((unsigned) bn.nsh, mb);
+  return fk_smu_index<false>(__builtin_fabs(dz) * bn.nmu_d, __builtin_amdgcn_rsq(s2),
+                      [&] { return dz2 * bn.nmu2; }, s2, sb, bn);
 }
 
 /* Survey pair (src/fcfc/2pt/metric_common.c:283-460, FMA vector formulas):
@@ -531,11 +565,9 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
     const double s2 = s - t;
     const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
     if (PLAIN) return sb + __umul24((unsigned) bn.nsh, fk_mu_plain((d * d) / (t + s), s2, bn));
-    const double tm = __builtin_fma(__builtin_fabs(d) * bn.nmu_d,
-        __builtin_amdgcn_rsq(t + s) * __builtin_amdgcn_rsq(s2), -bn.mu_sh);
-    unsigned mb = (unsigned) (int) tm;
-    if (fk_mu_near(tm, s2, bn)) mb = fk_mu_exact(((d * d) / (t + s)) * bn.nmu2, s2, tm, bn);
-    return sb + __umul24((unsigned) bn.nsh, mb);
+    return fk_smu_index<true>(__builtin_fabs(d) * bn.nmu_d,
+                        __builtin_amdgcn_rsq(t + s) * __builtin_amdgcn_rsq(s2),
+                        [&] { return ((d * d) / (t + s)) * bn.nmu2; }, s2, sb, bn);
   }
   /* (s_perp, pi): pbin holds pi^2 edges; with s2min == 0 the reference has
    * no lower bound on s_perp^2 (a rounding-negative value lands in bin 0) */
@@ -597,7 +629,8 @@ __device__ unsigned fk_box_variant(int var, const double a[3], const double bo[3
 
 /* ---------------- FP32 pair evaluation with exactness windows ----------- */
 /* Coordinates relative to the A tile centre in FP32.  Returns the histogram
- * index (>= ntot: not counted) and sets ambig when the pair lies within the
+ * index (>= ntot: not counted; (s, mu) bins: the index plus 0x4301 nsh, see
+ * fk_mu32) and sets ambig when the pair lies within the
  * rounding bound of a bin edge that matters (it is then re-evaluated in
  * FP64 and the index is ignored).  Window test of s^2 (and pi): with
  * u = s2 + e, s2 is within e of an integer k exactly when fract(u) < 2e,
@@ -613,6 +646,22 @@ __device__ unsigned fk_box_variant(int var, const double a[3], const double bo[3
  * comparisons with subnormals are slow), so the test is one compare of the
  * cell itself: fract(u) < cell.  A pair whose s (or pi) is certainly out of
  * range reads a flagless sentinel. */
+/* FP32 mu bin floor(t) from x = anmu r (|x - t| < D above s2_floor, the
+ * cells below s2_floor carry FK_F_ALW: every pair there is re-evaluated;
+ * t <= nmu (1 + 2^-20) <= 126), in fixed point as in fk_mu_exact: the fma
+ * with mu_c = 129 - sh (sh a multiple of 2^-16, sh >= D + 2^-16) rounds
+ * once onto the grid 2^-16 of [128, 256), so the float's bits are
+ * 0x43000000 + round((x - sh + 1) 2^16): the high half is 0x4300 + K, K =
+ * floor(x - sh) + 1, the low half the fraction f.  f < mu_t16 = floor(2^16
+ * (1 - 2 sh)) puts the exact t strictly inside (K - 1, K); otherwise
+ * ambig_mu.  Returns the index plus 0x4301 nsh: (0x4300 + K) nsh + sb. */
+__device__ __forceinline__ unsigned fk_mu32(float anmu, float r, unsigned sb,
+    const FkBins &bn, const FkWin &wn, bool &ambig_mu) {
+  const unsigned v = __float_as_uint(__builtin_fmaf(anmu, r, wn.mu_c));
+  ambig_mu = !((v & 0xFFFFu) < wn.mu_t16);
+  return fk_mad_hi16(v, (unsigned) bn.nsh, sb);
+}
+
 template <int BIN>
 __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     const FkBins &bn, const FkWin &wn, const unsigned *stab,
@@ -639,18 +688,11 @@ __device__ __forceinline__ unsigned fk_box32(float dx, float dy, float dz,
     ambig = (amb | ambp) & !out;
     return sb + __umul24((unsigned) bn.nsh, pb);
   }
-  /* mu bin = floor(t), t = nmu |dz| / s; above s2_floor the FP32 error of
-   * t (the fma's rounding included) is below mu_sh (fp32_windows) and
-   * t <= nmu (1 + 2^-20); the cells below s2_floor carry FK_F_ALW (every
-   * pair there is re-evaluated).  With t_sh = t - mu_sh, fract(t_sh) <
-   * mu_hi <= 1 - 2 mu_sh puts the exact t strictly inside (k, k + 1),
-   * k = floor(t_sh).  A pair out of range in s whose t is near an integer
-   * is re-evaluated too (rare, and cheaper than testing sb here); idle
-   * lanes sit where t = 1/2. */
-  const float t = __builtin_fmaf(wn.nmu_f * __builtin_fabsf(dz), __builtin_amdgcn_rsqf(s2), -wn.mu_sh);
+  /* mu bin = floor(t), t = nmu |dz| / s (fk_mu32).  A pair out of range in
+   * s whose t is near an integer is re-evaluated too (rare, and cheaper than
+   * testing sb here); idle lanes sit where t = 1/2. */
   ambig = amb;                                 /* the caller ORs the two */
-  ambig_mu = !(__builtin_amdgcn_fractf(t) < wn.mu_hi);
-  return sb + __umul24((unsigned) bn.nsh, (unsigned) (int) t);
+  return fk_mu32(wn.nmu_f * __builtin_fabsf(dz), __builtin_amdgcn_rsqf(s2), sb, bn, wn, ambig_mu);
 }
 
 /* FP32 survey pair.  d = b - a from coordinates relative to the A tile
@@ -698,10 +740,9 @@ __device__ __forceinline__ unsigned fk_sv32(float dx, float dy, float dz, float 
   const float u = s2 + wn.e_s2;
   const unsigned e = stab[min((int) u, bn.scap32)];
   ambig = __builtin_amdgcn_fractf(u) < __uint_as_float(e);
-  const float t = __builtin_fmaf(wn.nmu_f * __builtin_fabsf(p),
-      __builtin_amdgcn_rsqf(m2) * __builtin_amdgcn_rsqf(s2), -wn.mu_sh);
-  ambig_mu = !(__builtin_amdgcn_fractf(t) < wn.mu_hi);
-  return (e & 0xFFFFu) + __umul24((unsigned) bn.nsh, (unsigned) (int) t);
+  return fk_mu32(wn.nmu_f * __builtin_fabsf(p),
+                 __builtin_amdgcn_rsqf(m2) * __builtin_amdgcn_rsqf(s2), e & 0xFFFFu, bn, wn,
+                 ambig_mu);
 }
 
 /* Exact FP64 re-evaluation of a pair flagged by the FP32 windows (rare: the
@@ -735,6 +776,7 @@ __device__ __forceinline__ unsigned fk_recheck(const FkBins &bn, const FkJob &J,
   const double ey = bo[1] + c3[1] * J.g.Ly;
   const double ez = bo[2] + c3[2] * J.g.Lz;
   unsigned idx = fk_box64<BIN, HYB, true>(ex - ax, ey - ay, ez - az, bn, stab, ptab, s2e, pe);
+  if (BIN == FCFC_BIN_SMU) idx -= (unsigned) bn.nsh;
   if (idx >= (unsigned) bn.ntoh) idx = FK_NONE;
   if (CEN && fk_census<BIN, HYB>(bn, J, ax, ay, az, bo, c3, idx, stab, ptab, s2e, pe))
     (*ncensus)++;
@@ -750,6 +792,7 @@ __device__ __forceinline__ unsigned fk_recheck_sv(const FkBins &bn, const FkJob 
   const double bs = (BIN == FCFC_BIN_ISO) ? 0.0 : J.B.s[jg];
   unsigned idx = fk_sv64<BIN, HYB, false, true>(J.A.x[p], J.A.y[p], J.A.z[p], as, J.B.x[jg],
                                                 J.B.y[jg], J.B.z[jg], bs, bn, stab, ptab, s2e, pe);
+  if (BIN == FCFC_BIN_SMU) idx -= (unsigned) bn.nsh;
   return idx >= (unsigned) bn.ntoh ? FK_NONE : idx;
 }
 
@@ -780,6 +823,11 @@ k_count(const FkArgs a) {
   const int wave = tid >> 6;
   const FkBins &bn = a.bins;
   const unsigned ntot = (unsigned) bn.ntoh;   /* LDS histogram slots */
+  /* the pair functions' (s, mu) indices carry an offset (fk_mu_exact,
+   * fk_mu32); indices below it are never counted */
+  const unsigned ioff = (BIN != FCFC_BIN_SMU || PLAIN) ? 0u
+                        : (F32 ? 0x4301u : 1u) * (unsigned) bn.nsh;
+  const unsigned ntot_o = ntot + ioff;
 
   /* ---- LDS carve-up (lds_plan on the host computes the same offsets) ---- */
   unsigned int *hist_u = (unsigned int *) (lds + a.off_hist);
@@ -878,6 +926,10 @@ k_count(const FkArgs a) {
   double since_flush = 0;
   unsigned int *myhist_u = hist_u + (wave % a.nh) * bn.ntoh;
   double *myhist_d = hist_d + (wave % a.nh) * bn.ntoh;
+  /* the same histograms addressed by the pair functions' offset indices */
+  unsigned int *ohist_u = myhist_u - ioff;
+  double *ohist_d = myhist_d - ioff;
+  const double *ohist_g = hist_d - ioff;
   __shared__ FkJob sJ;
   const FkJob &J = sJ;
   int cur = -1;
@@ -1101,14 +1153,14 @@ k_count(const FkArgs a) {
             idx = fk_sv64<BIN, HYB, PLAIN>(ax[r], ay[r], az[r], as[r], bx, by, bz, bs, bn,
                                            stab, ptab, s2e, pe);
           if (!F32 && SELF && !(jj > ai[r])) idx = FK_NONE;
-          if (idx < ntot && !queued) {
+          if (idx < ntot_o && !queued) {
             if (GRAD) {
-              const double gk = hist_d[idx];
+              const double gk = ohist_g[idx];
               gacc[r] += gk * wj;
               bsum += gk * aw[r];
             }
-            else if (WT && jwt) atomicAdd(myhist_d + idx, aw[r] * wj);
-            else atomicAdd(myhist_u + idx, 1u);
+            else if (WT && jwt) atomicAdd(ohist_d + idx, aw[r] * wj);
+            else atomicAdd(ohist_u + idx, 1u);
           }
         }
         if (GRAD && J.gB) {
@@ -1994,14 +2046,13 @@ static void fp32_windows(const fcfc_bins *b, const FkGrid &g, int zmaxc, double 
   double D = 1.25 * (nmu * 10 * u + (sqrt(3.0) * u * E * nmu + nmu * (u * E + tiny)) / sqrt(s2f))
              + 4 * nmu * u;
   if (sv) D = 1.25 * nmu * dmu + 4 * nmu * u;
-  /* mu_sh >= D; mu_hi <= 1 - 2 mu_sh (0 when the windows cover everything) */
-  float sh = (float) std::min(D, 0.5);
-  if ((double) sh < std::min(D, 0.5)) sh = nextafterf(sh, 1.0f);
-  const double hi = 1.0 - 2.0 * (double) sh;
-  float hf = (float) hi;
-  if ((double) hf > hi) hf = nextafterf(hf, 0.0f);
-  w.mu_sh = sh;
-  w.mu_hi = hf;
+  /* fixed-point mu (fk_mu32): sh >= D + 2^-16 on the grid 2^-16; the
+   * fraction threshold is 0 (every pair re-evaluated) when the windows
+   * cover everything */
+  const double sh = ceil((std::min(D, 0.5) + ldexp(1.0, -16)) * 65536.0) / 65536.0;
+  w.mu_sh = (float) sh;
+  w.mu_c = (float) (129.0 - sh);
+  w.mu_t16 = (unsigned) std::max(0.0, floor(65536.0 * (1.0 - 2.0 * sh)));
   w.s2_floor = (float) s2f;
   w.nmu_f = (float) b->nmu;
   w.m2_floor = (float) m2f;
@@ -2105,6 +2156,8 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   int m = mode;
   const bool zb = zb_fits(b);
   if (m == MODE_C32 && !zb) m = MODE_C64;                      /* FP32 tables are zero-based */
+  /* the FP32 fixed-point mu value holds t <= nmu (1 + 2^-20) below 127 */
+  if (m == MODE_C32 && b->bintype == FCFC_BIN_SMU && b->nmu > FK_NMU32) m = MODE_C64;
   int hyb = (b->tabtype || !zb) ? 1 : 0;
   const int need_s = (geom == FCFC_GEOM_SURVEY && b->bintype != FCFC_BIN_ISO);
   int anywt = 0;
@@ -2303,8 +2356,15 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   fb.win2s = win2s;
   fb.win2p = win2p;
   fb.pskip32 = (b->bintype == FCFC_BIN_SPI) ? (float) (fb.pmax + 2.0) : 3.0e38f;
-  fb.mu_sh = ldexp((double) b->nmu, -20);
-  fb.mu_hi = 1.0 - 2.0 * fb.mu_sh;
+  {
+    /* fixed-point mu (fk_mu_exact): sh >= nmu 2^-20 + 2^-16 on the grid
+     * 2^-16, the addend 2^36 + 1 - sh exact */
+    const double sh = ceil((ldexp((double) b->nmu, -20) + ldexp(1.0, -16)) * 65536.0) / 65536.0;
+    fb.mu_c = ldexp(1.0, 36) + 1.0 - sh;
+    fb.mu_cmax = ldexp(1.0, 36) + b->nmu + 1.5;
+    fb.mu_t16 = (unsigned) std::max(0.0, floor(65536.0 * (1.0 - 2.0 * sh)));
+    fb.mu_pad = 0;
+  }
   fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
   a.tiles = tiles; a.ntiles = ntiles;
   a.queue = dqueue;

@@ -52,8 +52,10 @@ struct FkBins {
   double pmin, pmax;         /* pi range (box) or pi^2 range (survey)       */
   double nmu2;               /* nmu * nmu as double                          */
   double nmu_d;              /* nmu as double                                */
-  double mu_sh, mu_hi;       /* mu bin settled exactly when t - mu_sh has a  */
-                             /* fraction >= mu_hi = 1 - 2 mu_sh              */
+  double mu_c;               /* 2^36 + 1 - mu_sh: the fixed-point mu value  */
+  double mu_cmax;            /* 2^36 + nmu + 1.5: its ceiling (survey)       */
+  unsigned mu_t16, mu_pad;   /* mu bin settled exactly when the fraction     */
+                             /* (low 16 bits) of that value is >= mu_t16     */
   double s2_exact;           /* ... or s^2 is below this value               */
   double s2skip, pskip;      /* box: a pair at s^2 >= s2skip or pi >= pskip  */
   float s2skip32, pskip32;   /* is out of range (FP64 / FP32 values)         */

@@ -137,7 +137,13 @@ const char *fcfc_gpu_last_error(const fcfc_gpu_ctx *ctx);
 #define FCFC_GPU_PREC_FP64    0x00u  /* FP64 distances; mu bin from the
                                         reciprocal square root, settled
                                         exactly near its edges             */
-#define FCFC_GPU_PREC_FP32    0x10u  /* FP32 distances + FP64 edge recheck  */
+#define FCFC_GPU_PREC_FP32    0x10u  /* FP32 distances + FP64 edge recheck
+                                        (same counts as FP64; launches it
+                                        cannot serve run the FP64 form:
+                                        a lower s^2 (or pi) edge far above
+                                        zero, windows wider than half a
+                                        table cell, tables too long for
+                                        LDS, (s,mu) with nmu > 126)         */
 #define FCFC_GPU_PREC_PLAIN   0x80u  /* FP64, the reference's per-pair
                                         arithmetic transcribed directly (an
                                         IEEE division per (s,mu) pair); the
