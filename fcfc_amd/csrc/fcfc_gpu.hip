@@ -566,7 +566,7 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
     const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
     if (PLAIN) return sb + __umul24((unsigned) bn.nsh, fk_mu_plain((d * d) / (t + s), s2, bn));
     return fk_smu_index<true>(__builtin_fabs(d) * bn.nmu_d,
-                        __builtin_amdgcn_rsq(t + s) * __builtin_amdgcn_rsq(s2),
+                        __builtin_amdgcn_rsq((t + s) * s2),
                         [&] { return ((d * d) / (t + s)) * bn.nmu2; }, s2, sb, bn);
   }
   /* (s_perp, pi): pbin holds pi^2 edges; with s2min == 0 the reference has
@@ -741,7 +741,7 @@ __device__ __forceinline__ unsigned fk_sv32(float dx, float dy, float dz, float 
   const unsigned e = stab[min((int) u, bn.scap32)];
   ambig = __builtin_amdgcn_fractf(u) < __uint_as_float(e);
   return fk_mu32(wn.nmu_f * __builtin_fabsf(p),
-                 __builtin_amdgcn_rsqf(m2) * __builtin_amdgcn_rsqf(s2), e & 0xFFFFu, bn, wn,
+                 __builtin_amdgcn_rsqf(m2 * s2), e & 0xFFFFu, bn, wn,
                  ambig_mu);
 }
 
@@ -1950,8 +1950,8 @@ static kfn_t pick_kernel(int bintype, int geom, int wt, int mode, int hyb, int R
  * |m|^2, 4u of the reciprocal, 2u of the products) plus the reference's
  * terms, the s_perp^2 window the sum of the s^2 and pi^2 windows plus the
  * rounding of the difference, and the mu window (t = nmu |p| / (|m| s))
- * gets e_pi / s plus half the relative errors of s^2 and |m|^2 and 4u for
- * each reciprocal square root. */
+ * gets e_pi / s plus half the relative errors of s^2 and |m|^2 and 8u for
+ * the reciprocal square root of |m|^2 s^2 and that product. */
 static double fp32_extent(const FkGrid &g, int zmaxc) {
   return std::max(g.wx, std::max(g.wy, (zmaxc + 1) * g.wz));
 }

@@ -274,10 +274,13 @@ def test_large_box_small_bins(eng, bt):
         _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=True, precision=prec), refw, True)
 
 
-@pytest.mark.parametrize("nmu", [255, 3])
+@pytest.mark.parametrize("nmu", [255, 127, 126, 3])
 def test_mu_bins_many_and_few(eng, cats, nmu):
     """The finest mu binning the reference allows (MAX_NMU = 255,
-    define_comm.h:148) and a coarse one: the mu windows scale with nmu."""
+    define_comm.h:148) and a coarse one: the mu windows scale with nmu.
+    126 is the most mu bins the FP32 form's fixed-point mu value holds (it
+    re-evaluates some pairs in FP64); with more the FP32 request runs the
+    FP64 form (nothing re-evaluated), with the same counts."""
     (x, y, z, w), (xr, yr, zr) = cats
     b = _bins(F.GEOM_BOX, F.BIN_SMU, nmu=nmu)
     d = F.Catalog(b, x, y, z, w)
@@ -290,6 +293,8 @@ def test_mu_bins_many_and_few(eng, cats, nmu):
     for prec in ("fp64", "fp32", "plain"):
         print(f"case nmu={nmu} prec={prec}", flush=True)
         _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=False, precision=prec), ref, False, cen)
+        if prec == "fp32":
+            assert (eng.stats()["rechecked"] > 0) == (nmu <= 126)
         _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), refa, False, cena)
 
 
