@@ -500,6 +500,10 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="start with a 1 GiB device-to-device copy (calibrates HBM byte "
                          "counters of a profiler run)")
+    ap.add_argument("--shape", default=None, metavar="CD,ZD,T",
+                    help="c2 workload: fix the grid/tile shape of every count (cells per "
+                         "separation, z-cells per cell, A objects per tile) instead of "
+                         "autotuning, so that separate profiler runs see one shape")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default grid/tile shapes (by default the autotuner picks "
                          "them for each count and form before timing; its results are cached "
@@ -543,9 +547,13 @@ def main():
     log(f"workload ready in {time.time() - t0:.1f} s: D={D.n} R={R.n}")
 
     eng = F.Engine(0)
+    if args.shape:
+        cd, zd, tl = (int(v) for v in args.shape.split(","))
+        eng.set_tuning(cell_div=cd, tile=tl, zdiv=zd)
+        log(f"shape fixed: cells per separation {cd}, z-cells per cell {zd}, tile {tl}")
 
     def tune(p):
-        if args.no_autotune:
+        if args.no_autotune or args.shape:
             return
         for name, a, c, au, wt in (("DD", D, None, True, True), ("DR", D, R, False, True),
                                    ("RR", R, None, True, False)):

@@ -2755,10 +2755,12 @@ extern "C" int fcfc_gpu_plan_destroy(fcfc_gpu_plan *P) {
 
 /* Project-written autotuner: for jobs like (a, b, bins, flags) try grid and
  * tile shapes (cells per maximum separation, z-cells per cell, A objects
- * per tile), timing each on the first eighth of the tile list (a slab of the
- * box at the true density) and extrapolating to all tiles (counts of at most
- * 4096 tiles are timed whole).  Coordinate
- * descent over cell_div, then zdiv, then tile.  The winner is cached in the
+ * per tile), timing each on the first part of the tile list (a slab of the
+ * box at the true density: an eighth, a sixteenth beyond 16384 tiles) and
+ * extrapolating to all tiles (counts of at most 4096 tiles are timed
+ * whole).  Every pair of cell_div and zdiv is timed (the best zdiv depends
+ * on cell_div: a descent over one, then the other, missed the best pair of
+ * a cross count), then the tile size.  The winner is cached in the
  * context for this geometry / binning / precision and density bucket and
  * used by later counts whose shape is left automatic. */
 extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
@@ -2796,7 +2798,7 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     const int nt = c->last_ntiles;
     /* small counts (a few thousand tiles, milliseconds) are timed whole: an
      * eighth of them is too few tiles to rank shapes */
-    c->tile_cap = (nt <= 4096) ? 0 : std::max(1, nt / 8);
+    c->tile_cap = (nt <= 4096) ? 0 : std::max(1, nt / (nt > 16384 ? 16 : 8));
     e = run();
     if (e) return e;
     *est = c->stats.kernel_ms * (c->tile_cap ? (double) nt / (double) c->tile_cap : 1.0);
@@ -2805,16 +2807,14 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
   };
   int e = 0;
   const int cds[] = {2, 3, 4, 6};
-  for (int cd : cds) {
-    double est;
-    if ((e = measure(cd, 8, 0, &est))) break;
-    if (est < bestms) { bestms = est; bcd = cd; bT = last_T; }
-  }
   const int zds[] = {1, 2, 4, 8, 16};
-  for (int zd : zds) {
-    double est;
-    if (e || (e = measure(bcd, zd, 0, &est))) break;
-    if (est < bestms) { bestms = est; bzd = zd; bT = last_T; }
+  for (int cd : cds) {
+    for (int zd : zds) {
+      double est;
+      if ((e = measure(cd, zd, 0, &est))) break;
+      if (est < bestms) { bestms = est; bcd = cd; bzd = zd; bT = last_T; }
+    }
+    if (e) break;
   }
   /* the hybrid gradient kernel holds one object per lane: tiles of 2 FK_NT
    * objects would run as FK_NT, so only that size is timed */
