@@ -509,23 +509,30 @@ __device__ __forceinline__ unsigned fk_mu_plain(double num, double s2, const FkB
  * index that leaves the fast path is at least nsh (K >= 1).  CAP: survey
  * pairs, whose x has no bound near a = -b (|a + b|^2 cancels) and may be
  * NaN there, have the value capped at K = nmu + 1 (not counted, as the
- * reference drops mu >= 1). */
-template <bool CAP, class FA>
+ * reference drops mu >= 1).  MUT: sb is the raw cell of an integer table
+ * whose high half holds the fraction threshold, mu_t16, or 0 in the cells
+ * below s2_exact (whose pairs all take the exact path), so one compare of
+ * the two low halves replaces the threshold and s2_exact compares; the
+ * threshold then rides in the index too, which grows by mu_t16 << 16. */
+template <bool CAP, bool MUT, class FA>
 __device__ __forceinline__ unsigned fk_smu_index(double anmu, double r, FA fa, double s2,
     unsigned sb, const FkBins &bn) {
   double tm = __builtin_fma(anmu, r, bn.mu_c);
   if (CAP) tm = __builtin_fmin(tm, bn.mu_cmax);
   const unsigned v = (unsigned) __builtin_bit_cast(unsigned long long, tm);
   unsigned idx = fk_mad_hi16(v, (unsigned) bn.nsh, sb);
-  if (!((v & 0xFFFFu) < bn.mu_t16) || s2 < bn.s2_exact)
-    idx = sb + __umul24((unsigned) bn.nsh, fk_mu_exact(fa(), s2, tm - FK_MU_C0, bn) + 1u);
+  const bool exact = MUT ? !((v & 0xFFFFu) < (sb >> 16))
+                         : (!((v & 0xFFFFu) < bn.mu_t16) || s2 < bn.s2_exact);
+  if (exact)
+    idx = (sb & 0xFFFFu) + __umul24((unsigned) bn.nsh, fk_mu_exact(fa(), s2, tm - FK_MU_C0, bn) + 1u)
+          + (MUT ? bn.mu_t16 << 16 : 0u);
   return idx;
 }
 
 /* Box pair, B already at its periodic image (b' = fl(b + D)).  Returns the
- * histogram index, plus nsh for (s, mu) bins outside the plain form
- * (fk_smu_index); >= ntot (+ nsh): not counted. */
-template <int BIN, bool HYB, bool MK = false, bool PLAIN = false>
+ * histogram index, plus nsh (and mu_t16 << 16 with MUT) for (s, mu) bins
+ * outside the plain form (fk_smu_index); >= ntot (+ that): not counted. */
+template <int BIN, bool HYB, bool MK = false, bool PLAIN = false, bool MUT = false>
 __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
     const FkBins &bn, const unsigned *stab, const unsigned *ptab,
     const double *s2e, const double *pe) {
@@ -541,7 +548,7 @@ __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
   const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
   if (BIN == FCFC_BIN_ISO) return sb;
   if (PLAIN) return sb + __umul24((unsigned) bn.nsh, fk_mu_plain(dz2, s2, bn));
-  return fk_smu_index<false>(__builtin_fabs(dz) * bn.nmu_d, __builtin_amdgcn_rsq(s2),
+  return fk_smu_index<false, MUT>(__builtin_fabs(dz) * bn.nmu_d, __builtin_amdgcn_rsq(s2),
                       [&] { return dz2 * bn.nmu2; }, s2, sb, bn);
 }
 
@@ -549,7 +556,7 @@ __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
  * t = 2 a.b, s = |a|^2 + |b|^2, d = |a|^2 - |b|^2, pi^2 = d^2 / (t + s),
  * s^2 = s - t, s_perp^2 = (s - t) - pi^2.  (s, mu) bins need pi^2 only in
  * the exact branch of the mu bin: sqrt(pi^2) = |d| / sqrt(t + s). */
-template <int BIN, bool HYB, bool PLAIN = false, bool MK = false>
+template <int BIN, bool HYB, bool PLAIN = false, bool MK = false, bool MUT = false>
 __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
     double as, double bx, double by, double bz, double bs, const FkBins &bn,
     const unsigned *stab, const unsigned *ptab, const double *s2e, const double *pe) {
@@ -565,7 +572,7 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
     const double s2 = s - t;
     const unsigned sb = fk_sbin<HYB, MK>(s2, bn, stab, s2e);
     if (PLAIN) return sb + __umul24((unsigned) bn.nsh, fk_mu_plain((d * d) / (t + s), s2, bn));
-    return fk_smu_index<true>(__builtin_fabs(d) * bn.nmu_d,
+    return fk_smu_index<true, MUT>(__builtin_fabs(d) * bn.nmu_d,
                         __builtin_amdgcn_rsq((t + s) * s2),
                         [&] { return ((d * d) / (t + s)) * bn.nmu2; }, s2, sb, bn);
   }
@@ -824,9 +831,12 @@ k_count(const FkArgs a) {
   const FkBins &bn = a.bins;
   const unsigned ntot = (unsigned) bn.ntoh;   /* LDS histogram slots */
   /* the pair functions' (s, mu) indices carry an offset (fk_mu_exact,
-   * fk_mu32); indices below it are never counted */
+   * fk_mu32); indices below it are never counted.  MUT: the FP64 integer
+   * table cells carry the mu fraction threshold (fk_smu_index) */
+  constexpr bool MUT = (BIN == FCFC_BIN_SMU && !F32 && !PLAIN && !HYB);
   const unsigned ioff = (BIN != FCFC_BIN_SMU || PLAIN) ? 0u
-                        : (F32 ? 0x4301u : 1u) * (unsigned) bn.nsh;
+                        : F32 ? 0x4301u * (unsigned) bn.nsh
+                        : (unsigned) bn.nsh + (MUT ? bn.mu_t16 << 16 : 0u);
   const unsigned ntot_o = ntot + ioff;
 
   /* ---- LDS carve-up (lds_plan on the host computes the same offsets) ---- */
@@ -908,9 +918,13 @@ k_count(const FkArgs a) {
       raw[i] = (th << 16) | (v & 0xFFFFu);
     };
     const double s2f = (F32 && BIN == FCFC_BIN_SMU) ? (double) a.jobs[0].win.s2_floor : 0.0;
-    for (int i = tid; i < a.stab_c0 + bn.stab_len + 2; i += FK_NT)
-      put(stab_raw, i, entry(bn.stab, bn.stab_len, bn.ns, i - a.stab_c0, bn.soff, s2f, bn.s2min),
-          bn.win2s);
+    for (int i = tid; i < a.stab_c0 + bn.stab_len + 2; i += FK_NT) {
+      unsigned v = entry(bn.stab, bn.stab_len, bn.ns, i - a.stab_c0, bn.soff, s2f, bn.s2min);
+      /* MUT (zero-based: entry i holds (int) s2 = i): the mu fraction
+       * threshold, 0 where the cell reaches below s2_exact */
+      if (MUT && !((double) i < bn.s2_exact)) v |= bn.mu_t16 << 16;
+      put(stab_raw, i, v, bn.win2s);
+    }
     if (BIN == FCFC_BIN_SPI)
       for (int i = tid; i < a.ptab_c0 + bn.ptab_len + 2; i += FK_NT)
         put(ptab_raw, i, entry(bn.ptab, bn.ptab_len, bn.np, i - a.ptab_c0, bn.poff, 0.0, bn.pmin),
@@ -1147,11 +1161,11 @@ k_count(const FkArgs a) {
             queued = amb | ambm | ambx;
           }
           else if (GEOM == FCFC_GEOM_BOX)
-            idx = fk_box64<BIN, HYB, false, PLAIN>(bx - ax[r], by - ay[r], bz - az[r], bn, stab,
-                                                    ptab, s2e, pe);
+            idx = fk_box64<BIN, HYB, false, PLAIN, MUT>(bx - ax[r], by - ay[r], bz - az[r], bn,
+                                                         stab, ptab, s2e, pe);
           else
-            idx = fk_sv64<BIN, HYB, PLAIN>(ax[r], ay[r], az[r], as[r], bx, by, bz, bs, bn,
-                                           stab, ptab, s2e, pe);
+            idx = fk_sv64<BIN, HYB, PLAIN, false, MUT>(ax[r], ay[r], az[r], as[r], bx, by, bz,
+                                                       bs, bn, stab, ptab, s2e, pe);
           if (!F32 && SELF && !(jj > ai[r])) idx = FK_NONE;
           if (idx < ntot_o && !queued) {
             if (GRAD) {
@@ -2363,6 +2377,12 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     fb.mu_c = ldexp(1.0, 36) + 1.0 - sh;
     fb.mu_cmax = ldexp(1.0, 36) + b->nmu + 1.5;
     fb.mu_t16 = (unsigned) std::max(0.0, floor(65536.0 * (1.0 - 2.0 * sh)));
+    /* the FP64 integer-table kernels add mu_t16 << 16 to their (s, mu)
+     * indices (fk_smu_index): keep the largest, (nmu + 1) nsh + 0xFFFF +
+     * (mu_t16 << 16), below 2^32 (a lower threshold only sends more pairs
+     * to the exact path) */
+    const unsigned long long room = (0xFFFF0000ull - (unsigned long long) (b->nmu + 1) * fb.nsh) >> 16;
+    fb.mu_t16 = (unsigned) std::min<unsigned long long>(fb.mu_t16, room);
     fb.mu_pad = 0;
   }
   fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
