@@ -256,7 +256,7 @@ struct FkWin {
   float idle_x, idle_z;       /* FP32 coordinates of idle lanes             */
   float mu_sh;                /* error bound of the FP32 mu value t         */
   float mu_c;                 /* 129 - sh: the fixed-point t (fk_mu32)      */
-  unsigned mu_t16;            /* mu bin certain if its fraction is below    */
+  unsigned mu_t16;            /* mu bin certain if t's fraction is below it */
   float s2_floor;             /* pairs below this s^2 are re-evaluated      */
   float nmu_f;
   float m2_floor;             /* survey: pairs with |a + b|^2 below this    */
