@@ -940,10 +940,13 @@ k_count(const FkArgs a) {
   double since_flush = 0;
   unsigned int *myhist_u = hist_u + (wave % a.nh) * bn.ntoh;
   double *myhist_d = hist_d + (wave % a.nh) * bn.ntoh;
-  /* the same histograms addressed by the pair functions' offset indices */
-  unsigned int *ohist_u = myhist_u - ioff;
-  double *ohist_d = myhist_d - ioff;
-  const double *ohist_g = hist_d - ioff;
+  /* byte offsets in lds of these histograms, minus the pair functions'
+   * index offset times the slot size: lds + (o + idx * size) is slot
+   * idx - ioff (unsigned arithmetic, modulo 2^32; in range for every
+   * counted index) */
+  const unsigned ohist_u = (unsigned) a.off_hist + 4u * ((unsigned) (wave % a.nh) * ntot - ioff);
+  const unsigned ohist_d = (unsigned) a.off_hist + 8u * ((unsigned) (wave % a.nh) * ntot - ioff);
+  const unsigned ohist_g = (unsigned) a.off_hist - 8u * ioff;
   __shared__ FkJob sJ;
   const FkJob &J = sJ;
   int cur = -1;
@@ -1169,12 +1172,12 @@ k_count(const FkArgs a) {
           if (!F32 && SELF && !(jj > ai[r])) idx = FK_NONE;
           if (idx < ntot_o && !queued) {
             if (GRAD) {
-              const double gk = ohist_g[idx];
+              const double gk = *(const double *) (lds + (ohist_g + 8u * idx));
               gacc[r] += gk * wj;
               bsum += gk * aw[r];
             }
-            else if (WT && jwt) atomicAdd(ohist_d + idx, aw[r] * wj);
-            else atomicAdd(ohist_u + idx, 1u);
+            else if (WT && jwt) atomicAdd((double *) (lds + (ohist_d + 8u * idx)), aw[r] * wj);
+            else atomicAdd((unsigned int *) (lds + (ohist_u + 4u * idx)), 1u);
           }
         }
         if (GRAD && J.gB) {
