@@ -1226,6 +1226,9 @@ static void usage(void) {
 #define FAIL(code) do { ret = (code); goto done; } while (0)
 
 int main(int argc, char **argv) {
+  /* progress lines leave as they are printed, also into a pipe: a caller
+   * that captures them sees every stage that has finished */
+  setvbuf(stdout, NULL, _IOLBF, 0);
   int ret = 0;
   const char *conf_file = DEFAULT_CONF;
   int conf_given = 0;
@@ -1269,7 +1272,9 @@ int main(int argc, char **argv) {
   memset(&bins, 0, sizeof bins);
   cat_t *cat = NULL;
   double **cnt = NULL, **ncnt = NULL, *norm = NULL;
-  int *weighted = NULL, *comp_pc = NULL, *cnvt = NULL, any_cnvt = 0;
+  int *weighted = NULL, *comp_pc = NULL, *cnvt = NULL, any_cnvt = 0, *has_wt = NULL;
+  char (*pclab)[3] = NULL;
+  int *pci[2] = {NULL, NULL};
   char **sels = NULL;
   int nsel = 0;
   const char *zfile = NULL;
@@ -1278,6 +1283,8 @@ int main(int argc, char **argv) {
 
   ncat = split_list(kv_get("CATALOG"), &cats);
   if (ncat < 1) { P_ERR("CATALOG is not set\n"); return FCFC_ERR_CFG; }
+  /* labels are single letters A-Z (load_conf.c:665-690) */
+  if (ncat > 26) { P_ERR("there can be at most 26 catalogs set via CATALOG\n"); free_list(cats, ncat); return FCFC_ERR_CFG; }
   nlab = split_list(kv_get("CATALOG_LABEL"), &labels);
   npos = split_list(kv_get("POSITION"), &pos);
   nwt = split_list(kv_get("WEIGHT"), &wts);
@@ -1428,8 +1435,7 @@ int main(int argc, char **argv) {
   ncnt = calloc(npc, sizeof(double *));
   norm = calloc(npc, sizeof(double));
   weighted = calloc(npc, sizeof(int));
-  char (*pclab)[3] = calloc(npc, 3);
-  int *pci[2];
+  pclab = calloc(npc, 3);
   pci[0] = calloc(npc, sizeof(int)); pci[1] = calloc(npc, sizeof(int));
   for (int i = 0; i < npc; i++) {
     if (strlen(pcs[i]) != 2) { P_ERR("invalid PAIR_COUNT `%s'\n", pcs[i]); FAIL(FCFC_ERR_CFG); }
@@ -1484,7 +1490,7 @@ int main(int argc, char **argv) {
   if (bintype == FCFC_BIN_SMU && npole && ncf && nmpo != ncf) {
     P_ERR("MULTIPOLE_FILE must have as many entries as CF_ESTIMATOR\n"); FAIL(FCFC_ERR_CFG);
   }
-  int *has_wt = calloc(ncat, sizeof(int));
+  has_wt = calloc(ncat, sizeof(int));
   for (int k = 0; k < ncat; k++) has_wt[k] = (k < nwt && wts[k][0] && strcmp(wts[k], "1"));
   if (fcfc_gpu_init(&ctx, 0)) { P_ERR("no usable GPU\n"); FAIL(FCFC_ERR_UNKNOWN); }
 
@@ -1690,6 +1696,7 @@ done:
   free(cat);
   if (cnt) for (int i = 0; i < npc; i++) { free(cnt[i]); free(ncnt[i]); }
   free(cnt); free(ncnt); free(norm); free(weighted); free(comp_pc); free(cnvt);
+  free(has_wt); free(pclab); free(pci[0]); free(pci[1]);
   free_list(sels, nsel);
   free_list(cats, ncat); free_list(labels, nlab); free_list(pos, npos); free_list(wts, nwt);
   free_list(fmts, nfmt); free_list(pcs, npc); free_list(pcout, npco); free_list(cfs, ncf);

@@ -1724,7 +1724,8 @@ __global__ void k_minmax(int n, const double *__restrict__ x, const double *__re
                                      : fmax(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = 0;   /* unused slot */
+  /* out[8 + 6 b + q]: block b's minima (q < 3) and maxima (q >= 3); the
+   * first 8 slots are not written */
   if (threadIdx.x < 6) out[8 + blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
@@ -2157,11 +2158,16 @@ static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
   return 0;
 }
 
-/* A recorded count launch: kernel, grid, LDS bytes and arguments. */
+/* A recorded count launch: kernel, grid, LDS bytes and arguments, and the
+ * host copies of the strip triples and job descriptors its uploads read
+ * (asynchronous copies from pageable memory may read them after the call
+ * that issued them has returned: they live as long as the launch). */
 struct Launch {
   kfn_t fn = nullptr;
   int nblock = 0, lds = 0, ntiles = 0;
   FkArgs a;
+  std::vector<int> hstr;
+  std::vector<FkJob> hj;
 };
 
 /* Tile all jobs and prepare the persistent launch that counts (or
@@ -2304,9 +2310,11 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   int *dstrips = (int *) al.get(WS_STRIPS, sizeof(int) * (nstr ? nstr : 1));
   FkJob *djobs = nullptr;
   if (al.err) return al.err;
-  std::vector<int> hstr;
+  std::vector<int> &hstr = out->hstr;
+  hstr.clear();
   hstr.reserve(nstr);
-  std::vector<FkJob> hj(jobs.size());
+  std::vector<FkJob> &hj = out->hj;
+  hj.assign(jobs.size(), FkJob{});
   double s2f = 0;
   float win2s = 0.0f, win2p = 0.0f;
   if (m == MODE_C32)
@@ -2409,6 +2417,14 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   return 0;
 }
 
+/* Waits for the stream when it goes out of scope: declared after the
+ * launches of a call, it keeps their host copies alive until every upload
+ * has read them, on the error returns as well. */
+struct StreamDrain {
+  fcfc_gpu_ctx *c;
+  ~StreamDrain() { (void) hipStreamSynchronize(c->stream); }
+};
+
 static int launch(fcfc_gpu_ctx *c, const Launch &Lc) {
   if (Lc.ntiles > 0)
     hipLaunchKernelGGL(Lc.fn, dim3(Lc.nblock), dim3(FK_NT), Lc.lds, c->stream, Lc.a);
@@ -2421,6 +2437,7 @@ static int run_jobs(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     std::vector<PJob> &jobs, int mode, const double *G, void *dhist,
     unsigned long long *dstats, unsigned int *dqueue, float *kms) {
   Launch Lc;
+  StreamDrain drain{c};
   int e = build_launch(c, al, b, jobs, mode, G, dhist, dstats, dqueue, &Lc);
   if (e) return e;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -2521,6 +2538,7 @@ static int count_jobs(fcfc_gpu_ctx *c, int njob, const fcfc_cat *const *as,
   /* one persistent launch per group (each with its own queue word) */
   std::vector<std::vector<PJob>> groups = launch_groups(jobs);
   std::vector<Launch> L(groups.size());
+  StreamDrain drain{c};
   for (size_t q = 0; q < groups.size(); q++) {
     int e = build_launch(c, al, bins, groups[q], mode, nullptr, dh, misc + 8,
                          (unsigned int *) misc + q, &L[q]);
@@ -2628,6 +2646,7 @@ extern "C" int fcfc_gpu_count_backward(fcfc_gpu_ctx *c, const fcfc_cat *a,
   std::vector<CopyKey> copies;
   std::vector<FkStage> stages;
   std::vector<PJob> jobs(1);
+  StreamDrain drain{c};
   if (int e = setup_job(c, al, copies, pa, pb, bins, bsize, autoc, 1, MODE_GRAD, &jobs[0], stages))
     return e;
   double *dga = (double *) al.get(WS_GA, sizeof(double) * (size_t) (a->n > 0 ? a->n : 1));
@@ -2712,6 +2731,7 @@ extern "C" int fcfc_gpu_plan_create(fcfc_gpu_ctx *c, int njob, const fcfc_batch_
   P->jobs.resize(njob);
   std::vector<CopyKey> copies;
   std::vector<FkStage> stages;
+  StreamDrain drain{c};
   for (int k = 0; k < njob; k++) {
     const fcfc_batch_job &j = jobs[k];
     const int autoc = (j.flags & FCFC_GPU_AUTO) ? 1 : 0;
