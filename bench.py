@@ -335,9 +335,12 @@ def survey_workload(args):
     torch.cuda.synchronize()
     log(f"survey workload ready in {time.time() - t0:.1f} s: D={D.n} R={R.n}")
     eng = F.Engine(0)
+    if args.shape:
+        cd, zd, tl = (int(v) for v in args.shape.split(","))
+        eng.set_tuning(cell_div=cd, tile=tl, zdiv=zd)
     prec = args.precision
     others = [f for f in ("fp64", "fp32", "plain") if f != prec and f not in args.skip]
-    if not args.no_autotune:
+    if not args.no_autotune and not args.shape:
         for p in [prec] + others:
             for name, a, c, au in (("DD", D, None, True), ("DR", D, R, False), ("RR", R, None, True)):
                 log(f"autotune {p} {name}: {eng.autotune(a, c, b, isauto=au, withwt=True, precision=p)}")
@@ -501,9 +504,9 @@ def main():
                     help="start with a 1 GiB device-to-device copy (calibrates HBM byte "
                          "counters of a profiler run)")
     ap.add_argument("--shape", default=None, metavar="CD,ZD,T",
-                    help="c2 workload: fix the grid/tile shape of every count (cells per "
-                         "separation, z-cells per cell, A objects per tile) instead of "
-                         "autotuning, so that separate profiler runs see one shape")
+                    help="c2 and survey workloads: fix the grid/tile shape of every count "
+                         "(cells per separation, z-cells per cell, A objects per tile) instead "
+                         "of autotuning, so that separate runs and forms see one shape")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default grid/tile shapes (by default the autotuner picks "
                          "them for each count and form before timing; its results are cached "

@@ -144,8 +144,17 @@ __global__ void k_stage_scatter(int nobj, int nst, const FkStage *__restrict__ s
 /* Work items of the tile list: tile ti becomes S items (each a share of the
  * strip list) for ti < tt and St > S finer items from tt on, so that the
  * items the persistent workgroups take last are short and the launch ends
- * evenly; tiles from ncap on are left out (the autotuner's samples). */
-struct FkSplit { int S, tt, St, ncap; };
+ * evenly.  ncap tiles in all: of each run of G consecutive tiles of the
+ * tiling, the first run tiles (G = run = the tile count, except in the
+ * autotuner's samples: runs spread over the whole catalogue, each keeping
+ * the neighbourhood, and so the cache reuse, of a full count). */
+struct FkSplit { int S, tt, St, ncap, G, run; };
+
+/* list position of tile ti of the tiling, or -1 when it is not sampled */
+__host__ __device__ inline int split_sample(const FkSplit &s, int ti) {
+  const int g = ti / s.G, p = ti - g * s.G;
+  return p < s.run ? g * s.run + p : -1;
+}
 
 __host__ __device__ inline long long split_pos(const FkSplit &s, int ti) {
   return ti < s.tt ? (long long) s.S * ti
@@ -161,9 +170,10 @@ __device__ int row_tiles(const int *__restrict__ cs, int ncz, int T, int zmaxc,
   /* emit tile (p0, p1, iz0, iz1) as items over the strip list */
   auto emit = [&](int p0, int p1, int iz0, int iz1) {
     const int ti = t0 + nt;
-    if (out && ti < sp.ncap) {
-      const int S = ti < sp.tt ? sp.S : sp.St;
-      FkTile *o = out + split_pos(sp, ti);
+    const int si = out ? split_sample(sp, ti) : -1;
+    if (si >= 0 && si < sp.ncap) {
+      const int S = si < sp.tt ? sp.S : sp.St;
+      FkTile *o = out + split_pos(sp, si);
       for (int q = 0; q < S; q++)
         o[q] = FkTile{row, p0, p1, iz0, iz1, job,
                       (int) ((long long) nstrips * q / S),
@@ -208,14 +218,13 @@ __global__ void k_tile_count(int nrow, int njob, const FkTileJob *__restrict__ t
     const FkTileJob t = tj[fk_tile_job(tj, njob, gr)];
     const int r = gr - t.row0;
     ntile[gr] = row_tiles(t.cs + (size_t) r * t.ncz, t.ncz, t.T, t.zmaxc, r, 0, 0,
-                          FkSplit{1, 0, 1, 0}, 0, nullptr);
+                          FkSplit{1, 0, 1, 0, 1, 1}, 0, nullptr);
   }
 }
 /* the tiles of row gr are numbered from toff[gr] */
 __global__ void k_tile_emit(int nrow, int njob, const FkTileJob *__restrict__ tj,
     const int *__restrict__ toff, FkSplit sp, FkTile *__restrict__ tiles) {
   for (int gr = blockIdx.x * blockDim.x + threadIdx.x; gr < nrow; gr += gridDim.x * blockDim.x) {
-    if (toff[gr] >= sp.ncap) continue;
     const int k = fk_tile_job(tj, njob, gr);
     const FkTileJob t = tj[k];
     const int r = gr - t.row0;
@@ -481,6 +490,44 @@ __device__ __forceinline__ unsigned fk_mad_hi16(unsigned v, unsigned m, unsigned
   return r;
 }
 
+/* hi16(v) * 4 + c, c uniform: the LDS byte address of entry hi16(v) of a
+ * table of 32-bit entries at byte address c */
+__device__ __forceinline__ unsigned fk_mad4_hi16(unsigned v, unsigned c) {
+  unsigned r;
+  asm("v_mad_u32_u16 %0, %1, 4, %2 op_sel:[1,0,0,0]" : "=v"(r) : "v"(v), "s"(c));
+  return r;
+}
+
+/* lo16(a) * m + c in one instruction (the high half of a is ignored) */
+__device__ __forceinline__ unsigned fk_mad_lo16(unsigned a, unsigned m, unsigned c) {
+  unsigned r;
+  asm("v_mad_u32_u16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(m), "v"(c));
+  return r;
+}
+
+/* min / max of x and a uniform bound, one v_min_f64 / v_max_f64 (fmin and
+ * fmax would also canonicalise the bound on every call); a NaN x gives the
+ * bound */
+__device__ __forceinline__ double fk_min64(double x, double c) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(x), "s"(c));
+  return r;
+}
+__device__ __forceinline__ double fk_max64(double x, double c) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(x), "s"(c));
+  return r;
+}
+
+/* LDS byte address of a pointer into the workgroup's LDS, and a 32-bit
+ * LDS read at such an address (an ordinary load: no asm) */
+__device__ __forceinline__ unsigned fk_lds_addr(const void *p) {
+  return (unsigned) (uintptr_t) (const __attribute__((address_space(3))) char *) p;
+}
+__device__ __forceinline__ unsigned fk_lds_u32(unsigned addr) {
+  return *(const __attribute__((address_space(3))) unsigned *) (uintptr_t) addr;
+}
+
 __device__ __forceinline__ unsigned fk_mu_exact(double a, double s2, double t_sh,
     const FkBins &bn) {
   if (!(s2 >= FK_DBL_EPS)) return 0u;
@@ -518,7 +565,7 @@ template <bool CAP, bool MUT, class FA>
 __device__ __forceinline__ unsigned fk_smu_index(double anmu, double r, FA fa, double s2,
     unsigned sb, const FkBins &bn) {
   double tm = __builtin_fma(anmu, r, bn.mu_c);
-  if (CAP) tm = __builtin_fmin(tm, bn.mu_cmax);
+  if (CAP) tm = fk_min64(tm, bn.mu_cmax);
   const unsigned v = (unsigned) __builtin_bit_cast(unsigned long long, tm);
   unsigned idx = fk_mad_hi16(v, (unsigned) bn.nsh, sb);
   const bool exact = MUT ? !((v & 0xFFFFu) < (sb >> 16))
@@ -555,11 +602,15 @@ __device__ __forceinline__ unsigned fk_box64(double dx, double dy, double dz,
 /* Survey pair (src/fcfc/2pt/metric_common.c:283-460, FMA vector formulas):
  * t = 2 a.b, s = |a|^2 + |b|^2, d = |a|^2 - |b|^2, pi^2 = d^2 / (t + s),
  * s^2 = s - t, s_perp^2 = (s - t) - pi^2.  (s, mu) bins need pi^2 only in
- * the exact branch of the mu bin: sqrt(pi^2) = |d| / sqrt(t + s). */
+ * the exact branch of the mu bin: sqrt(pi^2) = |d| / sqrt(t + s).  MUT:
+ * integer tables whose cells carry a threshold in their high half ((s, mu):
+ * the mu fraction threshold; (s_perp, pi): fixed-point lookups, below);
+ * os (op) is the s (pi) table's LDS byte address for those lookups. */
 template <int BIN, bool HYB, bool PLAIN = false, bool MK = false, bool MUT = false>
 __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
     double as, double bx, double by, double bz, double bs, const FkBins &bn,
-    const unsigned *stab, const unsigned *ptab, const double *s2e, const double *pe) {
+    const unsigned *stab, const unsigned *ptab, const double *s2e, const double *pe,
+    unsigned os = 0u, unsigned op = 0u) {
   if (BIN == FCFC_BIN_ISO) {
     const double dx = ax - bx, dy = ay - by, dz = az - bz;
     const double s2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
@@ -578,6 +629,44 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
   }
   /* (s_perp, pi): pbin holds pi^2 edges; with s2min == 0 the reference has
    * no lower bound on s_perp^2 (a rounding-negative value lands in bin 0) */
+  if (MUT) {
+    /* Integer tables in the FP64 forms: no division on the fast path.
+     * With y = rcp(B) refined once (B = t + s, |y - 1/B| <= 2^-45.9 / B for
+     * the documented 2^-23 of v_rcp_f64 and B >= q_bmin), q = n y is within
+     * 2^-29.8 of the reference's correctly rounded n / B below 2^16, and
+     * s_perp^2 = (s - t) - q within 2^-29.7 of the reference's.  Each is
+     * read in fixed point as the mu bin is (fk_smu_index): x + q_c, q_c =
+     * 2^36 + 2 - h (h = 2^-15), lies in [2^36, 2^37) (clamped to it) and is
+     * rounded once onto the grid 2^-16, so the low word holds K = floor(x -
+     * h) + 2 in its high half and the fraction f in its low half.  The
+     * tables (staged by k_count, qt_entry) are indexed by K: entry K holds the bin of
+     * (int) x for x in (K - 2, K - 1) (entry 0: out, entry 1: the cell of
+     * (-1, 0)) and in its high half the largest f for which x is certainly
+     * in that interval (q_t16) when the bin changes at K - 1, 0xFFFF when
+     * it does not.  Anything else -- f above it, B below q_bmin (zero,
+     * negative, NaN) -- is binned with the reference's division. */
+    const double n = d * d, bb = t + s;
+    const double y0 = __builtin_amdgcn_rcp(bb);
+    const double q = n * __builtin_fma(y0, __builtin_fma(-bb, y0, 1.0), y0);
+    const double sp = (s - t) - q;
+    const unsigned vq = (unsigned) __builtin_bit_cast(unsigned long long,
+        fk_min64(q + bn.q_c, bn.q_capp));
+    const unsigned vs = (unsigned) __builtin_bit_cast(unsigned long long,
+        fk_min64(fk_max64(sp + bn.q_c, 68719476736.0), bn.q_caps));
+    const unsigned eq = fk_lds_u32(fk_mad4_hi16(vq, op));
+    const unsigned es = fk_lds_u32(fk_mad4_hi16(vs, os));
+    unsigned idx = fk_mad_lo16(eq, (unsigned) bn.nsh, es & 0xFFFFu);
+    const bool clear = (bb >= bn.q_bmin) & ((vq & 0xFFFFu) <= (eq >> 16)) &
+                       ((vs & 0xFFFFu) <= (es >> 16));
+    if (!clear) {
+      const double p2 = n / bb;
+      const double sp2 = (s - t) - p2;
+      const unsigned ks = min((unsigned) (int) sp2, (unsigned) (bn.soff + bn.stab_len)) + 2u;
+      const unsigned kp = min((unsigned) (int) p2, (unsigned) (bn.poff + bn.ptab_len)) + 2u;
+      idx = (stab[ks] & 0xFFFFu) + __umul24((unsigned) bn.nsh, ptab[kp] & 0xFFFFu);
+    }
+    return idx;
+  }
   const double p2 = (d * d) / (t + s);
   const double sp2 = (s - t) - p2;
   const unsigned sb = fk_sbin<HYB, MK>(sp2, bn, stab, s2e);
@@ -834,6 +923,10 @@ k_count(const FkArgs a) {
    * fk_mu32); indices below it are never counted.  MUT: the FP64 integer
    * table cells carry the mu fraction threshold (fk_smu_index) */
   constexpr bool MUT = (BIN == FCFC_BIN_SMU && !F32 && !PLAIN && !HYB);
+  /* QT: survey (s_perp, pi) in the FP64 forms with integer tables, pi^2
+   * and s_perp^2 read in fixed point (fk_sv64); the tables start at floor
+   * cell -2 and carry fraction thresholds (qt_entry below) */
+  constexpr bool QT = (GEOM == FCFC_GEOM_SURVEY && BIN == FCFC_BIN_SPI && !F32 && !PLAIN && !HYB);
   const unsigned ioff = (BIN != FCFC_BIN_SMU || PLAIN) ? 0u
                         : F32 ? 0x4301u * (unsigned) bn.nsh
                         : (unsigned) bn.nsh + (MUT ? bn.mu_t16 << 16 : 0u);
@@ -851,6 +944,7 @@ k_count(const FkArgs a) {
   unsigned *ptab_raw = (unsigned *) (lds + a.off_ptab);
   const unsigned *stab = HYB ? stab_raw + a.stab_c0 : stab_raw;
   const unsigned *ptab = HYB ? ptab_raw + a.ptab_c0 : ptab_raw;
+  const unsigned os_lds = fk_lds_addr(stab_raw), op_lds = fk_lds_addr(ptab_raw);
   /* FP32 re-evaluation queue of this wavefront: (A slot in the wavefront
    * << 8) | chunk index */
   uint16_t *wq = (uint16_t *) (lds + a.off_queue) + wave * FK_QCAP;
@@ -917,7 +1011,24 @@ k_count(const FkArgs a) {
                           : ((v & FK_F_EDGE) ? (__float_as_uint(win2) + 0xFFFFu) >> 16 : FK_F_NEVER);
       raw[i] = (th << 16) | (v & 0xFFFFu);
     };
+    /* QT tables: entry K holds the bin of (int) x for x in (K - 2, K - 1)
+     * (entry 0: out; entry 1, x in (-1, 0): cell 0) and in its high half
+     * the fraction threshold of fk_sv64's fixed-point lookup: q_t16 where
+     * the bin changes at x = K - 1, 0xFFFF where it does not; off + len + 4
+     * entries (the last one out, also for the ceilings q_caps / q_capp) */
+    auto qt_entry = [&](const uint16_t *t, int len, int off, int k) -> unsigned {
+      auto low = [&](int e) -> unsigned { return e == 0 ? FK_OUT : cell(t, len, max(e - 2, 0) - off); };
+      const unsigned v = low(k);
+      return ((v != low(k + 1) ? bn.q_t16 : 0xFFFFu) << 16) | v;
+    };
+    if (QT) {
+      for (int i = tid; i < bn.soff + bn.stab_len + 4; i += FK_NT)
+        stab_raw[i] = qt_entry(bn.stab, bn.stab_len, bn.soff, i);
+      for (int i = tid; i < bn.poff + bn.ptab_len + 4; i += FK_NT)
+        ptab_raw[i] = qt_entry(bn.ptab, bn.ptab_len, bn.poff, i);
+    }
     const double s2f = (F32 && BIN == FCFC_BIN_SMU) ? (double) a.jobs[0].win.s2_floor : 0.0;
+    if (!QT)
     for (int i = tid; i < a.stab_c0 + bn.stab_len + 2; i += FK_NT) {
       unsigned v = entry(bn.stab, bn.stab_len, bn.ns, i - a.stab_c0, bn.soff, s2f, bn.s2min);
       /* MUT (zero-based: entry i holds (int) s2 = i): the mu fraction
@@ -925,7 +1036,7 @@ k_count(const FkArgs a) {
       if (MUT && !((double) i < bn.s2_exact)) v |= bn.mu_t16 << 16;
       put(stab_raw, i, v, bn.win2s);
     }
-    if (BIN == FCFC_BIN_SPI)
+    if (BIN == FCFC_BIN_SPI && !QT)
       for (int i = tid; i < a.ptab_c0 + bn.ptab_len + 2; i += FK_NT)
         put(ptab_raw, i, entry(bn.ptab, bn.ptab_len, bn.np, i - a.ptab_c0, bn.poff, 0.0, bn.pmin),
             bn.win2p);
@@ -1167,8 +1278,9 @@ k_count(const FkArgs a) {
             idx = fk_box64<BIN, HYB, false, PLAIN, MUT>(bx - ax[r], by - ay[r], bz - az[r], bn,
                                                          stab, ptab, s2e, pe);
           else
-            idx = fk_sv64<BIN, HYB, PLAIN, false, MUT>(ax[r], ay[r], az[r], as[r], bx, by, bz,
-                                                       bs, bn, stab, ptab, s2e, pe);
+            idx = fk_sv64<BIN, HYB, PLAIN, false, MUT || QT>(ax[r], ay[r], az[r], as[r], bx, by,
+                                                             bz, bs, bn, stab, ptab, s2e, pe,
+                                                             os_lds, op_lds);
           if (!F32 && SELF && !(jj > ai[r])) idx = FK_NONE;
           if (idx < ntot_o && !queued) {
             if (GRAD) {
@@ -1340,9 +1452,11 @@ struct fcfc_gpu_ctx {
   int t_cell_div = 0, t_tile = 0, t_zdiv = 0, t_nblock = 0;
   int t_nh = 0;                /* LDS histogram copies (0: by occupancy) */
   int census = 0;
-  int tile_cap = 0;            /* autotuner: count only the first tiles */
+  int tile_frac = 0;           /* autotuner: count about 1/tile_frac of the
+                                  tiles (16 runs spread over the list) */
   /* tiles (before their split into work items) of the last launch built */
   int last_ntiles = 0;
+  int last_nsampled = 0;       /* ... and how many of them it counted */
   struct Tuned { int geom, bintype, mode, autoc, wt, bucket, cd, zd, T; };
   std::vector<Tuned> tuned;    /* autotuner results per density bucket */
   std::vector<DevBuf> bufs;
@@ -1893,10 +2007,12 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
   const int zero_based = (!hyb || mode == MODE_C32);
   L.stab_c0 = zero_based ? b->soff : 1;
   L.ptab_c0 = zero_based ? b->poff : 1;
-  L.off_stab = o; o = align16(o + (L.stab_c0 + (ssearch ? 0 : b->stab_len) + 2) * 4);
+  /* two entries after the last cell (four for the survey (s_perp, pi)
+   * tables of the FP64 forms, which start at floor cell -2) */
+  L.off_stab = o; o = align16(o + (L.stab_c0 + (ssearch ? 0 : b->stab_len) + 4) * 4);
   L.off_ptab = o;
   if (b->bintype == FCFC_BIN_SPI)
-    o = align16(o + (L.ptab_c0 + (psearch ? 0 : b->ptab_len) + 2) * 4);
+    o = align16(o + (L.ptab_c0 + (psearch ? 0 : b->ptab_len) + 4) * 4);
   L.total = o;
   return L;
 }
@@ -2278,21 +2394,30 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *) fn, FK_NT, L.total));
   if (occ < 1) occ = 1;
   const int nwg = c->t_nblock > 0 ? c->t_nblock : occ * c->ncu;
-  /* the autotuner's samples count only the first tile_cap tiles */
-  const int ncap = (c->tile_cap > 0 && c->tile_cap < ntile0) ? c->tile_cap : ntile0;
-  FkSplit sp{1, ncap, 1, ncap};
+  /* the autotuner's samples count about 1/tile_frac of the tiles, in 16
+   * runs of consecutive tiles spread evenly over the list, split into work
+   * items by the same rule as a full count (about 16 per workgroup), so
+   * that a sample ends as evenly */
+  int grp = std::max(ntile0, 1), run = grp;
+  if (c->tile_frac > 1 && ntile0 > 0) {
+    grp = (ntile0 + 15) / 16;
+    run = std::max(1, grp / c->tile_frac);
+  }
+  const int glast = ntile0 > 0 ? (ntile0 - 1) / grp : 0;
+  const int ncap = ntile0 > 0 ? glast * run + std::min(run, ntile0 - glast * grp) : 0;
+  FkSplit sp{1, ncap, 1, ncap, grp, run};
   if (ntile0 > 0) {
     int minstr = 1 << 30;
     for (auto &j : jobs) minstr = std::min(minstr, std::max(j.P.nstrips, 1));
     const int smax = std::min(minstr, 64);
-    sp.S = (int) std::min<long long>(std::max<long long>(1, (16LL * nwg + ntile0 - 1) / ntile0),
+    sp.S = (int) std::min<long long>(std::max<long long>(1, (16LL * nwg + ncap - 1) / ncap),
                                      smax);
     /* the tiles of about two items per workgroup at the end of the list
      * are split four times finer */
     sp.St = std::min(4 * sp.S, smax);
     sp.tt = (int) std::max<long long>(0, ncap - (2LL * nwg + sp.S - 1) / sp.S);
     if (sp.St <= sp.S) { sp.St = sp.S; sp.tt = ncap; }
-    if (split_pos(sp, ncap) > (1LL << 30)) sp = FkSplit{1, ncap, 1, ncap};
+    if (split_pos(sp, ncap) > (1LL << 30)) sp = FkSplit{1, ncap, 1, ncap, grp, run};
   }
   const int ntiles = (int) split_pos(sp, ncap);
   FkTile *tiles = (FkTile *) al.get(WS_TILES, sizeof(FkTile) * (size_t) (ntiles > 0 ? ntiles : 1));
@@ -2396,6 +2521,20 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     fb.mu_t16 = (unsigned) std::min<unsigned long long>(fb.mu_t16, room);
     fb.mu_pad = 0;
   }
+  {
+    /* survey (s_perp, pi) with integer tables (fk_sv64): x + q_c rounds
+     * onto the grid 2^-16 of [2^36, 2^37) with K = floor(x - h) + 2, h =
+     * 2^-15 >= 2^-16 plus the 2^-29.7 error bound of the fast pi^2 and
+     * s_perp^2; a fraction f <= q_t16 = 65532 puts x strictly inside
+     * (K - 2, K - 1) (f/2^16 + h + 2^-17 + 2^-29.7 < 1).  The ceilings are
+     * the last table entries, off + len + 3 (< 2^16: the tables fit in LDS) */
+    fb.q_c = ldexp(1.0, 36) + 2.0 - ldexp(1.0, -15);
+    fb.q_caps = ldexp(1.0, 36) + (double) (b->soff + b->stab_len + 3);
+    fb.q_capp = ldexp(1.0, 36) + (double) (b->poff + b->ptab_len + 3);
+    fb.q_bmin = ldexp(1.0, -1000);
+    fb.q_t16 = 65532u;
+    fb.q_pad = 0;
+  }
   fb.stab = dstab; fb.ptab = dptab; fb.s2bin = ds2e; fb.pbin = dpe;
   a.tiles = tiles; a.ntiles = ntiles;
   a.queue = dqueue;
@@ -2411,6 +2550,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
   int nblock = nwg;
   if (nblock > a.ntiles) nblock = std::max(a.ntiles, 1);
   c->last_ntiles = ntile0;
+  c->last_nsampled = ncap;
   c->stats.tile = jobs.empty() ? 0 : jobs[0].P.T;
   c->stats.cell_div = jobs.empty() ? 0 : (int) lround(sqrt(jobs[0].P.g.Rxy2) / jobs[0].P.g.wx);
   out->fn = fn; out->nblock = nblock; out->lds = L.total; out->ntiles = a.ntiles; out->a = a;
@@ -2832,43 +2972,59 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
   double bestms = 1e300;
   int bcd = 3, bzd = 8, bT = 0;
   int last_T = 0;
+  /* FCFC_GPU_TUNE_LOG set and not "0": every candidate's timing on stderr */
+  const char *tl = getenv("FCFC_GPU_TUNE_LOG");
+  const bool tune_log = tl && strcmp(tl, "0") != 0;
   auto measure = [&](int cd, int zd, int T, double *est) -> int {
     c->t_cell_div = cd; c->t_zdiv = zd; c->t_tile = T;
-    /* first pass: learn the number of tiles of this shape with a tiny cap */
-    c->tile_cap = 1;
+    /* first pass: learn the number of tiles of this shape, counting 16 */
+    c->tile_frac = 1 << 30;
     int e = run();
     if (e) return e;
     const int nt = c->last_ntiles;
     /* small counts (a few thousand tiles, milliseconds) are timed whole: an
      * eighth of them is too few tiles to rank shapes */
-    c->tile_cap = (nt <= 4096) ? 0 : std::max(1, nt / (nt > 16384 ? 16 : 8));
+    c->tile_frac = (nt <= 4096) ? 1 : (nt > 16384 ? 16 : 8);
     e = run();
     if (e) return e;
-    *est = c->stats.kernel_ms * (c->tile_cap ? (double) nt / (double) c->tile_cap : 1.0);
+    *est = c->stats.kernel_ms * (double) nt / (double) std::max(1, c->last_nsampled);
     last_T = c->stats.tile;
+    if (tune_log)
+      fprintf(stderr, "fcfc_gpu_autotune: cell_div %d zcell_div %d tile %d: %d of %d tiles "
+              "in %.3f ms, estimate %.3f ms\n", cd, zd, last_T, c->last_nsampled, nt,
+              c->stats.kernel_ms, *est);
     return 0;
   };
   int e = 0;
   const int cds[] = {2, 3, 4, 6};
   const int zds[] = {1, 2, 4, 8, 16};
+  struct Cand { double est; int cd, zd, T; };
+  std::vector<Cand> grid;
   for (int cd : cds) {
     for (int zd : zds) {
       double est;
       if ((e = measure(cd, zd, 0, &est))) break;
+      grid.push_back({est, cd, zd, last_T});
       if (est < bestms) { bestms = est; bcd = cd; bzd = zd; bT = last_T; }
     }
     if (e) break;
   }
-  /* the hybrid gradient kernel holds one object per lane: tiles of 2 FK_NT
+  /* the grid runs each shape at its automatic tile size (FK_NT objects for
+   * small cells); the best three shapes are also timed at the other size.
+   * The hybrid gradient kernel holds one object per lane: tiles of 2 FK_NT
    * objects would run as FK_NT, so only that size is timed */
+  std::sort(grid.begin(), grid.end(), [](const Cand &x, const Cand &y) { return x.est < y.est; });
   const int Ts[] = {FK_NT, 2 * FK_NT};
-  for (int T : Ts) {
-    double est;
-    if (T > FK_NT && grad && (bins->tabtype || !zb_fits(bins))) break;
-    if (e || (e = measure(bcd, bzd, T, &est))) break;
-    if (est < bestms) { bestms = est; bT = T; }
+  for (size_t k = 0; k < grid.size() && k < 3 && !e; k++) {
+    for (int T : Ts) {
+      double est;
+      if (T == grid[k].T) continue;
+      if (T > FK_NT && grad && (bins->tabtype || !zb_fits(bins))) continue;
+      if ((e = measure(grid[k].cd, grid[k].zd, T, &est))) break;
+      if (est < bestms) { bestms = est; bcd = grid[k].cd; bzd = grid[k].zd; bT = last_T; }
+    }
   }
-  c->tile_cap = 0;
+  c->tile_frac = 0;
   c->t_cell_div = save_cd; c->t_zdiv = save_zd; c->t_tile = save_t;
   if (e) return e;
   /* cache for the density bucket of this job, keyed as the count (or the

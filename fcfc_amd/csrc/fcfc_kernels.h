@@ -57,6 +57,10 @@ struct FkBins {
   unsigned mu_t16, mu_pad;   /* mu bin settled exactly when the fraction     */
                              /* (low 16 bits) of that value is >= mu_t16     */
   double s2_exact;           /* ... or s^2 is below this value               */
+  double q_c;                /* survey (s_perp, pi), FP64 integer tables,   */
+  double q_caps, q_capp;     /* pi^2 and s_perp^2 in fixed point (fk_sv64): */
+  double q_bmin;             /* addend, ceilings, least |a+b|^2 of the fast */
+  unsigned q_t16, q_pad;     /* path, fraction threshold of edge cells      */
   double s2skip, pskip;      /* box: a pair at s^2 >= s2skip or pi >= pskip  */
   float s2skip32, pskip32;   /* is out of range (FP64 / FP32 values)         */
   float win2s, win2p;        /* FP32 tables: window 2 e of the edge cells,   */

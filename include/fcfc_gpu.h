@@ -251,11 +251,12 @@ int fcfc_gpu_set_tuning(fcfc_gpu_ctx *ctx, int cell_div, int tile,
     int zcell_div, int nblock);
 
 /* Autotuner (no counterpart in the reference): times candidate grid / tile
- * shapes for counts like (a, b, bins, flags) on the first eighth (beyond
- * 16384 tiles: sixteenth) of their tile list (a slab of the box at the true
- * density; counts of at most 4096 tiles whole), every pair of cell_div
- * {2, 3, 4, 6} and zcell_div {1, 2, 4, 8, 16}, then tile {512, 1024}, and
- * keeps the fastest in the
+ * shapes for counts like (a, b, bins, flags) on an eighth (beyond 16384
+ * tiles: a sixteenth) of their tile list, taken as 16 runs of consecutive
+ * tiles spread over the whole list (counts of at most 4096 tiles whole):
+ * every pair of cell_div {2, 3, 4, 6} and zcell_div {1, 2, 4, 8, 16} at its
+ * automatic tile size, then the best three pairs at the other tile size of
+ * {512, 1024}, and keeps the fastest in the
  * context for this geometry, binning, precision and density bucket; later
  * counts whose shape is automatic use it.  With FCFC_GPU_GRADIENT in flags
  * it tunes fcfc_gpu_count_backward for such counts instead.  best (may be

@@ -141,6 +141,30 @@ def test_survey_near_observer(eng, bt):
         _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), ref, False, cen)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "plain"])
+def test_survey_lattice_edges(eng, prec):
+    """Survey (s_perp, pi) on an integer lattice (in rescaled units) placed
+    symmetrically about the line of sight, far from the observer: mirror
+    pairs (|a| = |b|) have pi^2 = 0 and an integer s_perp^2, pairs on the
+    axis an integer pi^2, so about 2e4 pairs sit on integers, many of them
+    bin edges (squares), where the fixed-point lookups of the FP64 form must
+    defer to the division and the FP32 windows to FP64.  Every reference
+    variant agrees here (census 0): each form equals the oracle exactly."""
+    b = _bins(F.GEOM_SURVEY, F.BIN_SPI)
+    f = b.rescale
+    g = np.arange(-6, 7) / f
+    zg = 1200.0 + np.arange(0, 13) / f
+    x, y, z = [a.ravel().copy() for a in np.meshgrid(g, g, zg, indexing="ij")]
+    d = F.Catalog(b, x, y, z)
+    assert np.all(d.x == np.round(d.x)) and np.all(d.z == np.round(d.z))
+    ref, cen = O.count(1, None, (d.x, d.y, d.z, d.s, None), None, True, _obins(b), variant=0,
+                       census_mask=0b111111)
+    assert cen == 0
+    print(f"case survey lattice spi prec={prec}", flush=True)
+    np.testing.assert_array_equal(eng.count_pairs(d, None, b, isauto=True, withwt=False,
+                                                  precision=prec), ref)
+
+
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
 def test_survey_hybrid(eng, bt):
     """Survey geometry with hybrid (non-integer) tables, every form."""
