@@ -1564,7 +1564,7 @@ extern "C" int fcfc_gpu_last_stats(const fcfc_gpu_ctx *c, fcfc_gpu_stats *st) {
 extern "C" int fcfc_gpu_set_tuning(fcfc_gpu_ctx *c, int cell_div, int tile,
     int zdiv, int nblock) {
   if (!c) return FCFC_ERR_ARG;
-  if (cell_div < 0 || cell_div > 32 || tile < 0 || tile > 2 * FK_NT || zdiv < 0 || zdiv > 64 || nblock < 0)
+  if (cell_div < 0 || cell_div > 32 || tile < 0 || tile > 4 * FK_NT || zdiv < 0 || zdiv > 64 || nblock < 0)
     return FCFC_ERR_ARG;
   c->t_cell_div = cell_div; c->t_tile = tile; c->t_zdiv = zdiv; c->t_nblock = nblock;
   return 0;
@@ -1709,7 +1709,7 @@ static int make_plan(fcfc_gpu_ctx *c, const fcfc_bins *b, const double *bsize,
   /* objects per tile: two per lane when the rows are long enough to fill
    * such tiles, one per lane otherwise (and always for the gradient) */
   P->T_auto = (tT <= 0);
-  if (tT > 0) P->T = std::min(tT, 2 * FK_NT);
+  if (tT > 0) P->T = std::min(tT, 4 * FK_NT);
   else {
     /* two objects per lane only when the z extent allowed to a tile holds
      * enough objects to fill such tiles (idle lanes cost full issue slots) */
@@ -2020,8 +2020,16 @@ static LdsPlan lds_plan(const fcfc_bins *b, int mode, int wt, int need_s, int nh
 typedef void (*kfn_t)(const FkArgs);
 
 /* Kernel variants: binning x geometry x weights x mode x table type x A
- * objects per lane (R = 1: tiles of FK_NT objects, R = 2: 2 FK_NT; hybrid
- * tables and the gradient use one R each). */
+ * objects per lane (R = 1: tiles of FK_NT objects, R = 2: 2 FK_NT, R = 4:
+ * 4 FK_NT).  Hybrid tables use one R (2; the gradient 1); R = 4 exists for
+ * the weighted FP64 counts with integer tables only, whose LDS-bound
+ * histogram updates gain from reading each staged B object for more A
+ * objects (fk_lanes_max gives the largest R of a kernel family). */
+static int fk_lanes_max(int mode, int wt, int hyb) {
+  if (hyb) return mode == MODE_GRAD ? 1 : 2;
+  return (wt && mode == MODE_C64) ? 4 : 2;
+}
+
 template <int BIN, int GEOM, bool WT, int MODE>
 static kfn_t pick5(int hyb, int R) {
   if constexpr (MODE == MODE_GRAD) {
@@ -2030,6 +2038,8 @@ static kfn_t pick5(int hyb, int R) {
   }
   else {
     if (hyb) return k_count<BIN, GEOM, WT, MODE, true, 2>;
+    if constexpr (WT && MODE == MODE_C64)
+      if (R == 4) return k_count<BIN, GEOM, WT, MODE, false, 4>;
     return R == 1 ? k_count<BIN, GEOM, WT, MODE, false, 1> : k_count<BIN, GEOM, WT, MODE, false, 2>;
   }
 }
@@ -2325,13 +2335,15 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     if (lds_plan(b, m, anywt, need_s, 1, hyb, 1, 0).total > lds_max) psearch = 1;
   }
   /* one kernel serves every job of the launch: tiles of the longest kind.
-   * A lane holds R objects, so no tile may exceed R FK_NT objects: the
-   * hybrid gradient kernel exists for R = 1 only, and a tile size set by
-   * the caller or the autotuner is clamped to what the kernel covers */
+   * A lane holds R objects (1, 2 or 4), so no tile may exceed R FK_NT
+   * objects: R is cut to what the kernel family provides (fk_lanes_max),
+   * and a tile size set by the caller or the autotuner is clamped to what
+   * the kernel covers */
   {
     int R = 1;
     for (auto &j : jobs) R = std::max(R, (j.P.T + FK_NT - 1) / FK_NT);
-    if (m == MODE_GRAD && hyb) R = 1;
+    if (R > 2) R = 4;
+    R = std::min(R, fk_lanes_max(m, anywt, hyb));
     for (auto &j : jobs) {
       j.P.R = R;
       if (j.P.T_auto || j.P.T > R * FK_NT) j.P.T = R * FK_NT;
@@ -3010,16 +3022,17 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
     if (e) break;
   }
   /* the grid runs each shape at its automatic tile size (FK_NT objects for
-   * small cells); the best three shapes are also timed at the other size.
-   * The hybrid gradient kernel holds one object per lane: tiles of 2 FK_NT
-   * objects would run as FK_NT, so only that size is timed */
+   * small cells); the best three shapes are also timed at the other sizes
+   * the kernel family holds (fk_lanes_max: 4 FK_NT for weighted FP64
+   * counts with integer tables, FK_NT for the hybrid gradient kernel) */
   std::sort(grid.begin(), grid.end(), [](const Cand &x, const Cand &y) { return x.est < y.est; });
-  const int Ts[] = {FK_NT, 2 * FK_NT};
+  const int lmax = fk_lanes_max(mode, grad || (flags & FCFC_GPU_WEIGHTED) ? 1 : 0,
+                                (bins->tabtype || !zb_fits(bins)) ? 1 : 0);
+  const int Ts[] = {FK_NT, 2 * FK_NT, 4 * FK_NT};
   for (size_t k = 0; k < grid.size() && k < 3 && !e; k++) {
     for (int T : Ts) {
       double est;
-      if (T == grid[k].T) continue;
-      if (T > FK_NT && grad && (bins->tabtype || !zb_fits(bins))) continue;
+      if (T == grid[k].T || T > lmax * FK_NT) continue;
       if ((e = measure(grid[k].cd, grid[k].zd, T, &est))) break;
       if (est < bestms) { bestms = est; bcd = grid[k].cd; bzd = grid[k].zd; bT = last_T; }
     }

@@ -246,7 +246,9 @@ int fcfc_gpu_count_backward(fcfc_gpu_ctx *ctx, const fcfc_cat *a,
 /* ---- tuning ------------------------------------------------------------- */
 /* Override the grid and launch shape (0 = automatic / autotuned):
  * cell_div = cells per maximum separation in x and y, tile = A objects per
- * tile, zcell_div = z-cells per x-cell, nblock = persistent workgroups. */
+ * tile (at most 2048; above 1024 only the weighted FP64 counts with integer
+ * tables hold such tiles, four objects per lane, other counts cut them to
+ * 1024), zcell_div = z-cells per x-cell, nblock = persistent workgroups. */
 int fcfc_gpu_set_tuning(fcfc_gpu_ctx *ctx, int cell_div, int tile,
     int zcell_div, int nblock);
 
@@ -255,8 +257,9 @@ int fcfc_gpu_set_tuning(fcfc_gpu_ctx *ctx, int cell_div, int tile,
  * tiles: a sixteenth) of their tile list, taken as 16 runs of consecutive
  * tiles spread over the whole list (counts of at most 4096 tiles whole):
  * every pair of cell_div {2, 3, 4, 6} and zcell_div {1, 2, 4, 8, 16} at its
- * automatic tile size, then the best three pairs at the other tile size of
- * {512, 1024}, and keeps the fastest in the
+ * automatic tile size, then the best three pairs at the other tile sizes of
+ * {512, 1024} (and 2048 for weighted FP64 counts with integer tables), and
+ * keeps the fastest in the
  * context for this geometry, binning, precision and density bucket; later
  * counts whose shape is automatic use it.  With FCFC_GPU_GRADIENT in flags
  * it tunes fcfc_gpu_count_backward for such counts instead.  best (may be

@@ -165,6 +165,46 @@ def test_survey_lattice_edges(eng, prec):
                                                   precision=prec), ref)
 
 
+@pytest.mark.parametrize("geom", [F.GEOM_BOX, F.GEOM_SURVEY])
+def test_four_objects_per_lane(eng, geom):
+    """Tiles of 2048 objects set by the caller: the weighted FP64 counts
+    with integer tables hold four A objects per lane, every other count
+    cuts such tiles to 1024.  Dense halos (cells of thousands of objects,
+    two cells per separation) fill the tiles; every count equals the
+    oracle."""
+    if geom == F.GEOM_BOX:
+        x, y, z, w = F.gen_clustered(81, 1000, 4, 3000, 1.0, L, 0.3)
+        xr, yr, zr = F.gen_uniform(82, 4000, L)
+        wr = np.linspace(0.7, 1.3, len(xr))
+        bts, g0, var = (F.BIN_SMU, F.BIN_SPI, F.BIN_ISO), 0, 1
+    else:
+        x, y, z, w = F.gen_survey(83, 6000)
+        xr, yr, zr, wr = F.gen_survey(84, 8000)
+        bts, g0, var = (F.BIN_SPI, F.BIN_SMU), 1, 0
+    try:
+        eng.set_tuning(cell_div=2, tile=2048, zdiv=1)
+        for bt in bts:
+            b = _bins(geom, bt)
+            d = F.Catalog(b, x, y, z, w)
+            r = F.Catalog(b, xr, yr, zr, wr)
+            ob = _obins(b)
+            bz = b.bsize_rescaled if geom == F.GEOM_BOX else None
+            print(f"case four per lane geom={geom} bt={bt}", flush=True)
+            refa, _ = O.count(g0, bz, _otuple(d), None, True, ob, variant=var, weighted=True)
+            got = eng.count_pairs(d, None, b, isauto=True, withwt=True)
+            assert eng.stats()["tile"] == 2048
+            _cmp(got, refa, True)
+            refx, _ = O.count(g0, bz, _otuple(d), _otuple(r), False, ob, variant=var, weighted=True)
+            _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True), refx, True)
+            refu, cen = O.count(g0, bz, (d.x, d.y, d.z, d.s, None), None, True, ob, variant=var,
+                                census_mask=0b111111)
+            got = eng.count_pairs(d, None, b, isauto=True, withwt=False)
+            assert eng.stats()["tile"] == 1024
+            _cmp(got, refu, False, cen)
+    finally:
+        eng.set_tuning()
+
+
 @pytest.mark.parametrize("bt", [F.BIN_ISO, F.BIN_SMU, F.BIN_SPI])
 def test_survey_hybrid(eng, bt):
     """Survey geometry with hybrid (non-integer) tables, every form."""
