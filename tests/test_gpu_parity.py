@@ -196,6 +196,10 @@ def test_four_objects_per_lane(eng, geom):
             _cmp(got, refa, True)
             refx, _ = O.count(g0, bz, _otuple(d), _otuple(r), False, ob, variant=var, weighted=True)
             _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True), refx, True)
+            for prec in ("fp32", "plain"):
+                got = eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec)
+                assert eng.stats()["tile"] == 1024, prec
+                _cmp(got, refx, True)
             refu, cen = O.count(g0, bz, (d.x, d.y, d.z, d.s, None), None, True, ob, variant=var,
                                 census_mask=0b111111)
             got = eng.count_pairs(d, None, b, isauto=True, withwt=False)
