@@ -396,6 +396,35 @@ def test_offset_tables(eng, cats, bt, smin, smax, ds, pmin, pmax, dpi, soff):
         _cmp(ga, refa, False, cena)
 
 
+@pytest.mark.parametrize("bt,smin,pmin,soff,poff", [
+    (F.BIN_SMU, 20.0, 0.0, 16, 0),
+    (F.BIN_SPI, 20.0, 30.0, 16, 36),
+])
+def test_offset_tables_survey(eng, bt, smin, pmin, soff, poff):
+    """Survey geometry with integer tables whose range starts above zero
+    (s_perp from 20, pi from 30 Mpc/h): the cells below the first edge are
+    out of range in the zero-based tables of every form, and in the
+    fixed-point (s_perp, pi) lookups of the FP64 form, whose tables start
+    at floor cell -2."""
+    x, y, z, w = F.gen_survey(61, 4000)
+    xr, yr, zr, wr = F.gen_survey(62, 6000)
+    b = F.Bins.linear(F.GEOM_SURVEY, bt, smin, 200.0, 5.0, nmu=20 if bt == F.BIN_SMU else 1,
+                      pmin=pmin, pmax=150.0 if bt == F.BIN_SPI else 0.0, dpi=10.0)
+    assert b.tabtype == 0 and int(b.s2bin[0]) == soff
+    if bt == F.BIN_SPI:
+        assert int(b.pbin[0]) == poff
+    d = F.Catalog(b, x, y, z, w)
+    r = F.Catalog(b, xr, yr, zr, wr)
+    ob = _obins(b)
+    refa, cena = O.count(1, None, (d.x, d.y, d.z, d.s, None), None, True, ob, variant=0,
+                         census_mask=0b111111)
+    refx, _ = O.count(1, None, _otuple(d), _otuple(r), False, ob, variant=0, weighted=True)
+    for prec in ("fp64", "fp32", "plain"):
+        print(f"case survey offset table bt={bt} prec={prec}", flush=True)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), refa, False, cena)
+        _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec), refx, True)
+
+
 def test_launch_shapes_identical(eng, cats):
     """Integer counts do not depend on the grid / tile / launch shape."""
     (x, y, z, w), (xr, yr, zr) = cats
