@@ -366,6 +366,28 @@ def test_mu_bins_many_and_few(eng, cats, nmu):
         _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), refa, False, cena)
 
 
+@pytest.mark.parametrize("nmu", [255, 3])
+def test_survey_mu_bins_many_and_few(eng, nmu):
+    """Survey (s, mu) with the finest and a coarse mu binning: the capped
+    fixed-point mu value of the survey form (K at most nmu + 1) at
+    nmu = 255, where the FP32 request runs the FP64 form, and at nmu = 3."""
+    x, y, z, w = F.gen_survey(63, 4000)
+    xr, yr, zr, wr = F.gen_survey(64, 6000)
+    b = _bins(F.GEOM_SURVEY, F.BIN_SMU, nmu=nmu)
+    d = F.Catalog(b, x, y, z, w)
+    r = F.Catalog(b, xr, yr, zr, wr)
+    ob = _obins(b)
+    refa, cena = O.count(1, None, (d.x, d.y, d.z, d.s, None), None, True, ob, variant=0,
+                         census_mask=0b111111)
+    refx, _ = O.count(1, None, _otuple(d), _otuple(r), False, ob, variant=0, weighted=True)
+    for prec in ("fp64", "fp32", "plain"):
+        print(f"case survey nmu={nmu} prec={prec}", flush=True)
+        _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), refa, False, cena)
+        if prec == "fp32":
+            assert (eng.stats()["rechecked"] > 0) == (nmu <= 126)
+        _cmp(eng.count_pairs(d, r, b, isauto=False, withwt=True, precision=prec), refx, True)
+
+
 @pytest.mark.parametrize("bt,smin,smax,ds,pmin,pmax,dpi,soff", [
     (F.BIN_SMU, 20.0, 200.0, 5.0, 0.0, 0.0, 1.0, 16),       # zero-based table, 16 cells below
     (F.BIN_SPI, 20.0, 200.0, 5.0, 30.0, 150.0, 10.0, 16),
