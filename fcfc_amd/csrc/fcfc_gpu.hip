@@ -144,10 +144,11 @@ __global__ void k_stage_scatter(int nobj, int nst, const FkStage *__restrict__ s
 /* Work items of the tile list: tile ti becomes S items (each a share of the
  * strip list) for ti < tt and St > S finer items from tt on, so that the
  * items the persistent workgroups take last are short and the launch ends
- * evenly.  ncap tiles in all: of each run of G consecutive tiles of the
- * tiling, the first run tiles (G = run = the tile count, except in the
- * autotuner's samples: runs spread over the whole catalogue, each keeping
- * the neighbourhood, and so the cache reuse, of a full count). */
+ * evenly.  ncap tiles in all: of each group of G consecutive tiles of the
+ * tiling, its first `run` tiles.  A full count has one group (G = run =
+ * the tile count); the autotuner's samples take runs spread over the whole
+ * catalogue, each keeping the neighbourhood, and so the cache reuse, of a
+ * full count. */
 struct FkSplit { int S, tt, St, ncap, G, run; };
 
 /* list position of tile ti of the tiling, or -1 when it is not sampled */
@@ -639,11 +640,11 @@ __device__ __forceinline__ unsigned fk_sv64(double ax, double ay, double az,
      * 2^36 + 2 - h (h = 2^-15), lies in [2^36, 2^37) (clamped to it) and is
      * rounded once onto the grid 2^-16, so the low word holds K = floor(x -
      * h) + 2 in its high half and the fraction f in its low half.  The
-     * tables (staged by k_count, qt_entry) are indexed by K: entry K holds the bin of
-     * (int) x for x in (K - 2, K - 1) (entry 0: out, entry 1: the cell of
-     * (-1, 0)) and in its high half the largest f for which x is certainly
-     * in that interval (q_t16) when the bin changes at K - 1, 0xFFFF when
-     * it does not.  Anything else -- f above it, B below q_bmin (zero,
+     * tables (staged by k_count, qt_entry) are indexed by K: entry K holds
+     * the bin of (int) x for x in (K - 2, K - 1) (entry 0: out, entry 1:
+     * the cell of (-1, 0)) and in its high half the largest f for which x
+     * is certainly in that interval (q_t16) when the bin changes at K - 1,
+     * 0xFFFF when it does not.  Anything else -- f above it, B below q_bmin (zero,
      * negative, NaN) -- is binned with the reference's division. */
     const double n = d * d, bb = t + s;
     const double y0 = __builtin_amdgcn_rcp(bb);
@@ -2285,9 +2286,9 @@ static int setup_job(fcfc_gpu_ctx *c, Alloc &al, std::vector<CopyKey> &copies,
 }
 
 /* A recorded count launch: kernel, grid, LDS bytes and arguments, and the
- * host copies of the strip triples and job descriptors its uploads read
- * (asynchronous copies from pageable memory may read them after the call
- * that issued them has returned: they live as long as the launch). */
+ * host copies of the strip triples and job descriptors its uploads read.
+ * They live as long as the launch, so that the asynchronous uploads do not
+ * depend on how the runtime stages pageable host memory. */
 struct Launch {
   kfn_t fn = nullptr;
   int nblock = 0, lds = 0, ntiles = 0;
@@ -2570,8 +2571,8 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
 }
 
 /* Waits for the stream when it goes out of scope: declared after the
- * launches of a call, it keeps their host copies alive until every upload
- * has read them, on the error returns as well. */
+ * launches of a call (destroyed before them), it keeps their host copies
+ * alive until the stream has drained, on the error returns as well. */
 struct StreamDrain {
   fcfc_gpu_ctx *c;
   ~StreamDrain() { (void) hipStreamSynchronize(c->stream); }
