@@ -95,10 +95,13 @@ __device__ __forceinline__ int fk_stage_find(const FkStage *__restrict__ st, int
 }
 
 __global__ void k_stage_assign(int nobj, int nst, const FkStage *__restrict__ st,
-    int *__restrict__ cid, int *__restrict__ rank, int *__restrict__ count) {
+    int *__restrict__ cid, int *__restrict__ rank, int *__restrict__ count, int *bad) {
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < nobj; o += gridDim.x * blockDim.x) {
     const FkStage &q = st[fk_stage_find(st, nst, o, &FkStage::obj0)];
     const int i = o - q.obj0;
+    /* a non-finite coordinate has no cell and no bin: the call fails */
+    if (!(__builtin_isfinite(q.x[i]) && __builtin_isfinite(q.y[i]) && __builtin_isfinite(q.z[i])))
+      atomicOr(bad, 1);
     const int ix = fk_cell1(q.x[i], q.g.ox, q.g.wx, q.g.ncx);
     const int iy = fk_cell1(q.y[i], q.g.oy, q.g.wy, q.g.ncy);
     const int iz = fk_cell1(q.z[i], q.g.oz, q.g.wz, q.g.ncz);
@@ -1935,24 +1938,29 @@ static int stage_run(fcfc_gpu_ctx *c, Alloc &al, std::vector<FkStage> &st) {
   FkStage *dst = (FkStage *) al.tmp(WS_STAGE, sizeof(FkStage) * st.size());
   int *cid = (int *) al.tmp(WS_CID, sizeof(int) * (size_t) (nobj > 0 ? nobj : 1));
   int *rank = (int *) al.tmp(WS_RANK, sizeof(int) * (size_t) (nobj > 0 ? nobj : 1));
-  int *count = (int *) al.tmp(WS_COUNT, sizeof(int) * (size_t) ncell);
+  /* count[ncell]: set when an object has a non-finite coordinate */
+  int *count = (int *) al.tmp(WS_COUNT, sizeof(int) * ((size_t) ncell + 1));
   int *gstart = (int *) al.tmp(WS_GSTART, sizeof(int) * ((size_t) ncell + 1));
   if (al.err) return al.err;
   HIPCHK(c, hipMemcpyAsync(dst, st.data(), sizeof(FkStage) * st.size(), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_fill_int, dim3(nblk(ncell, 256)), dim3(256), 0, c->stream, count, (int) ncell, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(ncell + 1, 256)), dim3(256), 0, c->stream, count,
+                     (int) ncell + 1, 0);
   if (nobj > 0)
     hipLaunchKernelGGL(k_stage_assign, dim3(nblk(nobj, 256)), dim3(256), 0, c->stream, (int) nobj, nst, dst,
-                       cid, rank, count);
+                       cid, rank, count, count + ncell);
   hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, c->stream, count, gstart, (int) ncell);
   hipLaunchKernelGGL(k_stage_starts, dim3(nblk(ncs, 256)), dim3(256), 0, c->stream, (int) ncs, nst, dst, gstart);
   if (nobj > 0)
     hipLaunchKernelGGL(k_stage_scatter, dim3(nblk(nobj, 256)), dim3(256), 0, c->stream, (int) nobj, nst, dst,
                        cid, rank, gstart);
   HIPCHK(c, hipGetLastError());
+  int bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, count + ncell, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   /* the descriptors are read by the kernels above: keep the host copy until
    * the copy has completed */
   HIPCHK(c, hipStreamSynchronize(c->stream));
   st.clear();
+  if (bad) { set_err(c, "a catalogue has a non-finite coordinate"); return FCFC_ERR_DATA; }
   return 0;
 }
 

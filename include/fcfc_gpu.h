@@ -160,6 +160,9 @@ const char *fcfc_gpu_last_error(const fcfc_gpu_ctx *ctx);
  * bsize: rescaled box side lengths (box geometry), NULL for survey.
  * cnt:   int64_t[ntot] (unweighted) or double[ntot] (FCFC_GPU_WEIGHTED);
  *        results are added to it.
+ * A catalogue with a NaN or infinite coordinate is rejected with
+ * FCFC_ERR_DATA (found while its grid is built, before any count runs;
+ * the reference does not check, and its tree code is undefined there).
  */
 int fcfc_gpu_count(fcfc_gpu_ctx *ctx, const fcfc_cat *a, const fcfc_cat *b,
     const fcfc_bins *bins, const double *bsize, uint32_t flags, void *cnt);

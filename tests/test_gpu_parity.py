@@ -166,6 +166,28 @@ def test_survey_lattice_edges(eng, prec):
 
 
 @pytest.mark.parametrize("geom", [F.GEOM_BOX, F.GEOM_SURVEY])
+def test_non_finite_coordinates_rejected(eng, cats, geom):
+    """A catalogue with a NaN or infinite coordinate has no cell and no bin:
+    the grid build flags it and the count fails with FCFC_ERR_DATA before
+    any count kernel runs; the engine then counts valid catalogues as
+    before."""
+    (x, y, z, w), _ = cats
+    b = _bins(geom, F.BIN_SMU)
+    if geom == F.GEOM_SURVEY:
+        x, y, z, w = F.gen_survey(65, 3000)
+    good = F.Catalog(b, x, y, z, w)
+    ref = eng.count_pairs(good, None, b, isauto=True, withwt=False)
+    for bad_value in (np.nan, np.inf):
+        xb = x.copy()
+        xb[len(xb) // 2] = bad_value
+        print(f"case non-finite geom={geom} value={bad_value}", flush=True)
+        with pytest.raises(F.FcfcError, match="non-finite") as ei:
+            eng.count_pairs(F.Catalog(b, xb, y, z, w), None, b, isauto=True, withwt=False)
+        assert ei.value.code == -21
+        np.testing.assert_array_equal(eng.count_pairs(good, None, b, isauto=True, withwt=False), ref)
+
+
+@pytest.mark.parametrize("geom", [F.GEOM_BOX, F.GEOM_SURVEY])
 def test_four_objects_per_lane(eng, geom):
     """Tiles of 2048 objects set by the caller: the weighted FP64 counts
     with integer tables hold four A objects per lane, every other count
