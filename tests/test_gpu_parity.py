@@ -317,8 +317,7 @@ def test_coincident_points(eng, geom):
         ref, cen = O.count(g0, b.bsize_rescaled if geom == F.GEOM_BOX else None,
                            (d.x, d.y, d.z, d.s, None), None, True, ob, variant=var,
                            census_mask=0b111111)
-        precs = ("fp64", "fp32", "plain") if geom == F.GEOM_BOX else ("fp64", "plain")
-        for prec in precs:
+        for prec in ("fp64", "fp32", "plain"):
             print(f"case coincident geom={geom} bt={bt} prec={prec}", flush=True)
             _cmp(eng.count_pairs(d, None, b, isauto=True, withwt=False, precision=prec), ref, False, cen)
 

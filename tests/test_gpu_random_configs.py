@@ -84,7 +84,7 @@ def test_random_config(eng, seed, lfrac=0.4):
     ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu, b.pbin if bt == F.BIN_SPI else None)
     og, bs, var, cmask = ((0, b.bsize_rescaled, 1, 0b11111) if geom == F.GEOM_BOX
                           else (1, None, 0, 0b111111))
-    forms = ("fp64", "fp32", "plain") if geom == F.GEOM_BOX else ("fp64", "plain")
+    forms = ("fp64", "fp32", "plain")
     tA = (A.x, A.y, A.z, A.s, None)
     tB = (B.x, B.y, B.z, B.s, None)
     wA = (A.x, A.y, A.z, A.s, A.w)
