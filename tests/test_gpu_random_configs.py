@@ -225,3 +225,65 @@ def test_random_large(eng, seed):
                 np.testing.assert_allclose(g, r, rtol=1e-6, atol=1e-9 * np.abs(r).max())
     finally:
         eng.set_tuning(0, 0, 0, 0)
+
+
+def _large_survey_config(seed):
+    """A larger survey configuration: binning type, integer or hybrid edges,
+    catalogue sizes and a random grid, tile and workgroup shape."""
+    rng = np.random.default_rng(7000 + seed)
+    bt = (F.BIN_ISO, F.BIN_SMU, F.BIN_SPI)[seed % 3]
+    if seed % 2 == 0:                 # integer-friendly edges; odd seeds: hybrid tables
+        ds = float(rng.choice([2.0, 4.0, 5.0]))
+        smin = float(rng.choice([0.0, ds]))
+    else:
+        ds = float(np.round(rng.uniform(1.3, 6.0), 2))
+        smin = float(np.round(rng.uniform(0.0, 2.0 * ds), 2))
+    ns = max(1, int((min(smin + int(rng.integers(5, 41)) * ds, 150.0) - smin) // ds))
+    nmu = int(rng.choice([5, 40, 100])) if bt == F.BIN_SMU else 1
+    dpi, pmax = (5.0, 5.0 * int(rng.integers(4, 21))) if bt == F.BIN_SPI else (1.0, 0.0)
+    b = F.Bins.linear(F.GEOM_SURVEY, bt, smin, smin + ns * ds, ds, nmu=nmu, pmin=0.0,
+                      pmax=pmax, dpi=dpi)
+    x, y, z, w = F.gen_survey(int(rng.integers(1 << 30)), int(rng.integers(30000, 80000)))
+    A = F.Catalog(b, x, y, z, w)
+    x, y, z, w = F.gen_survey(int(rng.integers(1 << 30)), int(rng.integers(30000, 100000)))
+    B = F.Catalog(b, x, y, z, w)
+    shape = (int(rng.choice([2, 3, 4, 6])), int(rng.choice([256, 512, 1024, 2048])),
+             int(rng.choice([1, 4, 16])), int(rng.choice([0, 7, 64, 300])))
+    return bt, b, A, B, shape
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_large_survey(eng, seed):
+    """Larger survey catalogues under random grid, tile (up to four objects
+    per lane for weighted FP64 counts) and workgroup shapes, every form,
+    auto and cross counts, against the oracle."""
+    bt, b, A, B, (cd, tile, zd, nb) = _large_survey_config(seed)
+    ob = O.oracle_bins(b.bintype, b.s2bin, b.nmu, b.pbin if bt == F.BIN_SPI else None)
+    print(f"case random large survey seed={seed} bt={bt} tab={b.tabtype} ns={b.ns} "
+          f"nmu={b.nmu} nA={A.n} nB={B.n} shape={(cd, tile, zd, nb)}", flush=True)
+    tA = (A.x, A.y, A.z, A.s, None)
+    tB = (B.x, B.y, B.z, B.s, None)
+    ref_a, cen_a = O.count(1, None, tA, None, True, ob, variant=0, census_mask=0b111111)
+    ref_x, cen_x = O.count(1, None, tA, tB, False, ob, variant=0, census_mask=0b111111)
+    wA = (A.x, A.y, A.z, A.s, A.w)
+    wB = (B.x, B.y, B.z, B.s, B.w)
+    ref_aw, _ = O.count(1, None, wA, None, True, ob, variant=0, weighted=True)
+    ref_xw, _ = O.count(1, None, wA, wB, False, ob, variant=0, weighted=True)
+
+    def close(g, r):
+        scale = np.maximum(np.abs(r), 1e-300)
+        assert np.max(np.abs(g - r) / scale) <= WT_RTOL
+
+    try:
+        eng.set_tuning(cd, tile, zd, nb)
+        for prec in ("fp64", "fp32", "plain"):
+            for g, r, cen in ((eng.count_pairs(A, None, b, isauto=True, precision=prec),
+                               ref_a, cen_a),
+                              (eng.count_pairs(A, B, b, precision=prec), ref_x, cen_x)):
+                assert np.abs(g.astype(np.int64) - r.astype(np.int64)).sum() <= 2 * cen
+                if cen == 0:
+                    np.testing.assert_array_equal(g, r)
+            close(eng.count_pairs(A, None, b, isauto=True, withwt=True, precision=prec), ref_aw)
+            close(eng.count_pairs(A, B, b, withwt=True, precision=prec), ref_xw)
+    finally:
+        eng.set_tuning(0, 0, 0, 0)
