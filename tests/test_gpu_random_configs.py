@@ -113,18 +113,21 @@ def test_random_config(eng, seed, lfrac=0.4):
         same(eng.count_pairs(B, A, b, precision=prec), ref_x, cen_x)
         close(eng.count_pairs(A, None, b, isauto=True, withwt=True, precision=prec), ref_aw)
         close(eng.count_pairs(A, B, b, withwt=True, precision=prec), ref_xw)
-    # one batched launch of mixed jobs
-    ba, bx, bxw = eng.count_batch([(A, None, True, False), (A, B, False, False),
-                                   (A, B, False, True)], b, precision="fp64")
-    same(ba, ref_a, cen_a)
-    same(bx, ref_x, cen_x)
-    close(bxw, ref_xw)
-    # a prepared plan on device copies of the catalogues, replayed twice
+    # one batched launch of mixed jobs, in every form
+    for prec in forms:
+        ba, bx, bxw = eng.count_batch([(A, None, True, False), (A, B, False, False),
+                                       (A, B, False, True)], b, precision=prec)
+        same(ba, ref_a, cen_a)
+        same(bx, ref_x, cen_x)
+        close(bxw, ref_xw)
+    # a prepared plan on device copies of the catalogues, replayed twice, in
+    # one form per seed
     import torch
     dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
     dA = F.Catalog(b, dev(A.x), dev(A.y), dev(A.z), dev(A.w))
     dB = F.Catalog(b, dev(B.x), dev(B.y), dev(B.z), dev(B.w))
-    plan = eng.plan([(dA, None, True, False), (dA, dB, False, True)], b)
+    plan = eng.plan([(dA, None, True, False), (dA, dB, False, True)], b,
+                    precision=forms[seed % 3])
     try:
         for _ in range(2):
             pa, pxw = plan.run()
