@@ -275,6 +275,24 @@ struct FkWin {
   float m2_floor;             /* survey: pairs with |a + b|^2 below this    */
 };                            /* are re-evaluated                          */
 
+/* A value that is the same in every lane (read from the job or tile held in
+ * LDS), moved to a scalar register: the compiler cannot tell that it is
+ * uniform and would otherwise keep it in a vector register. */
+__device__ __forceinline__ float fk_uni(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ FkWin fk_uni(const FkWin &w) {
+  FkWin u;
+  u.e_s2 = fk_uni(w.e_s2); u.e_s2x2 = fk_uni(w.e_s2x2);
+  u.e_pi = fk_uni(w.e_pi); u.e_pix2 = fk_uni(w.e_pix2);
+  u.idle_x = fk_uni(w.idle_x); u.idle_z = fk_uni(w.idle_z);
+  u.mu_sh = fk_uni(w.mu_sh); u.mu_c = fk_uni(w.mu_c);
+  u.mu_t16 = (unsigned) __builtin_amdgcn_readfirstlane((int) w.mu_t16);
+  u.s2_floor = fk_uni(w.s2_floor); u.nmu_f = fk_uni(w.nmu_f);
+  u.m2_floor = fk_uni(w.m2_floor);
+  return u;
+}
+
 /* Flags of the integer cells of the FP32 tables, set while they are staged
  * (the LDS cells then carry a float threshold in place of the flags). */
 #define FK_F_EDGE 0x10000u    /* a bin edge at the lower end of the integer cell */
@@ -920,7 +938,9 @@ k_count(const FkArgs a) {
   constexpr bool NEED_W = WT || GRAD;
   constexpr bool NEED_C = F32;
   const int tid = threadIdx.x;
-  const int wave = tid >> 6;
+  /* the wavefront index, in a scalar register (with everything derived
+   * from it: the wavefront's histogram copy and queue) */
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const FkBins &bn = a.bins;
   const unsigned ntot = (unsigned) bn.ntoh;   /* LDS histogram slots */
   /* the pair functions' (s, mu) indices carry an offset (fk_mu_exact,
@@ -1143,7 +1163,10 @@ k_count(const FkArgs a) {
     /* survey FP32: m = a + b = 2 a' + d + 2 c (a' = a - c, c = tile centre).
      * Idle lanes (a' = (10^6, 0, 0): s^2 and pi^2 far out of range) may
      * look ambiguous in mu: their queue bits are masked off */
-    const float c2x = (float) (2.0 * tcx), c2y = (float) (2.0 * tcy), c2z = (float) (2.0 * tcz);
+    const float c2x = fk_uni((float) (2.0 * tcx)), c2y = fk_uni((float) (2.0 * tcy)),
+                c2z = fk_uni((float) (2.0 * tcz));
+    /* the FP32 windows of the job, in scalar registers for the pair loop */
+    const FkWin wn = F32 ? fk_uni(J.win) : J.win;
     unsigned long long actm[R];
 #pragma unroll
     for (int r = 0; r < R; r++) actm[r] = __builtin_amdgcn_ballot_w64(t.p0 + tid * R + r < t.p1);
@@ -1249,11 +1272,11 @@ k_count(const FkArgs a) {
               const float ex = gx - fax[r], ey = gy - fay[r], ez = gz - faz[r];
               idx = fk_sv32<BIN>(ex, ey, ez, __builtin_fmaf(2.0f, fax[r], ex) + c2x,
                                  __builtin_fmaf(2.0f, fay[r], ey) + c2y,
-                                 __builtin_fmaf(2.0f, faz[r], ez) + c2z, bn, J.win, stab_raw,
+                                 __builtin_fmaf(2.0f, faz[r], ez) + c2z, bn, wn, stab_raw,
                                  ptab_raw, amb, ambm, ambx);
             }
             else
-              idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, J.win, stab_raw,
+              idx = fk_box32<BIN>(gx - fax[r], gy - fay[r], gz - faz[r], bn, wn, stab_raw,
                                   ptab_raw, amb, ambm);
             if (SELF) {
               const bool keep = jj > ai[r];
