@@ -2054,12 +2054,14 @@ typedef void (*kfn_t)(const FkArgs);
 /* Kernel variants: binning x geometry x weights x mode x table type x A
  * objects per lane (R = 1: tiles of FK_NT objects, R = 2: 2 FK_NT, R = 4:
  * 4 FK_NT).  Hybrid tables use one R (2; the gradient 1); R = 4 exists for
- * the weighted FP64 counts with integer tables only, whose LDS-bound
+ * the weighted FP64 box counts with integer tables only, whose LDS-bound
  * histogram updates gain from reading each staged B object for more A
- * objects (fk_lanes_max gives the largest R of a kernel family). */
-static int fk_lanes_max(int mode, int wt, int hyb) {
+ * objects (fk_lanes_max gives the largest R of a kernel family).  The
+ * survey kernels hold four more FP64 values per object and spill at R = 4
+ * (weighted (s_perp, pi) DR: 950 ms against 830 at R = 2). */
+static int fk_lanes_max(int geom, int mode, int wt, int hyb) {
   if (hyb) return mode == MODE_GRAD ? 1 : 2;
-  return (wt && mode == MODE_C64) ? 4 : 2;
+  return (wt && mode == MODE_C64 && geom == FCFC_GEOM_BOX) ? 4 : 2;
 }
 
 template <int BIN, int GEOM, bool WT, int MODE>
@@ -2070,7 +2072,7 @@ static kfn_t pick5(int hyb, int R) {
   }
   else {
     if (hyb) return k_count<BIN, GEOM, WT, MODE, true, 2>;
-    if constexpr (WT && MODE == MODE_C64)
+    if constexpr (WT && MODE == MODE_C64 && GEOM == FCFC_GEOM_BOX)
       if (R == 4) return k_count<BIN, GEOM, WT, MODE, false, 4>;
     return R == 1 ? k_count<BIN, GEOM, WT, MODE, false, 1> : k_count<BIN, GEOM, WT, MODE, false, 2>;
   }
@@ -2375,7 +2377,7 @@ static int build_launch(fcfc_gpu_ctx *c, Alloc &al, const fcfc_bins *b,
     int R = 1;
     for (auto &j : jobs) R = std::max(R, (j.P.T + FK_NT - 1) / FK_NT);
     if (R > 2) R = 4;
-    R = std::min(R, fk_lanes_max(m, anywt, hyb));
+    R = std::min(R, fk_lanes_max(geom, m, anywt, hyb));
     for (auto &j : jobs) {
       j.P.R = R;
       if (j.P.T_auto || j.P.T > R * FK_NT) j.P.T = R * FK_NT;
@@ -3055,10 +3057,10 @@ extern "C" int fcfc_gpu_autotune(fcfc_gpu_ctx *c, const fcfc_cat *a,
   }
   /* the grid runs each shape at its automatic tile size (FK_NT objects for
    * small cells); the best three shapes are also timed at the other sizes
-   * the kernel family holds (fk_lanes_max: 4 FK_NT for weighted FP64
+   * the kernel family holds (fk_lanes_max: 4 FK_NT for weighted FP64 box
    * counts with integer tables, FK_NT for the hybrid gradient kernel) */
   std::sort(grid.begin(), grid.end(), [](const Cand &x, const Cand &y) { return x.est < y.est; });
-  const int lmax = fk_lanes_max(mode, grad || (flags & FCFC_GPU_WEIGHTED) ? 1 : 0,
+  const int lmax = fk_lanes_max(bins->geometry, mode, grad || (flags & FCFC_GPU_WEIGHTED) ? 1 : 0,
                                 (bins->tabtype || !zb_fits(bins)) ? 1 : 0);
   const int Ts[] = {FK_NT, 2 * FK_NT, 4 * FK_NT};
   for (size_t k = 0; k < grid.size() && k < 3 && !e; k++) {

@@ -249,9 +249,9 @@ int fcfc_gpu_count_backward(fcfc_gpu_ctx *ctx, const fcfc_cat *a,
 /* ---- tuning ------------------------------------------------------------- */
 /* Override the grid and launch shape (0 = automatic / autotuned):
  * cell_div = cells per maximum separation in x and y, tile = A objects per
- * tile (at most 2048; above 1024 only the weighted FP64 counts with integer
- * tables hold such tiles, four objects per lane, other counts cut them to
- * 1024), zcell_div = z-cells per x-cell, nblock = persistent workgroups. */
+ * tile (at most 2048; above 1024 only the weighted FP64 box counts with
+ * integer tables hold such tiles, four objects per lane, other counts cut
+ * them to 1024), zcell_div = z-cells per x-cell, nblock = persistent workgroups. */
 int fcfc_gpu_set_tuning(fcfc_gpu_ctx *ctx, int cell_div, int tile,
     int zcell_div, int nblock);
 
@@ -261,7 +261,7 @@ int fcfc_gpu_set_tuning(fcfc_gpu_ctx *ctx, int cell_div, int tile,
  * tiles spread over the whole list (counts of at most 4096 tiles whole):
  * every pair of cell_div {2, 3, 4, 6} and zcell_div {1, 2, 4, 8, 16} at its
  * automatic tile size, then the best three pairs at the other tile sizes of
- * {512, 1024} (and 2048 for weighted FP64 counts with integer tables), and
+ * {512, 1024} (and 2048 for weighted FP64 box counts with integer tables), and
  * keeps the fastest in the
  * context for this geometry, binning, precision and density bucket; later
  * counts whose shape is automatic use it.  With FCFC_GPU_GRADIENT in flags
